@@ -1,0 +1,155 @@
+/*
+ * yell_amd.h -- C ABI of the MI355X (gfx950) negacyclic NTT / RNS
+ * polynomial-arithmetic engine.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns
+ * YA_OK (0) or a positive YA_ERR_* code; ya_last_error() gives the detail of
+ * the calling thread's last failure.  No exception crosses this boundary.
+ *
+ * Layout.  A polynomial with L residues of degree N is residue-major: residue
+ * r occupies words [r*N, (r+1)*N) of one device buffer and is reduced modulo
+ * the prime P[r] (the reference's poly<N> keeps residue r in its own buffer,
+ * include/yell/poly.hpp:46-47, and uses P[r] for it, poly.hpp:117).  Device
+ * buffers come from ya_malloc (or any hipMalloc'd memory).  `stream` is a
+ * hipStream_t passed as void*; NULL is the default stream.  Calls taking a
+ * stream are asynchronous with respect to the host.
+ *
+ * Value ranges follow the reference exactly: forward / backward / mul / add /
+ * sub / add_product_of return canonical residues in [0, p); forward_lazy
+ * returns the reference's exact representative in [0, 4p); negate maps 0 to p.
+ * Preconditions (as the reference, lib/ntt.cpp:45-46,66-68, ops.hpp:64,79):
+ * forward inputs < 4p, backward inputs < 2p, add/sub inputs < p.
+ *
+ * Supported sizes: degree N = 2^k with 3 <= k <= 17; nmoduli L >= 1 with
+ * (first prime index + L) <= ya_num_moduli() (1000).
+ */
+#ifndef YELL_AMD_H
+#define YELL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  YA_OK = 0,
+  YA_ERR_INVALID = 1,      /* bad size, degree, modulus index or pointer */
+  YA_ERR_NO_DEVICE = 2,    /* no HIP device / HIP runtime failure at init */
+  YA_ERR_ALLOC = 3,        /* device or host allocation failed */
+  YA_ERR_LAUNCH = 4,       /* kernel launch or HIP call failed */
+  YA_ERR_UNSUPPORTED = 5,  /* valid request this build does not handle */
+  YA_ERR_REJECTED = 6,     /* server admission refused (too large / closed) */
+  YA_ERR_TIMEOUT = 7,      /* wait timed out */
+  YA_ERR_MISMATCH = 8      /* caller tables do not match the resident ones */
+};
+
+/* ---- status / device --------------------------------------------------- */
+const char *ya_status_string(int status);
+const char *ya_last_error(void);
+/* Selects the HIP device for this thread's later calls and creates its state. */
+int ya_init(int device);
+int ya_device_count(int *count);
+int ya_device_synchronize(void);
+
+/* ---- constants (include/yell/params_64.hpp:19-34) ----------------------- */
+/* params::kMaxNbModuli */
+size_t ya_num_moduli(void);
+/* params::P[cm], params::Pn[cm], params::primitive_roots[cm]; 0 if cm is out of range. */
+uint64_t ya_prime(size_t cm);
+uint64_t ya_barrett_const(size_t cm);
+uint64_t ya_root(size_t cm);
+
+/* ---- NTT tables (include/yell/ntt.hpp:50-56, details/ntt.hpp:27-104) ---- */
+/* ntt<N>::init_ntt_tables(K): builds and makes device-resident the tables of
+ * (degree, cm) for cm < count.  Thread-safe; each table is built once. */
+int ya_ntt_init_tables(size_t degree, size_t count);
+/* ntt<N>::ntt_precomputed::init(cm) on the host, into caller arrays of
+ * `degree` words each; scalars[4] = {invDegree, shoupInvDegree,
+ * invphiInvDegree, shoupInvphiInvDegree}.  Touches no device. */
+int ya_ntt_table_host(size_t degree, size_t cm, uint64_t *phi, uint64_t *phi_shoup,
+                      uint64_t *invphi, uint64_t *invphi_shoup, uint64_t *scalars);
+/* Copies the device-resident table of (degree, cm) back to the host (same
+ * arrays as ya_ntt_table_host), building it first if needed. */
+int ya_ntt_table_download(size_t degree, size_t cm, uint64_t *phi, uint64_t *phi_shoup,
+                          uint64_t *invphi, uint64_t *invphi_shoup, uint64_t *scalars);
+
+/* ---- device memory / streams / events (plumbing) ------------------------ */
+/* Replaces yell::mem_alloc / mem_free (lib/mem.cpp:14-24) for device buffers;
+ * allocations are 256-byte aligned. */
+int ya_malloc(void **ptr, size_t bytes);
+int ya_free(void *ptr);
+int ya_memset_zero(void *ptr, size_t bytes, void *stream);
+int ya_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int ya_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int ya_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int ya_host_alloc_pinned(void **ptr, size_t bytes);
+int ya_host_free_pinned(void *ptr);
+int ya_stream_create(void **stream);
+int ya_stream_destroy(void *stream);
+int ya_stream_synchronize(void *stream);
+int ya_event_create(void **event);
+int ya_event_destroy(void *event);
+int ya_event_record(void *event, void *stream);
+int ya_event_synchronize(void *event);
+int ya_event_elapsed_ms(float *ms, void *start, void *end);
+
+/* ---- one residue (include/yell/details/ntt.hpp:106-166) ----------------- */
+/* ntt<N>::forward(op, cm) / forward_lazy(op, cm) / backward(op, cm), in place
+ * on `degree` device words, modulo P[cm]. */
+int ya_ntt_forward(uint64_t *x, size_t degree, size_t cm, void *stream);
+int ya_ntt_forward_lazy(uint64_t *x, size_t degree, size_t cm, void *stream);
+int ya_ntt_backward(uint64_t *x, size_t degree, size_t cm, void *stream);
+
+/* The out-of-line seam itself (lib/ntt.cpp:89, :151; declared at
+ * include/yell/details/ntt.hpp:10-23) on HOST memory, synchronous:
+ * negacylic_forward_lazy / negacylic_backward_lazy(x, degree, w, wshoup, p).
+ * p must be one of P[cm]; w / wshoup must be that (degree, cm)'s phi (resp.
+ * invphi) table and companions (checked on a sample of words, else
+ * YA_ERR_MISMATCH).  The device uses its own resident copy of the tables. */
+int ya_negacylic_forward_lazy(uint64_t *x, size_t degree, const uint64_t *w,
+                              const uint64_t *wshoup, uint64_t p);
+int ya_negacylic_backward_lazy(uint64_t *x, size_t degree, const uint64_t *w,
+                               const uint64_t *wshoup, uint64_t p);
+
+/* ---- residue-major polynomials (include/yell/details/poly.hpp) ---------- */
+/* poly<N>::forward / forward_lazy / backward (details/poly.hpp:171-186):
+ * residue r uses P[r], r < nmoduli. */
+int ya_poly_forward(uint64_t *poly, size_t degree, size_t nmoduli, void *stream);
+int ya_poly_forward_lazy(uint64_t *poly, size_t degree, size_t nmoduli, void *stream);
+int ya_poly_backward(uint64_t *poly, size_t degree, size_t nmoduli, void *stream);
+/* operator* / + / - and *= += -= (include/yell/ops.hpp:7-41, poly.hpp:163-168):
+ * out[r][i] = op(a[r][i], b[r][i]).  out may alias a (the in-place forms). */
+int ya_poly_mul(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree,
+                size_t nmoduli, void *stream);
+int ya_poly_add(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree,
+                size_t nmoduli, void *stream);
+int ya_poly_sub(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree,
+                size_t nmoduli, void *stream);
+/* poly<N>::negate (details/poly.hpp:151-158): v -> p - v (0 -> p). */
+int ya_poly_negate(uint64_t *x, size_t degree, size_t nmoduli, void *stream);
+/* poly<N>::add_product_of (details/poly.hpp:119-132): acc = acc + a*b mod p. */
+int ya_poly_add_product_of(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t degree,
+                           size_t nmoduli, void *stream);
+
+/* One RNS poly-multiplication, fused: c = backward(forward(a) * forward(b)),
+ * the call sequence a.forward(); b.forward(); c = a * b; c.backward() of the
+ * reference (details/poly.hpp:171-186, ops.hpp:7-24) without modifying a, b.
+ * c must not alias a or b. */
+int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                     size_t nmoduli, void *stream);
+
+/* ---- host samplers (input source; details/poly.hpp:201-260) ------------- */
+/* Seeded replacements of poly::set(uniform) and poly::set(gaussian): fill a
+ * HOST residue-major buffer.  uniform: per residue, 64-bit word & (2^62-1),
+ * minus p once if >= p.  gaussian: one centred discrete Gaussian sample per
+ * coefficient (parameter sigma: density proportional to exp(-x^2/(2 sigma^2))),
+ * shared by all residues and mapped r < 0 ? p + r : r. */
+int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed);
+int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YELL_AMD_H */

@@ -1,0 +1,142 @@
+// ref_shim.cpp -- extern "C" entry points over the reference's own sources.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled (by oracle/Makefile) together with the
+// reference's lib/ntt.cpp, lib/ops.cpp, lib/math.cpp and lib/params.cpp, read
+// in place under the reference checkout, into oracle/_ref/libyellref.so.  It
+// drives ntt<N>::forward / forward_lazy / backward and the ops:: functors
+// directly on caller buffers (the poly<N> container is bypassed because its
+// samplers are unseeded and its header pulls in GMP/MPFR).  Used to pin the
+// C restatement in yell_oracle.c and as the CPU baseline of bench.py.
+#include "yell/ntt.hpp"
+#include "yell/ops.hpp"
+
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+using T = yell::params::value_type;
+
+namespace {
+
+// poly-mult on caller arrays, the call sequence of details/poly.hpp:171-186
+// and ops.hpp:7-24: forward(a), forward(b), rop = a * b, rop.backward().
+template <size_t N>
+void polymul_n(T *c, const T *a, const T *b, size_t L, T *wa, T *wb) {
+  yell::ops::mulmod mulmod;
+  std::memcpy(wa, a, sizeof(T) * N * L);
+  std::memcpy(wb, b, sizeof(T) * N * L);
+  for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::forward(wa + cm * N, cm);
+  for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::forward(wb + cm * N, cm);
+  for (size_t cm = 0; cm < L; ++cm) {
+    T *dst = c + cm * N;
+    const T *x = wa + cm * N, *y = wb + cm * N;
+    for (size_t i = 0; i < N; ++i) dst[i] = mulmod(x[i], y[i], cm);
+  }
+  for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::backward(c + cm * N, cm);
+}
+
+template <size_t N>
+struct Ops {
+  static void fwd(T *x, size_t cm) { yell::ntt<N>::forward(x, cm); }
+  static void fwd_lazy(T *x, size_t cm) { yell::ntt<N>::forward_lazy(x, cm); }
+  static void bwd(T *x, size_t cm) { yell::ntt<N>::backward(x, cm); }
+  static void tables(size_t cm, T *phi, T *phis, T *iphi, T *iphis, T *sc) {
+    auto t = yell::ntt<N>::init_table(cm);
+    std::memcpy(phi, t->phiTbl, sizeof(T) * N);
+    std::memcpy(phis, t->shoupPhiTbl, sizeof(T) * N);
+    std::memcpy(iphi, t->invphiTbl, sizeof(T) * N);
+    std::memcpy(iphis, t->shoupInvphiTbl, sizeof(T) * N);
+    sc[0] = t->invDegree;
+    sc[1] = t->shoupInvDegree;
+    sc[2] = t->invphiInvDegree;
+    sc[3] = t->shoupInvphiInvDegree;
+  }
+  static void init(size_t k) { yell::ntt<N>::init_ntt_tables(k); }
+  static void polymul(T *c, const T *a, const T *b, size_t L) {
+    std::vector<T> wa(N * L), wb(N * L);
+    polymul_n<N>(c, a, b, L, wa.data(), wb.data());
+  }
+  // `reps` poly-mults on each of `threads` std::threads, each thread on its
+  // own copy of the operands; read-only shared tables (built beforehand).
+  // Returns the wall-clock seconds of the timed region.
+  static double bench(const T *a, const T *b, size_t L, int threads, int reps) {
+    init(L);
+    std::vector<std::thread> pool;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int th = 0; th < threads; ++th) {
+      pool.emplace_back([=]() {
+        std::vector<T> wa(N * L), wb(N * L), c(N * L);
+        for (int r = 0; r < reps; ++r) polymul_n<N>(c.data(), a, b, L, wa.data(), wb.data());
+      });
+    }
+    for (auto &t : pool) t.join();
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+  }
+};
+
+#define YREF_SWITCH(logn, EXPR)                                                   \
+  switch (logn) {                                                                 \
+    case 2: { using O = Ops<(1u << 2)>; EXPR; } break;                            \
+    case 3: { using O = Ops<(1u << 3)>; EXPR; } break;                            \
+    case 4: { using O = Ops<(1u << 4)>; EXPR; } break;                            \
+    case 5: { using O = Ops<(1u << 5)>; EXPR; } break;                            \
+    case 6: { using O = Ops<(1u << 6)>; EXPR; } break;                            \
+    case 7: { using O = Ops<(1u << 7)>; EXPR; } break;                            \
+    case 8: { using O = Ops<(1u << 8)>; EXPR; } break;                            \
+    case 9: { using O = Ops<(1u << 9)>; EXPR; } break;                            \
+    case 10: { using O = Ops<(1u << 10)>; EXPR; } break;                          \
+    case 11: { using O = Ops<(1u << 11)>; EXPR; } break;                          \
+    case 12: { using O = Ops<(1u << 12)>; EXPR; } break;                          \
+    case 13: { using O = Ops<(1u << 13)>; EXPR; } break;                          \
+    case 14: { using O = Ops<(1u << 14)>; EXPR; } break;                          \
+    case 15: { using O = Ops<(1u << 15)>; EXPR; } break;                          \
+    case 16: { using O = Ops<(1u << 16)>; EXPR; } break;                          \
+    case 17: { using O = Ops<(1u << 17)>; EXPR; } break;                          \
+    default: return -1;                                                           \
+  }
+
+}  // namespace
+
+extern "C" {
+
+int yref_forward(int logn, T *x, size_t cm) { YREF_SWITCH(logn, O::fwd(x, cm)); return 0; }
+int yref_forward_lazy(int logn, T *x, size_t cm) { YREF_SWITCH(logn, O::fwd_lazy(x, cm)); return 0; }
+int yref_backward(int logn, T *x, size_t cm) { YREF_SWITCH(logn, O::bwd(x, cm)); return 0; }
+int yref_tables(int logn, size_t cm, T *phi, T *phis, T *iphi, T *iphis, T *sc) {
+  YREF_SWITCH(logn, O::tables(cm, phi, phis, iphi, iphis, sc));
+  return 0;
+}
+int yref_init_tables(int logn, size_t k) { YREF_SWITCH(logn, O::init(k)); return 0; }
+int yref_polymul(int logn, T *c, const T *a, const T *b, size_t L) {
+  YREF_SWITCH(logn, O::polymul(c, a, b, L));
+  return 0;
+}
+double yref_bench_polymul(int logn, const T *a, const T *b, size_t L, int threads, int reps) {
+  double s = -1.0;
+  switch (logn) {
+    case 12: s = Ops<(1u << 12)>::bench(a, b, L, threads, reps); break;
+    case 13: s = Ops<(1u << 13)>::bench(a, b, L, threads, reps); break;
+    case 14: s = Ops<(1u << 14)>::bench(a, b, L, threads, reps); break;
+    case 15: s = Ops<(1u << 15)>::bench(a, b, L, threads, reps); break;
+    case 16: s = Ops<(1u << 16)>::bench(a, b, L, threads, reps); break;
+    default: break;
+  }
+  return s;
+}
+
+T yref_mulmod(T x, T y, size_t cm) { return yell::ops::mulmod{}(x, y, cm); }
+T yref_mulmod_shoup(T x, T y, T yp, size_t cm) { return yell::ops::mulmod_shoup{}(x, y, yp, cm); }
+T yref_addmod(T x, T y, size_t cm) { return yell::ops::addmod{}(x, y, cm); }
+T yref_submod(T x, T y, size_t cm) { return yell::ops::submod{}(x, y, cm); }
+T yref_muladd(T rop, T x, T y, size_t cm) { return yell::ops::muladd{}(rop, x, y, cm); }
+T yref_shoupify(T x, size_t cm) { return yell::ops::shoupify(x, cm); }
+T yref_barrett(T hi, T lo, size_t cm) {
+  yell::params::gt_value_type v = ((yell::params::gt_value_type)hi << 64) | lo;
+  yell::ops::barret_reduction(&v, cm);
+  return (T)v;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+"""ctypes access to the CPU checker (oracle/): the C restatement, the GMP
+schoolbook and, where it was built, the reference compiled from its own
+sources (oracle/_ref).  Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+ORACLE_GMP = os.path.join(ROOT, "oracle", "lib", "liboracle_gmp.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libyellref.so")
+
+u64, sz, vp, i32 = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _bind(lib, sigs):
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    return lib
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(ORACLE):
+            raise RuntimeError("oracle not built: make -C oracle")
+        self.lib = _bind(ctypes.CDLL(ORACLE), {
+            "or_prime": (u64, [sz]), "or_barrett_const": (u64, [sz]), "or_root": (u64, [sz]),
+            "or_barrett_hl": (u64, [u64, u64, sz]), "or_shoupify": (u64, [u64, sz]),
+            "or_mulmod": (u64, [u64, u64, sz]), "or_mulmod_shoup": (u64, [u64, u64, u64, sz]),
+            "or_addmod": (u64, [u64, u64, sz]), "or_submod": (u64, [u64, u64, sz]),
+            "or_muladd": (u64, [u64, u64, u64, sz]),
+            "or_negacyclic_forward_lazy": (i32, [vp, sz, vp, vp, u64]),
+            "or_negacyclic_backward_lazy": (i32, [vp, sz, vp, vp, u64]),
+            "or_tables_init": (i32, [sz, sz, vp, vp, vp, vp, vp]),
+            "or_forward": (i32, [vp, sz, sz]), "or_forward_lazy": (i32, [vp, sz, sz]),
+            "or_backward": (i32, [vp, sz, sz]),
+            "or_poly_forward": (i32, [vp, sz, sz]), "or_poly_forward_lazy": (i32, [vp, sz, sz]),
+            "or_poly_backward": (i32, [vp, sz, sz]),
+            "or_poly_mul": (None, [vp, vp, vp, sz, sz]), "or_poly_add": (None, [vp, vp, vp, sz, sz]),
+            "or_poly_sub": (None, [vp, vp, vp, sz, sz]), "or_poly_negate": (None, [vp, sz, sz]),
+            "or_poly_add_product_of": (None, [vp, vp, vp, sz, sz]),
+            "or_polymul": (i32, [vp, vp, vp, sz, sz]),
+            "or_schoolbook": (None, [vp, vp, vp, sz, sz]),
+        })
+
+    def prime(self, cm):
+        return int(self.lib.or_prime(cm))
+
+    def tables(self, n, cm):
+        arrs = [np.empty(n, dtype=np.uint64) for _ in range(4)]
+        sc = np.empty(4, dtype=np.uint64)
+        assert self.lib.or_tables_init(n, cm, *[_p(a) for a in arrs], _p(sc)) == 0
+        return arrs + [sc]
+
+    def _inplace(self, fn, x, *args):
+        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        assert getattr(self.lib, fn)(_p(y), *args) == 0
+        return y
+
+    def forward(self, x, n, cm):
+        return self._inplace("or_forward", x, n, cm)
+
+    def forward_lazy(self, x, n, cm):
+        return self._inplace("or_forward_lazy", x, n, cm)
+
+    def backward(self, x, n, cm):
+        return self._inplace("or_backward", x, n, cm)
+
+    def poly_forward(self, x, n, L):
+        return self._inplace("or_poly_forward", x, n, L)
+
+    def poly_forward_lazy(self, x, n, L):
+        return self._inplace("or_poly_forward_lazy", x, n, L)
+
+    def poly_backward(self, x, n, L):
+        return self._inplace("or_poly_backward", x, n, L)
+
+    def negacyclic_forward_lazy(self, x, n, w, ws, p):
+        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        assert self.lib.or_negacyclic_forward_lazy(_p(y), n, _p(w), _p(ws), p) == 0
+        return y
+
+    def negacyclic_backward_lazy(self, x, n, w, ws, p):
+        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        assert self.lib.or_negacyclic_backward_lazy(_p(y), n, _p(w), _p(ws), p) == 0
+        return y
+
+    def _binary(self, fn, a, b, n, L):
+        out = np.empty(n * L, dtype=np.uint64)
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        getattr(self.lib, fn)(_p(out), _p(a), _p(b), n, L)
+        return out
+
+    def poly_mul(self, a, b, n, L):
+        return self._binary("or_poly_mul", a, b, n, L)
+
+    def poly_add(self, a, b, n, L):
+        return self._binary("or_poly_add", a, b, n, L)
+
+    def poly_sub(self, a, b, n, L):
+        return self._binary("or_poly_sub", a, b, n, L)
+
+    def poly_negate(self, a, n, L):
+        return self._inplace("or_poly_negate", a, n, L)
+
+    def poly_add_product_of(self, acc, a, b, n, L):
+        y = np.ascontiguousarray(acc, dtype=np.uint64).copy()
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        self.lib.or_poly_add_product_of(_p(y), _p(a), _p(b), n, L)
+        return y
+
+    def polymul(self, a, b, n, L):
+        out = np.empty(n * L, dtype=np.uint64)
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        assert self.lib.or_polymul(_p(out), _p(a), _p(b), n, L) == 0
+        return out
+
+    def schoolbook(self, a, b, n, cm):
+        out = np.empty(n, dtype=np.uint64)
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        self.lib.or_schoolbook(_p(out), _p(a), _p(b), n, cm)
+        return out
+
+
+class GmpOracle:
+    def __init__(self):
+        if not os.path.exists(ORACLE_GMP):
+            raise RuntimeError("GMP oracle not built (gmp.h absent)")
+        self.lib = _bind(ctypes.CDLL(ORACLE_GMP), {
+            "or_gmp_schoolbook": (i32, [vp, vp, vp, sz, u64])})
+
+    def schoolbook(self, a, b, n, p):
+        out = np.empty(n, dtype=np.uint64)
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        assert self.lib.or_gmp_schoolbook(_p(out), _p(a), _p(b), n, p) == 0
+        return out
+
+
+class RefLib:
+    """The reference's own lib/{ntt,ops,math,params}.cpp compiled in oracle/_ref."""
+
+    def __init__(self):
+        if not os.path.exists(REF):
+            raise RuntimeError("oracle/_ref not built (reference checkout absent)")
+        self.lib = _bind(ctypes.CDLL(REF), {
+            "yref_forward": (i32, [i32, vp, sz]), "yref_forward_lazy": (i32, [i32, vp, sz]),
+            "yref_backward": (i32, [i32, vp, sz]),
+            "yref_tables": (i32, [i32, sz, vp, vp, vp, vp, vp]),
+            "yref_init_tables": (i32, [i32, sz]),
+            "yref_polymul": (i32, [i32, vp, vp, vp, sz]),
+            "yref_bench_polymul": (ctypes.c_double, [i32, vp, vp, sz, i32, i32]),
+            "yref_mulmod": (u64, [u64, u64, sz]), "yref_mulmod_shoup": (u64, [u64, u64, u64, sz]),
+            "yref_addmod": (u64, [u64, u64, sz]), "yref_submod": (u64, [u64, u64, sz]),
+            "yref_muladd": (u64, [u64, u64, u64, sz]), "yref_shoupify": (u64, [u64, sz]),
+            "yref_barrett": (u64, [u64, u64, sz]),
+        })
+
+    def _inplace(self, fn, x, n, cm):
+        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        assert getattr(self.lib, fn)(n.bit_length() - 1, _p(y), cm) == 0
+        return y
+
+    def forward(self, x, n, cm):
+        return self._inplace("yref_forward", x, n, cm)
+
+    def forward_lazy(self, x, n, cm):
+        return self._inplace("yref_forward_lazy", x, n, cm)
+
+    def backward(self, x, n, cm):
+        return self._inplace("yref_backward", x, n, cm)
+
+    def tables(self, n, cm):
+        arrs = [np.empty(n, dtype=np.uint64) for _ in range(4)]
+        sc = np.empty(4, dtype=np.uint64)
+        assert self.lib.yref_tables(n.bit_length() - 1, cm, *[_p(a) for a in arrs], _p(sc)) == 0
+        return arrs + [sc]
+
+    def polymul(self, a, b, n, L):
+        out = np.empty(n * L, dtype=np.uint64)
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        assert self.lib.yref_polymul(n.bit_length() - 1, _p(out), _p(a), _p(b), L) == 0
+        return out
+
+
+def rand_residues(rng, n, L, primes, bound_mult=1):
+    """Seeded words in [0, bound_mult * P[r]) per residue r (numpy Generator)."""
+    out = np.empty(n * L, dtype=np.uint64)
+    for r in range(L):
+        hi = int(primes[r]) * bound_mult
+        out[r * n:(r + 1) * n] = rng.integers(0, hi, size=n, dtype=np.uint64)
+    return out

@@ -1,0 +1,196 @@
+"""ctypes binding of include/yell_amd.h (the engine's C ABI).
+
+Loads yell_amd/lib/libyell_amd.so built in-tree; raises ImportError-style
+RuntimeError when it is missing (there is no fallback path).  Every binding
+keeps the C signature: device buffers are integers (device addresses), host
+buffers are numpy uint64 arrays.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libyell_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "yell_amd.h")
+
+u64 = ctypes.c_uint64
+sz = ctypes.c_size_t
+vp = ctypes.c_void_p
+i32 = ctypes.c_int
+u64p = ctypes.POINTER(ctypes.c_uint64)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "ya_status_string": (ctypes.c_char_p, [i32]),
+    "ya_last_error": (ctypes.c_char_p, []),
+    "ya_init": (i32, [i32]),
+    "ya_device_count": (i32, [ctypes.POINTER(i32)]),
+    "ya_device_synchronize": (i32, []),
+    "ya_num_moduli": (sz, []),
+    "ya_prime": (u64, [sz]),
+    "ya_barrett_const": (u64, [sz]),
+    "ya_root": (u64, [sz]),
+    "ya_ntt_init_tables": (i32, [sz, sz]),
+    "ya_ntt_table_host": (i32, [sz, sz, vp, vp, vp, vp, vp]),
+    "ya_ntt_table_download": (i32, [sz, sz, vp, vp, vp, vp, vp]),
+    "ya_malloc": (i32, [ctypes.POINTER(vp), sz]),
+    "ya_free": (i32, [vp]),
+    "ya_memset_zero": (i32, [vp, sz, vp]),
+    "ya_memcpy_htod": (i32, [vp, vp, sz, vp]),
+    "ya_memcpy_dtoh": (i32, [vp, vp, sz, vp]),
+    "ya_memcpy_dtod": (i32, [vp, vp, sz, vp]),
+    "ya_host_alloc_pinned": (i32, [ctypes.POINTER(vp), sz]),
+    "ya_host_free_pinned": (i32, [vp]),
+    "ya_stream_create": (i32, [ctypes.POINTER(vp)]),
+    "ya_stream_destroy": (i32, [vp]),
+    "ya_stream_synchronize": (i32, [vp]),
+    "ya_event_create": (i32, [ctypes.POINTER(vp)]),
+    "ya_event_destroy": (i32, [vp]),
+    "ya_event_record": (i32, [vp, vp]),
+    "ya_event_synchronize": (i32, [vp]),
+    "ya_event_elapsed_ms": (i32, [ctypes.POINTER(ctypes.c_float), vp, vp]),
+    "ya_ntt_forward": (i32, [vp, sz, sz, vp]),
+    "ya_ntt_forward_lazy": (i32, [vp, sz, sz, vp]),
+    "ya_ntt_backward": (i32, [vp, sz, sz, vp]),
+    "ya_negacylic_forward_lazy": (i32, [vp, sz, vp, vp, u64]),
+    "ya_negacylic_backward_lazy": (i32, [vp, sz, vp, vp, u64]),
+    "ya_poly_forward": (i32, [vp, sz, sz, vp]),
+    "ya_poly_forward_lazy": (i32, [vp, sz, sz, vp]),
+    "ya_poly_backward": (i32, [vp, sz, sz, vp]),
+    "ya_poly_mul": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya_poly_add": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya_poly_sub": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya_poly_negate": (i32, [vp, sz, sz, vp]),
+    "ya_poly_add_product_of": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya_poly_multiply": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya_sample_uniform": (i32, [vp, sz, sz, u64]),
+    "ya_sample_gaussian": (i32, [vp, sz, sz, ctypes.c_double, u64]),
+}
+
+
+class YaError(RuntimeError):
+    def __init__(self, code, fn, detail):
+        super().__init__("%s failed: %s (%d): %s" % (fn, STATUS.get(code, "?"), code, detail))
+        self.code = code
+
+
+STATUS = {0: "YA_OK", 1: "YA_ERR_INVALID", 2: "YA_ERR_NO_DEVICE", 3: "YA_ERR_ALLOC",
+          4: "YA_ERR_LAUNCH", 5: "YA_ERR_UNSUPPORTED", 6: "YA_ERR_REJECTED", 7: "YA_ERR_TIMEOUT",
+          8: "YA_ERR_MISMATCH"}
+
+_lib = None
+
+
+def declared_symbols(header=HEADER):
+    """Function names declared in include/*.h (for the export test)."""
+    names = set()
+    for h in ([header] if isinstance(header, str) else header):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(ya_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def load(path=LIB_PATH):
+    """Load the HIP engine.  Raises RuntimeError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libyell_amd.so not built at %s (run python3 yell_amd/build.py)" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, fn="call"):
+    if rc != 0:
+        detail = _lib.ya_last_error().decode() if _lib is not None else ""
+        raise YaError(rc, fn, detail)
+    return rc
+
+
+def call(name, *args):
+    lib = load()
+    return check(getattr(lib, name)(*args), name)
+
+
+def ptr(a):
+    """Host numpy array -> void* for ctypes."""
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceBuffer:
+    """A device allocation of uint64 words (freed on close / garbage collection)."""
+
+    def __init__(self, nwords):
+        self.nwords = int(nwords)
+        p = ctypes.c_void_p()
+        call("ya_malloc", ctypes.byref(p), self.nwords * 8)
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_host(cls, arr, stream=None):
+        arr = np.ascontiguousarray(arr, dtype=np.uint64)
+        b = cls(arr.size)
+        b.upload(arr, stream)
+        return b
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr, dtype=np.uint64)
+        assert arr.size <= self.nwords
+        call("ya_memcpy_htod", self.ptr, ptr(arr), arr.size * 8, stream)
+
+    def download(self, nwords=None, stream=None):
+        n = self.nwords if nwords is None else nwords
+        out = np.empty(n, dtype=np.uint64)
+        call("ya_memcpy_dtoh", ptr(out), self.ptr, n * 8, stream)
+        return out
+
+    def close(self):
+        if self.ptr and _lib is not None:
+            _lib.ya_free(self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def primes(n):
+    lib = load()
+    return np.array([lib.ya_prime(i) for i in range(n)], dtype=np.uint64)
+
+
+def sample_uniform(degree, nmoduli, seed):
+    out = np.empty(degree * nmoduli, dtype=np.uint64)
+    call("ya_sample_uniform", ptr(out), degree, nmoduli, seed)
+    return out
+
+
+def sample_gaussian(degree, nmoduli, sigma, seed):
+    out = np.empty(degree * nmoduli, dtype=np.uint64)
+    call("ya_sample_gaussian", ptr(out), degree, nmoduli, sigma, seed)
+    return out
+
+
+def table_host(degree, cm):
+    arrs = [np.empty(degree, dtype=np.uint64) for _ in range(4)]
+    sc = np.empty(4, dtype=np.uint64)
+    call("ya_ntt_table_host", degree, cm, *[ptr(a) for a in arrs], ptr(sc))
+    return arrs + [sc]
+
+
+def table_download(degree, cm):
+    arrs = [np.empty(degree, dtype=np.uint64) for _ in range(4)]
+    sc = np.empty(4, dtype=np.uint64)
+    call("ya_ntt_table_download", degree, cm, *[ptr(a) for a in arrs], ptr(sc))
+    return arrs + [sc]

@@ -1,0 +1,95 @@
+#!/usr/bin/env python3
+"""Generate the 62-bit NTT-friendly prime table used by every kernel.
+
+Rule (reproduces the constants of the reference, include/yell/params_64.hpp:19-34):
+  * P[i]  : the i-th largest prime of the form 2^62 - k*2^21 + 1 (k = 1, 2, ...),
+            so every P[i] - 1 is divisible by 2^21 = 2 * kMaxPolyDegree;
+  * Pn[i] : low 64 bits of floor(2^128 / P[i])   (Barrett constant, params_64.hpp:29-31);
+  * R[i]  : g^((P[i]-1)/2^21) mod P[i], g the smallest generator of Z/P[i]Z
+            (a primitive 2^21-th root of unity, params_64.hpp:33-34).
+
+Writes params_table.inc (C initialiser lists) next to this script.
+Usage: python3 gen_params.py [count]
+"""
+import os
+import sys
+
+LOG_MAX_DEGREE = 20              # kMaxPolyDegree = 2^20
+TWO_N_MAX = 1 << (LOG_MAX_DEGREE + 1)
+
+
+def is_prime(n):
+    """Deterministic Miller-Rabin for n < 3.3e24 (bases: first 13 primes)."""
+    if n < 2:
+        return False
+    small = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41)
+    for q in small:
+        if n % q == 0:
+            return n == q
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in small:
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def prime_factors(n):
+    out, f = set(), 2
+    while f * f <= n:
+        while n % f == 0:
+            out.add(f)
+            n //= f
+        f += 1
+    if n > 1:
+        out.add(n)
+    return out
+
+
+def smallest_generator(p):
+    qs = prime_factors(p - 1)
+    g = 2
+    while True:
+        if all(pow(g, (p - 1) // q, p) != 1 for q in qs):
+            return g
+        g += 1
+
+
+def generate(count):
+    primes, k = [], 1
+    while len(primes) < count:
+        p = (1 << 62) - k * TWO_N_MAX + 1
+        if is_prime(p):
+            primes.append(p)
+        k += 1
+    pn = [((1 << 128) // p) & ((1 << 64) - 1) for p in primes]
+    roots = [pow(smallest_generator(p), (p - 1) // TWO_N_MAX, p) for p in primes]
+    return primes, pn, roots
+
+
+def main():
+    count = int(sys.argv[1]) if len(sys.argv) > 1 else 1000
+    primes, pn, roots = generate(count)
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "params_table.inc"), "w") as f:
+        f.write("// Generated by gen_params.py: %d primes 2^62 - k*2^21 + 1, descending.\n" % count)
+        f.write("// Columns: prime, low64(floor(2^128/prime)), primitive 2^21-th root.\n")
+        f.write("#define YA_NUM_MODULI %d\n" % count)
+        for name, vals in (("YA_TABLE_P", primes), ("YA_TABLE_PN", pn), ("YA_TABLE_ROOT", roots)):
+            f.write("#define %s { \\\n" % name)
+            for i in range(0, len(vals), 4):
+                f.write("  " + ", ".join("0x%016xULL" % v for v in vals[i:i + 4]) + ", \\\n")
+            f.write("}\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+// ya_blocks.hpp -- workgroup-level NTT building blocks shared by the kernel
+// translation units: a forward transform of one block of 2^LB words ending
+// in registers, and an inverse transform starting from registers.
+//
+// Block kernels transform 2^LB <= 2^14 consecutive words of a residue in one
+// workgroup: T = 2^(LB-LE) threads hold E = 2^LE words each, run LE layers in
+// registers per pass and exchange words through LDS between passes.  For
+// N > 2^14 a column ("top") kernel runs the layers on index bits [14, logn)
+// first (forward) or last (inverse), so every block still fits one
+// workgroup's LDS.
+#pragma once
+
+#include <type_traits>
+
+#include "ya_kernels.hpp"
+
+namespace ya {
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward transform of index bits [0, LB) of one block, words loaded from
+// global `src` (block base).  On return x holds the words of window [0, LE)
+// for thread t (i = e | t << LE), in [0, 4p).
+template <int LB, int LE>
+__device__ __forceinline__ void fwd_block(uint64_t (&x)[1 << LE], const uint64_t *__restrict__ src,
+                                          uint64_t *lds, const Tw *__restrict__ tw, uint32_t nlog,
+                                          uint32_t boff, uint64_t p, uint32_t t) {
+  constexpr int NP = (LB + LE - 1) / LE;
+  static_for<0, NP>([&](auto PC) {
+    constexpr int P = decltype(PC)::value;
+    constexpr int HI = LB - P * LE;
+    constexpr int LO = HI - LE > 0 ? HI - LE : 0;
+    constexpr int WLO = LO;
+    if constexpr (P == 0) {
+      gmem_load<WLO, LE>(src, x, t);
+    } else {
+      lds_load<WLO, LE>(lds, x, t);
+    }
+    const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
+    fwd_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, p);
+    if constexpr (P + 1 < NP) {
+      __syncthreads();
+      lds_store<WLO, LE>(lds, x, t);
+      __syncthreads();
+    }
+  });
+}
+
+// Inverse transform of index bits [0, LB) of one block starting from words
+// in registers (window [0, LE), i = e | t << LE, each < 2p), written to
+// global `dst` (block base).  TOP selects what happens at bit LB-1:
+//   TOP_GS      plain GS layer (block of a larger residue), outputs in [0, 2p);
+//   TOP_MERGED  the residue's top bit: merged N^-1 layer, canonical outputs;
+//   TOP_SKIP    the residue's top bit left undone (negacylic_backward_lazy,
+//               lib/ntt.cpp:192), outputs in [0, 2p).
+enum { TOP_GS = 0, TOP_MERGED = 1, TOP_SKIP = 2 };
+
+template <int LB, int LE, int TOP>
+__device__ __forceinline__ void inv_block(uint64_t (&x)[1 << LE], uint64_t *__restrict__ dst,
+                                          uint64_t *lds, const Tw *__restrict__ tw, uint32_t nlog,
+                                          uint32_t boff, const ModDesc &d, uint32_t t) {
+  constexpr int NP = (LB + LE - 1) / LE;
+  static_for<0, NP>([&](auto QC) {
+    constexpr int Q = decltype(QC)::value;
+    constexpr int LO = Q * LE;
+    constexpr int HI = LO + LE < LB ? LO + LE : LB;
+    constexpr int WLO = LO + LE <= LB ? LO : LB - LE;
+    constexpr bool LASTQ = (Q + 1 == NP);
+    if constexpr (Q > 0) lds_load<WLO, LE>(lds, x, t);
+    const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
+    if constexpr (LASTQ && TOP == TOP_MERGED) {
+      inv_layers<WLO, LE, HI - 1, LO>(x, tw, nlog, gbase, d.p);
+      inv_last_layer<LE, LB - 1 - WLO>(x, d);
+    } else if constexpr (LASTQ && TOP == TOP_SKIP) {
+      inv_layers<WLO, LE, HI - 1, LO>(x, tw, nlog, gbase, d.p);
+    } else {
+      inv_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, d.p);
+    }
+    if constexpr (!LASTQ) {
+      __syncthreads();
+      lds_store<WLO, LE>(lds, x, t);
+      __syncthreads();
+    } else {
+      gmem_store<WLO, LE>(dst, x, t);
+    }
+  });
+}
+
+// Runtime (lb, le) -> template instance.  Instantiated shapes: le = 3 for
+// lb = 3, le = 4 for lb in [4, 14], and le = 3 as the alternative for lb in
+// [10, 13]; any other request falls back to le = 4 (clamp_le).
+#define YA_LB_SWITCH(lb, le, CALL)                         \
+  switch ((lb) * 8 + (le)) {                               \
+    case 3 * 8 + 3: return CALL(3, 3);                     \
+    case 4 * 8 + 4: return CALL(4, 4);                     \
+    case 5 * 8 + 4: return CALL(5, 4);                     \
+    case 6 * 8 + 4: return CALL(6, 4);                     \
+    case 7 * 8 + 4: return CALL(7, 4);                     \
+    case 8 * 8 + 4: return CALL(8, 4);                     \
+    case 9 * 8 + 4: return CALL(9, 4);                     \
+    case 10 * 8 + 3: return CALL(10, 3);                   \
+    case 10 * 8 + 4: return CALL(10, 4);                   \
+    case 11 * 8 + 3: return CALL(11, 3);                   \
+    case 11 * 8 + 4: return CALL(11, 4);                   \
+    case 12 * 8 + 3: return CALL(12, 3);                   \
+    case 12 * 8 + 4: return CALL(12, 4);                   \
+    case 13 * 8 + 3: return CALL(13, 3);                   \
+    case 13 * 8 + 4: return CALL(13, 4);                   \
+    case 14 * 8 + 4: return CALL(14, 4);                   \
+    default: return hipErrorInvalidValue;                  \
+  }
+
+inline int clamp_le(int lb, int le) {
+  if (lb <= 3) return 3;
+  if (le == 3 && lb >= 10 && lb <= 13) return 3;
+  return 4;
+}
+
+}  // namespace ya

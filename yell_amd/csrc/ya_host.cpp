@@ -1,0 +1,611 @@
+// ya_host.cpp -- host runtime of the engine: constant tables, the
+// device-resident NTT table cache, memory/stream plumbing and the C ABI of
+// include/yell_amd.h for single polynomials.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/yell_amd.h"
+#include "params_table.inc"
+#include "ya_internal.hpp"
+#include "ya_kernels.hpp"
+
+namespace ya {
+
+static const uint64_t kP[YA_NUM_MODULI] = YA_TABLE_P;
+static const uint64_t kPn[YA_NUM_MODULI] = YA_TABLE_PN;
+static const uint64_t kRoot[YA_NUM_MODULI] = YA_TABLE_ROOT;
+
+static thread_local std::string t_err;
+static thread_local int t_device = -1;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(YA_ERR_LAUNCH, "%s: %s", what, hipGetErrorString(e));
+}
+
+uint64_t prime(size_t cm) { return kP[cm]; }
+
+// ---------------------------------------------------------------------------
+// Table construction (include/yell/details/ntt.hpp:27-77) with exact 128-bit
+// arithmetic: every entry is a canonical residue or floor(w * 2^64 / p), so
+// any exact evaluation reproduces the reference's words.
+using u128 = unsigned __int128;
+
+static inline uint64_t mulmod_exact(uint64_t a, uint64_t b, uint64_t p) {
+  return (uint64_t)(((u128)a * b) % p);
+}
+static inline uint64_t shoupify(uint64_t w, uint64_t p) {
+  return (uint64_t)(((u128)w << 64) / p);
+}
+static uint64_t powmod(uint64_t a, uint64_t e, uint64_t p) {
+  uint64_t r = 1;
+  while (e) {
+    if (e & 1) r = mulmod_exact(r, a, p);
+    a = mulmod_exact(a, a, p);
+    e >>= 1;
+  }
+  return r;
+}
+static inline uint32_t bitrev(uint32_t v, int bits) {
+  return bits == 0 ? 0u : (__builtin_bitreverse32(v) >> (32 - bits));
+}
+
+struct HostTable {
+  std::vector<uint64_t> phi, phi_s, iphi, iphi_s;
+  uint64_t sc[4];
+};
+
+static void build_table_host(int logn, size_t cm, HostTable &t) {
+  const size_t n = (size_t)1 << logn;
+  const uint64_t p = kP[cm];
+  uint64_t phi = kRoot[cm];
+  for (int i = 0; i < 20 - logn; ++i) phi = mulmod_exact(phi, phi, p);  // :39-43
+  t.phi.resize(n);
+  t.phi_s.resize(n);
+  t.iphi.resize(n);
+  t.iphi_s.resize(n);
+  std::vector<uint64_t> pw(n), ipw(n);
+  uint64_t v = 1;
+  for (size_t i = 0; i < n; ++i) {  // :47-53
+    pw[i] = v;
+    v = mulmod_exact(v, phi, p);
+  }
+  const uint64_t invphi = mulmod_exact(v, pw[n - 1], p);  // :58, phi^N * phi^(N-1)
+  v = 1;
+  for (size_t i = 0; i < n; ++i) {  // :60-65
+    ipw[i] = v;
+    v = mulmod_exact(v, invphi, p);
+  }
+  for (size_t i = 0; i < n; ++i) {  // :68-71 (bit reversal)
+    const size_t j = bitrev((uint32_t)i, logn);
+    t.phi[j] = pw[i];
+    t.iphi[j] = ipw[i];
+  }
+  for (size_t i = 0; i < n; ++i) {
+    t.phi_s[i] = shoupify(t.phi[i], p);
+    t.iphi_s[i] = shoupify(t.iphi[i], p);
+  }
+  t.sc[0] = powmod((uint64_t)n % p, p - 2, p);              // :73
+  t.sc[2] = mulmod_exact(t.sc[0], n >= 2 ? t.iphi[1] : 1, p);  // :74
+  t.sc[1] = shoupify(t.sc[0], p);
+  t.sc[3] = shoupify(t.sc[2], p);
+}
+
+// ---------------------------------------------------------------------------
+// Per-device state: the resident tables and descriptor array.
+struct Slot {
+  std::atomic<int> ready{0};
+  Tw *dev = nullptr;           // [fwd N pairs | inv N pairs]
+  uint64_t check[8] = {0};     // phi[1], phi_s[1], phi[N-1], phi_s[N-1], inv likewise
+};
+
+struct DeviceState {
+  int device = 0;
+  hipStream_t tstream = nullptr;
+  ModDesc *d_desc = nullptr;  // [(kMaxLogN + 1) * YA_NUM_MODULI]
+  std::mutex mu;
+  std::unique_ptr<Slot[]> slots;  // same indexing as d_desc
+};
+
+static std::mutex g_dev_mu;
+static std::vector<std::unique_ptr<DeviceState>> g_devs;
+
+static inline size_t slot_index(int logn, size_t cm) { return (size_t)logn * YA_NUM_MODULI + cm; }
+
+int current_device(DeviceState **out) {
+  int dev = t_device;
+  if (dev < 0) {
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(YA_ERR_NO_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    t_device = dev;
+  }
+  std::lock_guard<std::mutex> g(g_dev_mu);
+  if ((int)g_devs.size() <= dev) g_devs.resize(dev + 1);
+  if (!g_devs[dev]) {
+    auto st = std::make_unique<DeviceState>();
+    st->device = dev;
+    hipError_t e = hipSetDevice(dev);
+    if (e != hipSuccess) return fail(YA_ERR_NO_DEVICE, "hipSetDevice(%d): %s", dev, hipGetErrorString(e));
+    e = hipStreamCreateWithFlags(&st->tstream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(YA_ERR_NO_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    const size_t nslots = (size_t)(kMaxLogN + 1) * YA_NUM_MODULI;
+    e = hipMalloc(&st->d_desc, nslots * sizeof(ModDesc));
+    if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMalloc(desc): %s", hipGetErrorString(e));
+    e = hipMemset(st->d_desc, 0, nslots * sizeof(ModDesc));
+    if (e != hipSuccess) return hip_fail(e, "hipMemset(desc)");
+    st->slots.reset(new Slot[nslots]);
+    g_devs[dev] = std::move(st);
+  }
+  *out = g_devs[dev].get();
+  return YA_OK;
+}
+
+// Builds (degree 2^logn, cm) for cm in [cm0, cm0 + count) where missing.
+int ensure_tables(DeviceState *st, int logn, size_t cm0, size_t count) {
+  if (logn < kMinLogN || logn > kMaxLogN) return fail(YA_ERR_INVALID, "unsupported log2(degree) %d", logn);
+  if (cm0 + count > YA_NUM_MODULI || count == 0)
+    return fail(YA_ERR_INVALID, "moduli [%zu, %zu) out of range", cm0, cm0 + count);
+  bool all = true;
+  for (size_t cm = cm0; cm < cm0 + count && all; ++cm)
+    all = st->slots[slot_index(logn, cm)].ready.load(std::memory_order_acquire) != 0;
+  if (all) return YA_OK;
+
+  std::lock_guard<std::mutex> g(st->mu);
+  std::vector<size_t> todo;
+  for (size_t cm = cm0; cm < cm0 + count; ++cm)
+    if (!st->slots[slot_index(logn, cm)].ready.load(std::memory_order_acquire)) todo.push_back(cm);
+  if (todo.empty()) return YA_OK;
+
+  const size_t n = (size_t)1 << logn;
+  std::vector<HostTable> host(todo.size());
+  {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nth = (unsigned)std::min<size_t>(hw, todo.size());
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned k = 0; k < nth; ++k)
+      pool.emplace_back([&]() {
+        for (size_t j; (j = next.fetch_add(1)) < todo.size();) build_table_host(logn, todo[j], host[j]);
+      });
+    for (auto &th : pool) th.join();
+  }
+  hipError_t e = hipSetDevice(st->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  std::vector<Tw> pairs(2 * n);
+  for (size_t j = 0; j < todo.size(); ++j) {
+    const size_t cm = todo[j];
+    const HostTable &t = host[j];
+    Slot &s = st->slots[slot_index(logn, cm)];
+    for (size_t i = 0; i < n; ++i) {
+      pairs[i] = Tw{t.phi[i], t.phi_s[i]};
+      pairs[n + i] = Tw{t.iphi[i], t.iphi_s[i]};
+    }
+    Tw *dev = nullptr;
+    e = hipMalloc(&dev, 2 * n * sizeof(Tw));
+    if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMalloc(table 2^%d): %s", logn, hipGetErrorString(e));
+    e = hipMemcpyAsync(dev, pairs.data(), 2 * n * sizeof(Tw), hipMemcpyHostToDevice, st->tstream);
+    if (e != hipSuccess) return hip_fail(e, "upload table");
+    ModDesc d;
+    d.fwd = dev;
+    d.inv = dev + n;
+    d.p = kP[cm];
+    d.pn = kPn[cm];
+    d.inv_n = t.sc[0];
+    d.inv_n_s = t.sc[1];
+    d.inv_n_phi = t.sc[2];
+    d.inv_n_phi_s = t.sc[3];
+    e = hipMemcpyAsync(st->d_desc + slot_index(logn, cm), &d, sizeof d, hipMemcpyHostToDevice,
+                       st->tstream);
+    if (e != hipSuccess) return hip_fail(e, "upload descriptor");
+    e = hipStreamSynchronize(st->tstream);  // `pairs` and `d` are reused / go out of scope
+    if (e != hipSuccess) return hip_fail(e, "table stream sync");
+    s.dev = dev;
+    const uint64_t ck[8] = {t.phi[1], t.phi_s[1], t.phi[n - 1], t.phi_s[n - 1],
+                            t.iphi[1], t.iphi_s[1], t.iphi[n - 1], t.iphi_s[n - 1]};
+    std::memcpy(s.check, ck, sizeof ck);
+    s.ready.store(1, std::memory_order_release);
+  }
+  return YA_OK;
+}
+
+const ModDesc *desc_ptr(DeviceState *st, int logn, size_t cm) {
+  return st->d_desc + slot_index(logn, cm);
+}
+
+const ModDesc *desc_table_base(DeviceState *st) { return st->d_desc; }
+
+int log2_degree(size_t degree) {
+  if (degree < 2 || (degree & (degree - 1))) return -1;
+  return __builtin_ctzll(degree);
+}
+
+int check_shape(size_t degree, size_t cm0, size_t nmoduli, int *logn) {
+  const int l = log2_degree(degree);
+  if (l < kMinLogN || l > kMaxLogN)
+    return fail(YA_ERR_INVALID, "degree %zu unsupported (powers of two 2^%d..2^%d)", degree,
+                kMinLogN, kMaxLogN);
+  if (nmoduli == 0 || cm0 + nmoduli > YA_NUM_MODULI)
+    return fail(YA_ERR_INVALID, "moduli [%zu, %zu) out of range [0, %d)", cm0, cm0 + nmoduli,
+                YA_NUM_MODULI);
+  *logn = l;
+  return YA_OK;
+}
+
+}  // namespace ya
+
+using namespace ya;
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+const char *ya_status_string(int s) {
+  switch (s) {
+    case YA_OK: return "ok";
+    case YA_ERR_INVALID: return "invalid argument";
+    case YA_ERR_NO_DEVICE: return "no HIP device";
+    case YA_ERR_ALLOC: return "allocation failed";
+    case YA_ERR_LAUNCH: return "kernel launch / HIP call failed";
+    case YA_ERR_UNSUPPORTED: return "unsupported";
+    case YA_ERR_REJECTED: return "rejected by admission control";
+    case YA_ERR_TIMEOUT: return "timed out";
+    case YA_ERR_MISMATCH: return "tables do not match";
+    default: return "unknown status";
+  }
+}
+
+const char *ya_last_error(void) { return t_err.c_str(); }
+
+int ya_init(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(YA_ERR_NO_DEVICE, "no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(YA_ERR_INVALID, "device %d of %d", device, n);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(YA_ERR_NO_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+  t_device = device;
+  DeviceState *st;
+  return current_device(&st);
+}
+
+int ya_device_count(int *count) {
+  if (!count) return fail(YA_ERR_INVALID, "null count");
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(YA_ERR_NO_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  return YA_OK;
+}
+
+int ya_device_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? YA_OK : hip_fail(e, "hipDeviceSynchronize");
+}
+
+size_t ya_num_moduli(void) { return YA_NUM_MODULI; }
+uint64_t ya_prime(size_t cm) { return cm < YA_NUM_MODULI ? kP[cm] : 0; }
+uint64_t ya_barrett_const(size_t cm) { return cm < YA_NUM_MODULI ? kPn[cm] : 0; }
+uint64_t ya_root(size_t cm) { return cm < YA_NUM_MODULI ? kRoot[cm] : 0; }
+
+int ya_ntt_init_tables(size_t degree, size_t count) {
+  int logn;
+  int rc = check_shape(degree, 0, count, &logn);
+  if (rc) return rc;
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  return ensure_tables(st, logn, 0, count);
+}
+
+int ya_ntt_table_host(size_t degree, size_t cm, uint64_t *phi, uint64_t *phi_s, uint64_t *iphi,
+                      uint64_t *iphi_s, uint64_t *scalars) {
+  const int logn = log2_degree(degree);
+  if (logn < 1 || logn > 20 || cm >= YA_NUM_MODULI) return fail(YA_ERR_INVALID, "bad degree/cm");
+  if (!phi || !phi_s || !iphi || !iphi_s || !scalars) return fail(YA_ERR_INVALID, "null output");
+  HostTable t;
+  build_table_host(logn, cm, t);
+  std::memcpy(phi, t.phi.data(), degree * 8);
+  std::memcpy(phi_s, t.phi_s.data(), degree * 8);
+  std::memcpy(iphi, t.iphi.data(), degree * 8);
+  std::memcpy(iphi_s, t.iphi_s.data(), degree * 8);
+  std::memcpy(scalars, t.sc, sizeof t.sc);
+  return YA_OK;
+}
+
+int ya_ntt_table_download(size_t degree, size_t cm, uint64_t *phi, uint64_t *phi_s, uint64_t *iphi,
+                          uint64_t *iphi_s, uint64_t *scalars) {
+  int logn;
+  int rc = check_shape(degree, cm, 1, &logn);
+  if (rc) return rc;
+  if (!phi || !phi_s || !iphi || !iphi_s || !scalars) return fail(YA_ERR_INVALID, "null output");
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, cm, 1))) return rc;
+  std::vector<Tw> pairs(2 * degree);
+  ModDesc d;
+  hipError_t e = hipMemcpy(pairs.data(), st->slots[slot_index(logn, cm)].dev,
+                           2 * degree * sizeof(Tw), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "download table");
+  e = hipMemcpy(&d, desc_ptr(st, logn, cm), sizeof d, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "download descriptor");
+  for (size_t i = 0; i < degree; ++i) {
+    phi[i] = pairs[i].w;
+    phi_s[i] = pairs[i].ws;
+    iphi[i] = pairs[degree + i].w;
+    iphi_s[i] = pairs[degree + i].ws;
+  }
+  scalars[0] = d.inv_n;
+  scalars[1] = d.inv_n_s;
+  scalars[2] = d.inv_n_phi;
+  scalars[3] = d.inv_n_phi_s;
+  return YA_OK;
+}
+
+// ---- memory / streams / events -------------------------------------------
+#define YA_HIP(expr, what)                         \
+  do {                                             \
+    hipError_t e_ = (expr);                        \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int ya_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return fail(YA_ERR_INVALID, "null ptr");
+  *ptr = nullptr;
+  if (bytes == 0) return YA_OK;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return YA_OK;
+}
+int ya_free(void *ptr) {
+  if (ptr) YA_HIP(hipFree(ptr), "hipFree");
+  return YA_OK;
+}
+int ya_memset_zero(void *ptr, size_t bytes, void *stream) {
+  YA_HIP(hipMemsetAsync(ptr, 0, bytes, (hipStream_t)stream), "hipMemsetAsync");
+  return YA_OK;
+}
+static int copy(void *dst, const void *src, size_t bytes, hipMemcpyKind k, void *stream) {
+  if (bytes == 0) return YA_OK;
+  if (!dst || !src) return fail(YA_ERR_INVALID, "null pointer");
+  YA_HIP(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream), "hipMemcpyAsync");
+  YA_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return YA_OK;
+}
+int ya_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream) {
+  return copy(dst, src, bytes, hipMemcpyHostToDevice, stream);
+}
+int ya_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream) {
+  return copy(dst, src, bytes, hipMemcpyDeviceToHost, stream);
+}
+int ya_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream) {
+  if (bytes == 0) return YA_OK;
+  YA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream), "hipMemcpyAsync");
+  return YA_OK;
+}
+int ya_host_alloc_pinned(void **ptr, size_t bytes) {
+  if (!ptr) return fail(YA_ERR_INVALID, "null ptr");
+  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return YA_OK;
+}
+int ya_host_free_pinned(void *ptr) {
+  if (ptr) YA_HIP(hipHostFree(ptr), "hipHostFree");
+  return YA_OK;
+}
+int ya_stream_create(void **stream) {
+  if (!stream) return fail(YA_ERR_INVALID, "null stream");
+  hipStream_t s;
+  YA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  *stream = s;
+  return YA_OK;
+}
+int ya_stream_destroy(void *stream) {
+  if (stream) YA_HIP(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+  return YA_OK;
+}
+int ya_stream_synchronize(void *stream) {
+  YA_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return YA_OK;
+}
+int ya_event_create(void **event) {
+  if (!event) return fail(YA_ERR_INVALID, "null event");
+  hipEvent_t ev;
+  YA_HIP(hipEventCreate(&ev), "hipEventCreate");
+  *event = ev;
+  return YA_OK;
+}
+int ya_event_destroy(void *event) {
+  if (event) YA_HIP(hipEventDestroy((hipEvent_t)event), "hipEventDestroy");
+  return YA_OK;
+}
+int ya_event_record(void *event, void *stream) {
+  YA_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+  return YA_OK;
+}
+int ya_event_synchronize(void *event) {
+  YA_HIP(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+  return YA_OK;
+}
+int ya_event_elapsed_ms(float *ms, void *start, void *end) {
+  if (!ms) return fail(YA_ERR_INVALID, "null ms");
+  YA_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end), "hipEventElapsedTime");
+  return YA_OK;
+}
+
+// ---- NTT ------------------------------------------------------------------
+static int ntt_common(uint64_t *x, size_t degree, size_t cm0, size_t nres, int dir, void *stream) {
+  int logn;
+  int rc = check_shape(degree, cm0, nres, &logn);
+  if (rc) return rc;
+  if (!x) return fail(YA_ERR_INVALID, "null buffer");
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, cm0, nres))) return rc;
+  const ModDesc *d = desc_ptr(st, logn, cm0);
+  hipError_t e;
+  if (dir == 2) e = launch_backward(x, d, logn, nres, tuned_shape(logn, 0), (hipStream_t)stream);
+  else e = launch_forward(x, d, logn, nres, dir == 1, tuned_shape(logn, 0), (hipStream_t)stream);
+  return e == hipSuccess ? YA_OK : hip_fail(e, "NTT launch");
+}
+
+int ya_ntt_forward(uint64_t *x, size_t degree, size_t cm, void *stream) {
+  return ntt_common(x, degree, cm, 1, 0, stream);
+}
+int ya_ntt_forward_lazy(uint64_t *x, size_t degree, size_t cm, void *stream) {
+  return ntt_common(x, degree, cm, 1, 1, stream);
+}
+int ya_ntt_backward(uint64_t *x, size_t degree, size_t cm, void *stream) {
+  return ntt_common(x, degree, cm, 1, 2, stream);
+}
+int ya_poly_forward(uint64_t *poly, size_t degree, size_t nmoduli, void *stream) {
+  return ntt_common(poly, degree, 0, nmoduli, 0, stream);
+}
+int ya_poly_forward_lazy(uint64_t *poly, size_t degree, size_t nmoduli, void *stream) {
+  return ntt_common(poly, degree, 0, nmoduli, 1, stream);
+}
+int ya_poly_backward(uint64_t *poly, size_t degree, size_t nmoduli, void *stream) {
+  return ntt_common(poly, degree, 0, nmoduli, 2, stream);
+}
+
+static int seam_host(uint64_t *x, size_t degree, const uint64_t *w, const uint64_t *ws, uint64_t p,
+                     bool inverse) {
+  int logn;
+  int rc = check_shape(degree, 0, 1, &logn);
+  if (rc) return rc;
+  if (!x || !w || !ws) return fail(YA_ERR_INVALID, "null pointer");
+  size_t cm = YA_NUM_MODULI;
+  for (size_t k = 0; k < YA_NUM_MODULI; ++k)
+    if (kP[k] == p) {
+      cm = k;
+      break;
+    }
+  if (cm == YA_NUM_MODULI) return fail(YA_ERR_INVALID, "modulus %llu is not in the prime table",
+                                       (unsigned long long)p);
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, cm, 1))) return rc;
+  const uint64_t *ck = st->slots[slot_index(logn, cm)].check + (inverse ? 4 : 0);
+  if (w[1] != ck[0] || ws[1] != ck[1] || w[degree - 1] != ck[2] || ws[degree - 1] != ck[3])
+    return fail(YA_ERR_MISMATCH, "caller tables are not the (degree %zu, P[%zu]) %s tables", degree,
+                cm, inverse ? "inverse" : "forward");
+  uint64_t *d = nullptr;
+  hipError_t e = hipMalloc(&d, degree * 8);
+  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMalloc: %s", hipGetErrorString(e));
+  e = hipMemcpy(d, x, degree * 8, hipMemcpyHostToDevice);
+  const ModDesc *dd = desc_ptr(st, logn, cm);
+  if (e == hipSuccess) {
+    if (inverse) e = launch_inverse_lazy(d, dd, logn, 1, nullptr);
+    else e = launch_forward(d, dd, logn, 1, true, tuned_shape(logn, 0), nullptr);
+  }
+  if (e == hipSuccess) e = hipMemcpy(x, d, degree * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? YA_OK : hip_fail(e, "seam");
+}
+
+int ya_negacylic_forward_lazy(uint64_t *x, size_t degree, const uint64_t *w, const uint64_t *ws,
+                              uint64_t p) {
+  return seam_host(x, degree, w, ws, p, false);
+}
+int ya_negacylic_backward_lazy(uint64_t *x, size_t degree, const uint64_t *w, const uint64_t *ws,
+                               uint64_t p) {
+  return seam_host(x, degree, w, ws, p, true);
+}
+
+// ---- pointwise ------------------------------------------------------------
+static int pw_common(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree,
+                     size_t nmoduli, void *stream) {
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!out || !a || (!b && op != PW_NEGATE)) return fail(YA_ERR_INVALID, "null buffer");
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  for (size_t r0 = 0; r0 < nmoduli; r0 += 65535) {
+    const size_t nr = std::min<size_t>(65535, nmoduli - r0);
+    const size_t off = r0 * degree;
+    hipError_t e = launch_pointwise(op, out + off, a + off, b ? b + off : nullptr,
+                                    desc_ptr(st, logn, r0), degree, nr, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "pointwise launch");
+  }
+  return YA_OK;
+}
+
+int ya_poly_mul(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                void *stream) {
+  return pw_common(PW_MUL, out, a, b, degree, nmoduli, stream);
+}
+int ya_poly_add(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                void *stream) {
+  return pw_common(PW_ADD, out, a, b, degree, nmoduli, stream);
+}
+int ya_poly_sub(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                void *stream) {
+  return pw_common(PW_SUB, out, a, b, degree, nmoduli, stream);
+}
+int ya_poly_negate(uint64_t *x, size_t degree, size_t nmoduli, void *stream) {
+  return pw_common(PW_NEGATE, x, x, nullptr, degree, nmoduli, stream);
+}
+int ya_poly_add_product_of(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t degree,
+                           size_t nmoduli, void *stream) {
+  return pw_common(PW_MULADD, acc, a, b, degree, nmoduli, stream);
+}
+
+int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                     size_t nmoduli, void *stream) {
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!a || !b || !c) return fail(YA_ERR_INVALID, "null buffer");
+  if (c == a || c == b) return fail(YA_ERR_INVALID, "c must not alias a or b");
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  const ModDesc *d = desc_ptr(st, logn, 0);
+  hipStream_t s = (hipStream_t)stream;
+  if (logn <= kMaxBlockLog) {
+    hipError_t e = launch_polymul(c, a, b, d, logn, nmoduli, tuned_shape(logn, 1), s);
+    return e == hipSuccess ? YA_OK : hip_fail(e, "polymul launch");
+  }
+  // Large rings: forward(a) into c, forward(b) into stream-ordered scratch,
+  // c = c * scratch, backward(c).
+  const size_t words = degree * nmoduli;
+  uint64_t *tmp = nullptr;
+  hipError_t e = hipMallocAsync((void **)&tmp, words * 8, s);
+  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMallocAsync: %s", hipGetErrorString(e));
+  const NttShape sh = tuned_shape(logn, 0);
+  e = hipMemcpyAsync(c, a, words * 8, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(tmp, b, words * 8, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = launch_forward(c, d, logn, nmoduli, true, sh, s);
+  if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, true, sh, s);
+  for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
+    const size_t nr = std::min<size_t>(65535, nmoduli - r0);
+    e = launch_pointwise(PW_MUL, c + r0 * degree, c + r0 * degree, tmp + r0 * degree, d + r0,
+                         degree, nr, s);
+  }
+  if (e == hipSuccess) e = launch_backward(c, d, logn, nmoduli, sh, s);
+  hipError_t ef = hipFreeAsync(tmp, s);
+  if (e == hipSuccess) e = ef;
+  return e == hipSuccess ? YA_OK : hip_fail(e, "polymul (large ring)");
+}
+
+}  // extern "C"

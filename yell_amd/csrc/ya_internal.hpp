@@ -1,0 +1,36 @@
+// ya_internal.hpp -- declarations shared by the host-side translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ya_kernels.hpp"
+
+namespace ya {
+
+struct DeviceState;
+
+// Records a formatted message for ya_last_error() and returns `code`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int hip_fail(hipError_t e, const char *what);
+
+// State of the calling thread's current HIP device (created on first use).
+int current_device(DeviceState **out);
+// Makes the tables of (2^logn, cm) for cm in [cm0, cm0+count) device-resident.
+int ensure_tables(DeviceState *st, int logn, size_t cm0, size_t count);
+// Device descriptor of (2^logn, cm); descriptors of consecutive cm are adjacent.
+const ModDesc *desc_ptr(DeviceState *st, int logn, size_t cm);
+// Base of the device descriptor array: entry (logn, cm) is at logn * 1000 + cm.
+const ModDesc *desc_table_base(DeviceState *st);
+
+int log2_degree(size_t degree);
+int check_shape(size_t degree, size_t cm0, size_t nmoduli, int *logn);
+uint64_t prime(size_t cm);
+
+// Tuned launch shape for a degree class.  kind 0: standalone NTT kernels,
+// kind 1: fused poly-multiplication, kind 2: batched poly-multiplication.
+NttShape tuned_shape(int logn, int kind);
+void set_tuned_shape(int logn, int kind, NttShape s);
+
+}  // namespace ya

@@ -1,0 +1,48 @@
+// ya_kernels.hpp -- host-side launch wrappers of the HIP kernels (C++ only,
+// no templates leak out).  All return hipError_t of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ya_ntt.hpp"
+
+namespace ya {
+
+// Largest block handled in one workgroup's LDS (2^14 words = 128 KiB + padding).
+constexpr int kMaxBlockLog = 14;
+constexpr int kMinLogN = 3;
+constexpr int kMaxLogN = 17;
+
+// Elements per thread (log2) chosen for a block of 2^lb words; 0 = default.
+struct NttShape {
+  int le = 4;
+};
+
+// forward / forward_lazy over `nres` residues of 2^logn words, residue r at
+// data + r * 2^logn with descriptor descs[r].
+hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, bool lazy,
+                          NttShape shape, hipStream_t st);
+hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
+                           NttShape shape, hipStream_t st);
+
+// negacylic_backward_lazy (lib/ntt.cpp:151-210): every inverse layer except
+// the top one, outputs in [0, 2p).
+hipError_t launch_inverse_lazy(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
+                               hipStream_t st);
+
+// c = backward(forward(a) (*) forward(b)) for `nres` residues (a, b, c share
+// the residue-major layout); c may not alias a or b.
+hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
+                          int logn, size_t nres, NttShape shape, hipStream_t st);
+
+enum PwOp { PW_MUL = 0, PW_ADD = 1, PW_SUB = 2, PW_NEGATE = 3, PW_MULADD = 4, PW_CORRECT = 5 };
+
+// Element-wise op over `nres` residues of `n` words; residue r uses descs[r].p / pn.
+//   MUL/ADD/SUB:  out = a op b;  NEGATE: out = p - a;  MULADD: out = out + a*b;
+//   CORRECT: out = a reduced from [0,4p) to [0,p).
+hipError_t launch_pointwise(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
+                            const ModDesc *descs, size_t n, size_t nres, hipStream_t st);
+
+}  // namespace ya

@@ -1,0 +1,50 @@
+// ya_kernels_pm.hip -- fused RNS poly-multiplication kernel.
+#include "ya_blocks.hpp"
+
+namespace ya {
+
+// Fused c = backward(forward(a) (*) forward(b)) for one residue of 2^LB words
+// per workgroup.  forward(a) is parked in c at the positions this thread
+// later reads back itself, so no other buffer is needed.
+template <int LB, int LE>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
+          const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs) {
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t res = blockIdx.x;
+  const ModDesc d = descs[res];
+  const size_t off = (size_t)res << LB;
+  const uint32_t t = threadIdx.x;
+  uint64_t x[1 << LE], y[1 << LE];
+  fwd_block<LB, LE>(x, a + off, lds, d.fwd, LB, 0u, d.p, t);
+  gmem_store<0, LE>(c + off, x, t);
+  fwd_block<LB, LE>(x, b + off, lds, d.fwd, LB, 0u, d.p, t);
+  gmem_load<0, LE>(c + off, y, t);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+  inv_block<LB, LE, TOP_MERGED>(x, c + off, lds, d.inv, LB, 0u, d, t);
+}
+
+namespace {
+
+template <int LB, int LE>
+hipError_t polymul_lb(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
+                      size_t nres, hipStream_t st) {
+  hipLaunchKernelGGL((k_polymul<LB, LE>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c, a,
+                     b, descs);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
+                          int logn, size_t nres, NttShape shape, hipStream_t st) {
+  if (logn < kMinLogN || logn > kMaxLogN || nres == 0) return hipErrorInvalidValue;
+  if (logn > kMaxBlockLog) return hipErrorNotSupported;  // composed by the caller
+  const int le = clamp_le(logn, shape.le);
+#define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, st)
+  YA_LB_SWITCH(logn, le, YA_PM_CALL)
+#undef YA_PM_CALL
+}
+
+}  // namespace ya

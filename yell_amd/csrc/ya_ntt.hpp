@@ -1,0 +1,153 @@
+// ya_ntt.hpp -- register/LDS building blocks of the negacyclic NTT on gfx950.
+//
+// Index conventions.  A residue has N = 2^logn words.  The reference's layer
+// with butterfly half-distance h = 2^b (lib/ntt.cpp:102-147 forward, :162-208
+// inverse) pairs the words whose indices differ only in bit b; the pair
+// (i, i + h) uses twiddle tab[(N >> (b+1)) + (i >> (b+1))] in both
+// directions (tab = the bit-reversed phi or inverse-phi table,
+// include/yell/details/ntt.hpp:47-71).  The forward transform visits
+// b = logn-1 ... 0 with the CT butterfly; the inverse visits b = 0 ... logn-2
+// with the GS butterfly and ends with the merged (b = logn-1, x N^-1) layer
+// (include/yell/details/ntt.hpp:151-159).  The pairs of one layer are
+// disjoint, so any schedule that keeps this per-pair layer order computes
+// the same words: that is what lets several layers run in registers.
+//
+// A "pass" holds E = 2^LE words per thread whose indices differ only in the
+// window of bits [WLO, WLO + LE); the remaining index bits come from the
+// thread number t: the low WLO bits of i are the low WLO bits of t, the bits
+// above the window are t >> WLO.  Between passes the words are exchanged
+// through LDS with a padded layout (one spare word per 16) so that strided
+// windows do not all land in the same banks.
+#pragma once
+
+#include "ya_arith.hpp"
+
+namespace ya {
+
+// Per-(degree, prime) device descriptor.  fwd/inv point at N twiddle pairs in
+// bit-reversed order (the reference's phiTbl/shoupPhiTbl and
+// invphiTbl/shoupInvphiTbl, interleaved).
+struct ModDesc {
+  const Tw *fwd;
+  const Tw *inv;
+  uint64_t p;
+  uint64_t pn;           // Barrett constant low64(floor(2^128 / p))
+  uint64_t inv_n;        // N^-1 mod p and its Shoup companion
+  uint64_t inv_n_s;
+  uint64_t inv_n_phi;    // N^-1 * invphiTbl[1] and its Shoup companion
+  uint64_t inv_n_phi_s;
+};
+
+__device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
+
+template <int LB>
+struct LdsWords {
+  static constexpr uint32_t value = (1u << LB) + ((1u << LB) >> 4);
+};
+
+// Global index of word e of thread t for window [WLO, WLO+LE), in a block of
+// 2^LB words: i = tlo | e << WLO | thi << (WLO+LE).
+template <int WLO, int LE>
+__device__ __forceinline__ uint32_t win_base(uint32_t t) {
+  const uint32_t tlo = t & ((1u << WLO) - 1u);
+  const uint32_t thi = t >> WLO;
+  return tlo | (thi << (WLO + LE));
+}
+
+template <int WLO, int LE>
+__device__ __forceinline__ void lds_store(uint64_t *s, const uint64_t (&x)[1 << LE], uint32_t t) {
+  const uint32_t base = win_base<WLO, LE>(t);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) s[lds_pad(base | ((uint32_t)e << WLO))] = x[e];
+}
+
+template <int WLO, int LE>
+__device__ __forceinline__ void lds_load(const uint64_t *s, uint64_t (&x)[1 << LE], uint32_t t) {
+  const uint32_t base = win_base<WLO, LE>(t);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = s[lds_pad(base | ((uint32_t)e << WLO))];
+}
+
+template <int WLO, int LE>
+__device__ __forceinline__ void gmem_load(const uint64_t *__restrict__ g, uint64_t (&x)[1 << LE],
+                                          uint32_t t) {
+  const uint32_t base = win_base<WLO, LE>(t);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = g[base | ((uint32_t)e << WLO)];
+}
+
+template <int WLO, int LE>
+__device__ __forceinline__ void gmem_store(uint64_t *__restrict__ g, const uint64_t (&x)[1 << LE],
+                                           uint32_t t) {
+  const uint32_t base = win_base<WLO, LE>(t);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) g[base | ((uint32_t)e << WLO)] = x[e];
+}
+
+// Forward (CT) layers b = HI-1 ... LO of one pass.  `gbase` is the residue
+// index of word 0 of this thread with the window bits and the bits below the
+// window cleared (i.e. block offset | thi << (WLO+LE)); nlog = log2 N of the
+// whole residue.
+template <int WLO, int LE, int HI, int LO>
+__device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
+                                           uint32_t nlog, uint32_t gbase, uint64_t p) {
+  const uint64_t p2 = p << 1;
+#pragma unroll
+  for (int b = HI - 1; b >= LO; --b) {
+    const int j = b - WLO;
+    const uint32_t toff = ((1u << nlog) >> (b + 1)) + (gbase >> (b + 1));
+#pragma unroll
+    for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
+      const Tw t = tw[toff + g];
+#pragma unroll
+      for (int k = 0; k < (1 << j); ++k) {
+        const int e0 = (g << (j + 1)) | k;
+        ct_butterfly(x[e0], x[e0 | (1 << j)], t, p, p2);
+      }
+    }
+  }
+}
+
+// Inverse (GS) layers b = LO ... HI-1 of one pass (same indexing).
+template <int WLO, int LE, int HI, int LO>
+__device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
+                                           uint32_t nlog, uint32_t gbase, uint64_t p) {
+  const uint64_t p2 = p << 1;
+#pragma unroll
+  for (int b = LO; b < HI; ++b) {
+    const int j = b - WLO;
+    const uint32_t toff = ((1u << nlog) >> (b + 1)) + (gbase >> (b + 1));
+#pragma unroll
+    for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
+      const Tw t = tw[toff + g];
+#pragma unroll
+      for (int k = 0; k < (1 << j); ++k) {
+        const int e0 = (g << (j + 1)) | k;
+        gs_butterfly(x[e0], x[e0 | (1 << j)], t, p, p2);
+      }
+    }
+  }
+}
+
+// The merged last inverse layer on window bit J (include/yell/details/ntt.hpp:151-159):
+// x0 = (u + v) * N^-1, x1 = (u - v + 2p) * N^-1 * invphi[1], each by Shoup, canonical.
+template <int LE, int J>
+__device__ __forceinline__ void inv_last_layer(uint64_t (&x)[1 << LE], const ModDesc &d) {
+  const uint64_t p = d.p, p2 = p << 1;
+#pragma unroll
+  for (int e0 = 0; e0 < (1 << LE); ++e0) {
+    if (e0 & (1 << J)) continue;
+    const uint64_t u = x[e0], v = x[e0 | (1 << J)];
+    x[e0] = mulmod_shoup(u + v, d.inv_n, d.inv_n_s, p);
+    x[e0 | (1 << J)] = mulmod_shoup(u - v + p2, d.inv_n_phi, d.inv_n_phi_s, p);
+  }
+}
+
+template <int LE>
+__device__ __forceinline__ void correct_all(uint64_t (&x)[1 << LE], uint64_t p) {
+  const uint64_t p2 = p << 1;
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = correct4p(x[e], p, p2);
+}
+
+}  // namespace ya

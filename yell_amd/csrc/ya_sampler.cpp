@@ -1,0 +1,106 @@
+// ya_sampler.cpp -- seeded host samplers, the input source of the engine.
+//
+// The reference's samplers draw from an unseeded Salsa20 keystream
+// (lib/prng/fastrandombytes.cpp:21-34), so their outputs cannot be
+// reproduced; these keep the reference's mapping of random words to residues
+// and are seeded:
+//   uniform   include/yell/details/poly.hpp:372-396
+//   gaussian  include/yell/details/poly.hpp:408-431 (one signed sample per
+//             coefficient shared by all residues, r < 0 ? p + r : r)
+// The random words come from xoshiro256** (Blackman & Vigna) seeded through
+// splitmix64; the Gaussian is a cumulative-distribution-table sampler over
+// |x| <= ceil(13 sigma) (tail mass below 2^-120).
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/yell_amd.h"
+#include "ya_internal.hpp"
+
+namespace {
+
+struct Xoshiro {
+  uint64_t s[4];
+  explicit Xoshiro(uint64_t seed) {
+    for (auto &w : s) {  // splitmix64
+      seed += 0x9e3779b97f4a7c15ULL;
+      uint64_t z = seed;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+      w = z ^ (z >> 31);
+    }
+  }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t r = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+};
+
+}  // namespace
+
+extern "C" int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed) {
+  int logn;
+  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!out) return ya::fail(YA_ERR_INVALID, "null output");
+  Xoshiro g(seed);
+  for (size_t cm = 0; cm < nmoduli; ++cm) {
+    const uint64_t p = ya::prime(cm);
+    const int top = 63 - __builtin_clzll(p);  // floor(log2 p)
+    const uint64_t mask = (top >= 63) ? ~0ULL : ((1ULL << (top + 1)) - 1);
+    uint64_t *dst = out + cm * degree;
+    for (size_t i = 0; i < degree; ++i) {
+      uint64_t v = g.next() & mask;
+      if (v >= p) v -= p;
+      dst[i] = v;
+    }
+  }
+  return YA_OK;
+}
+
+extern "C" int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, double sigma,
+                                  uint64_t seed) {
+  int logn;
+  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!out) return ya::fail(YA_ERR_INVALID, "null output");
+  if (!(sigma > 0.0) || sigma > 1e6) return ya::fail(YA_ERR_INVALID, "sigma %g out of range", sigma);
+  const int tail = (int)std::ceil(13.0 * sigma);
+  std::vector<long double> w(tail + 1);
+  long double tot = 0;
+  for (int k = 0; k <= tail; ++k) {
+    w[k] = std::exp(-(long double)k * k / (2.0L * sigma * sigma)) * (k ? 2.0L : 1.0L);
+    tot += w[k];
+  }
+  // cdf[k] = P(|X| <= k) scaled to 2^64; the last entry saturates.
+  std::vector<uint64_t> cdf(tail + 1);
+  long double acc = 0;
+  for (int k = 0; k <= tail; ++k) {
+    acc += w[k] / tot;
+    const long double v = acc * 18446744073709551616.0L;
+    cdf[k] = (k == tail || v >= 18446744073709551615.0L) ? ~0ULL : (uint64_t)v;
+  }
+  Xoshiro g(seed);
+  std::vector<int64_t> r(degree);
+  for (size_t i = 0; i < degree; ++i) {
+    const uint64_t u = g.next();
+    int k = 0;
+    while (k < tail && u >= cdf[k]) ++k;
+    const bool neg = k != 0 && (g.next() >> 63);
+    r[i] = neg ? -(int64_t)k : (int64_t)k;
+  }
+  for (size_t cm = 0; cm < nmoduli; ++cm) {
+    const uint64_t p = ya::prime(cm);
+    uint64_t *dst = out + cm * degree;
+    for (size_t i = 0; i < degree; ++i) dst[i] = r[i] < 0 ? p + r[i] : (uint64_t)r[i];
+  }
+  return YA_OK;
+}
