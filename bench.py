@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Throughput bench of the RNS poly-multiplication path on one MI355X.
+
+Workload: N = 2^14, L = 8 (primes P[0..7]); operand a uniform per residue,
+operand b a centred discrete Gaussian (sigma = 3.2) reduced mod each prime,
+both from fixed seeds; one poly-mult = forward(a), forward(b), Barrett
+pointwise product, backward.  A step is one batched launch over --batch
+polynomials already resident in HBM.  The same polynomials are also run as
+plain one-poly-per-launch calls, and a bounded sample is timed on the host
+CPU with the reference's own code (oracle/_ref, compiled from its sources) or,
+where that is absent, the oracle restatement.
+
+Prints ONE JSON line.  `value` = batched poly-mults/s.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from yell_amd import capi  # noqa: E402
+
+METRIC = "bit-exact poly-mults/s at N=2^14,L=8 batched vs plain; p50/p99 served latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def modmuls_per_polymul(n, L):
+    lg = n.bit_length() - 1
+    fwd = (n // 2) * lg
+    inv = (n // 2) * (lg - 1) + n
+    return L * (2 * fwd + inv + n)
+
+
+class Events:
+    def __init__(self):
+        self.a, self.b = ctypes.c_void_p(), ctypes.c_void_p()
+        capi.call("ya_event_create", ctypes.byref(self.a))
+        capi.call("ya_event_create", ctypes.byref(self.b))
+
+    def elapsed_ms(self):
+        capi.call("ya_event_synchronize", self.b)
+        ms = ctypes.c_float()
+        capi.call("ya_event_elapsed_ms", ctypes.byref(ms), self.a, self.b)
+        return ms.value
+
+
+def make_inputs(n, L, batch, seed):
+    a = np.empty(batch * n * L, dtype=np.uint64)
+    b = np.empty(batch * n * L, dtype=np.uint64)
+    w = n * L
+    for k in range(batch):
+        a[k * w:(k + 1) * w] = capi.sample_uniform(n, L, seed + 2 * k)
+        b[k * w:(k + 1) * w] = capi.sample_gaussian(n, L, 3.2, seed + 2 * k + 1)
+    return a, b
+
+
+def cpu_baseline(a0, b0, n, L, gpu_c0, seconds):
+    """Time the reference's poly-mult on the host cores (bounded sample) and
+    check the GPU result of polynomial 0 against it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle  # checker only: this leg measures and verifies, it is never the product
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    try:
+        ref = _oracle.RefLib()
+        kind = "reference"
+        want = ref.polymul(a0, b0, n, L)
+        lg = n.bit_length() - 1
+        t1 = ref.lib.yref_bench_polymul(lg, capi.ptr(a0), capi.ptr(b0), L, 1, 1)
+        reps = max(1, int(seconds / max(t1, 1e-6) / 1.0))
+        reps_1 = max(1, int(0.25 * seconds / max(t1, 1e-6)))
+        s1 = ref.lib.yref_bench_polymul(lg, capi.ptr(a0), capi.ptr(b0), L, 1, reps_1)
+        sN = ref.lib.yref_bench_polymul(lg, capi.ptr(a0), capi.ptr(b0), L, threads,
+                                        max(1, reps // threads * 1))
+        per_thread = max(1, reps // threads)
+        value = threads * per_thread / sN
+        single = reps_1 / s1
+        sample = ("%d threads x %d poly-mults (N=%d, L=%d) of the reference's own ntt<N>/ops code, "
+                  "Release flags; single core: %d poly-mults" % (threads, per_thread, n, L, reps_1))
+    except RuntimeError:
+        orc = _oracle.Oracle()
+        kind = "port"
+        threads = 1
+        t0 = time.perf_counter()
+        want = orc.polymul(a0, b0, n, L)
+        t1 = time.perf_counter() - t0
+        reps = max(1, int(seconds / max(t1, 1e-6)))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            orc.polymul(a0, b0, n, L)
+        value = reps / (time.perf_counter() - t0)
+        single = value
+        sample = "%d poly-mults (N=%d, L=%d) of the C restatement, one thread" % (reps, n, L)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": value, "unit": "poly-mults/s", "cores": threads, "kind": kind,
+            "sample": sample, "single_core_value": single, "cpu_model": model,
+            "host_cpus": os.cpu_count()}, bool(np.array_equal(want, gpu_c0))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and this path does not "
+              "shard across GPUs (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    n, L, B = 1 << 14, 8, args.batch
+    w = n * L
+    capi.load()
+    capi.call("ya_init", 0)
+    log("inputs: batch", B, "N", n, "L", L)
+    a, b = make_inputs(n, L, B, seed=1234)
+    da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
+    dc = capi.DeviceBuffer(B * w)
+    stream = ctypes.c_void_p()
+    capi.call("ya_stream_create", ctypes.byref(stream))
+    capi.call("ya_ntt_init_tables", n, L)
+
+    def batched():
+        capi.call("ya_poly_multiply_many", dc.ptr, da.ptr, db.ptr, n, L, B, stream)
+
+    def plain():
+        for k in range(B):
+            off = k * w * 8
+            capi.call("ya_poly_multiply", dc.ptr + off, da.ptr + off, db.ptr + off, n, L, stream)
+
+    ev = Events()
+    for _ in range(args.warmup):
+        batched()
+    capi.call("ya_stream_synchronize", stream)
+    log("timing batched:", args.steps, "steps")
+    t0 = time.perf_counter()
+    capi.call("ya_event_record", ev.a, stream)
+    for _ in range(args.steps):
+        batched()
+    capi.call("ya_event_record", ev.b, stream)
+    ms = ev.elapsed_ms()
+    wall = time.perf_counter() - t0
+    ms_step = ms / args.steps
+    value = B / (ms_step / 1e3)
+    c0 = dc.download(w)
+
+    plain()
+    capi.call("ya_stream_synchronize", stream)
+    log("timing plain:", args.plain_steps, "steps of", B, "launches")
+    capi.call("ya_event_record", ev.a, stream)
+    for _ in range(args.plain_steps):
+        plain()
+    capi.call("ya_event_record", ev.b, stream)
+    ms_plain = ev.elapsed_ms() / args.plain_steps
+    plain_value = B / (ms_plain / 1e3)
+    c0_plain = dc.download(w)
+
+    bytes_per = 3 * n * L * 8
+    achieved = bytes_per * B / (ms_step / 1e3) / 1e9
+    mm = modmuls_per_polymul(n, L)
+    out = {
+        "metric": METRIC, "value": value, "unit": "poly-mults/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded uniform x Gaussian sigma=3.2 operands)",
+        "config": {"workload": "RNS poly-multiplication N=16384 L=8 (P[0..7]), a uniform, "
+                               "b Gaussian sigma=3.2; %d polys per batched launch" % B,
+                   "degree": n, "nmoduli": L, "batch": B},
+        "plain": {"value": plain_value, "unit": "poly-mults/s", "ms_per_step": ms_plain,
+                  "launches_per_step": B},
+        "batched_over_plain": value / plain_value,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_polymul<14,4>", "bytes_per_polymul": bytes_per},
+        "modmul_rate": {"modmuls_per_polymul": mm,
+                        "achieved_T_per_s": mm * B / (ms_step / 1e3) / 1e12},
+        "timed_wall_s": wall,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (~%.0f s)" % args.cpu_seconds)
+        cb, exact = cpu_baseline(a[:w].copy(), b[:w].copy(), n, L, c0, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["bit_exact_vs_cpu"] = exact and bool(np.array_equal(c0, c0_plain))
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

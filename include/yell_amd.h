@@ -140,6 +140,40 @@ int ya_poly_add_product_of(uint64_t *acc, const uint64_t *a, const uint64_t *b, 
 int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
                      size_t nmoduli, void *stream);
 
+/* `count` independent poly-multiplications of the same shape in one launch:
+ * polynomial k occupies words [k*L*N, (k+1)*L*N) of a, b and c, each in the
+ * layout above (residue r of every polynomial uses P[r]).  Same results as
+ * `count` calls of ya_poly_multiply. */
+int ya_poly_multiply_many(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                          size_t nmoduli, size_t count, void *stream);
+
+/* ---- batched form (uneven degrees and modulus counts) ------------------- *
+ * No single counterpart in the reference: a batch of the reference's
+ * poly-multiplications (the sequence above), possibly of different degrees
+ * and modulus counts, run by one persistent kernel that drains a queue of
+ * work items.  Each job's c equals what ya_poly_multiply gives for it.
+ * Degrees 2^7 .. 2^15 (YA_ERR_UNSUPPORTED otherwise).  A plan fixes the
+ * buffers; the caller may rewrite their contents between runs.  Runs of one
+ * plan must not overlap (one stream at a time). */
+typedef struct ya_polymul_job {
+  const uint64_t *a;  /* device, [nmoduli][degree] */
+  const uint64_t *b;
+  uint64_t *c;        /* must not overlap a or b */
+  uint32_t degree;
+  uint32_t nmoduli;
+} ya_polymul_job;
+typedef struct ya_batch_plan ya_batch_plan;
+int ya_batch_plan_create(ya_batch_plan **plan, const ya_polymul_job *jobs, size_t njobs);
+/* Launches the persistent kernel on `stream`. */
+int ya_batch_plan_run(ya_batch_plan *plan, void *stream);
+/* Same work as a hipGraph: captured on first use, then replayed on `stream`. */
+int ya_batch_plan_run_graph(ya_batch_plan *plan, void *stream);
+int ya_batch_plan_info(const ya_batch_plan *plan, size_t *items, size_t *residues,
+                       size_t *workgroups);
+int ya_batch_plan_destroy(ya_batch_plan *plan);
+/* One-shot: create, run on `stream`, wait, destroy. */
+int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
+
 /* ---- host samplers (input source; details/poly.hpp:201-260) ------------- */
 /* Seeded replacements of poly::set(uniform) and poly::set(gaussian): fill a
  * HOST residue-major buffer.  uniform: per residue, 64-bit word & (2^62-1),
@@ -148,6 +182,14 @@ int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t d
  * shared by all residues and mapped r < 0 ? p + r : r. */
 int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed);
 int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed);
+
+/* ---- measurement ---------------------------------------------------------
+ * No counterpart in the reference.  Device throughput of the VALU work this
+ * engine is bound by, all operands in registers: kind 0 = v_mad_u64_u32
+ * (32x32+64 -> 64), kind 1 = v_mul_lo_u32, kind 2 = this engine's lazy CT
+ * butterfly (lib/ntt.cpp:69-83).  Result in operations (butterflies) per
+ * second over the whole device. */
+int ya_probe_valu(int kind, double *ops_per_second);
 
 #ifdef __cplusplus
 }

@@ -108,7 +108,9 @@ class Oracle:
         return self._binary("or_poly_sub", a, b, n, L)
 
     def poly_negate(self, a, n, L):
-        return self._inplace("or_poly_negate", a, n, L)
+        y = np.ascontiguousarray(a, dtype=np.uint64).copy()
+        self.lib.or_poly_negate(_p(y), n, L)
+        return y
 
     def poly_add_product_of(self, acc, a, b, n, L):
         y = np.ascontiguousarray(acc, dtype=np.uint64).copy()

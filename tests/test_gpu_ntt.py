@@ -1,0 +1,175 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the
+reference-produced golden vectors.  Integer work: every comparison is
+bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+from _oracle import Oracle
+from gpu_util import device, polymul, rand_bound, run_binary, run_inplace, say
+from yell_amd import capi
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = np.load(os.path.join(HERE, "golden", "ntt_ref.npz"))
+orc = Oracle()
+P = [orc.prime(i) for i in range(32)]
+LOGNS = list(range(3, 18))
+
+
+@pytest.mark.parametrize("logn", [3, 10, 14, 16, 17])
+def test_resident_tables(logn):
+    device()
+    n = 1 << logn
+    for cm in (0, 7):
+        say("tables", logn, cm)
+        for a, b in zip(capi.table_download(n, cm), orc.tables(n, cm)):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("logn", LOGNS)
+def test_poly_forward_lazy_and_forward(logn):
+    n, L = 1 << logn, (3 if logn <= 14 else 2)
+    rng = np.random.default_rng(logn)
+    x = rand_bound(rng, n, L, 4, P)
+    say("forward_lazy", logn, L)
+    np.testing.assert_array_equal(run_inplace("ya_poly_forward_lazy", x, n, L),
+                                  orc.poly_forward_lazy(x, n, L))
+    say("forward", logn, L)
+    np.testing.assert_array_equal(run_inplace("ya_poly_forward", x, n, L),
+                                  orc.poly_forward(x, n, L))
+
+
+@pytest.mark.parametrize("logn", LOGNS)
+def test_poly_backward(logn):
+    n, L = 1 << logn, (3 if logn <= 14 else 2)
+    rng = np.random.default_rng(50 + logn)
+    x = rand_bound(rng, n, L, 2, P)
+    say("backward", logn, L)
+    np.testing.assert_array_equal(run_inplace("ya_poly_backward", x, n, L),
+                                  orc.poly_backward(x, n, L))
+
+
+@pytest.mark.parametrize("logn", [4, 12, 15])
+def test_single_residue_with_modulus_index(logn):
+    n, cm = 1 << logn, 9
+    rng = np.random.default_rng(7)
+    x = rng.integers(0, 4 * orc.prime(cm), size=n, dtype=np.uint64)
+    say("ntt cm", logn, cm)
+    np.testing.assert_array_equal(run_inplace("ya_ntt_forward_lazy", x, n, cm),
+                                  orc.forward_lazy(x, n, cm))
+    np.testing.assert_array_equal(run_inplace("ya_ntt_forward", x, n, cm), orc.forward(x, n, cm))
+    y = rng.integers(0, 2 * orc.prime(cm), size=n, dtype=np.uint64)
+    np.testing.assert_array_equal(run_inplace("ya_ntt_backward", y, n, cm), orc.backward(y, n, cm))
+
+
+@pytest.mark.parametrize("logn", [3, 8, 13, 14, 15, 17])
+def test_host_seam(logn):
+    device()
+    n, cm = 1 << logn, 4
+    phi, phis, iphi, iphis, _ = orc.tables(n, cm)
+    rng = np.random.default_rng(11)
+    p = orc.prime(cm)
+    x = rng.integers(0, 4 * p, size=n, dtype=np.uint64)
+    say("seam fwd", logn)
+    y = x.copy()
+    capi.call("ya_negacylic_forward_lazy", capi.ptr(y), n, capi.ptr(phi), capi.ptr(phis), p)
+    np.testing.assert_array_equal(y, orc.negacyclic_forward_lazy(x, n, phi, phis, p))
+    x2 = rng.integers(0, 2 * p, size=n, dtype=np.uint64)
+    y = x2.copy()
+    say("seam bwd", logn)
+    capi.call("ya_negacylic_backward_lazy", capi.ptr(y), n, capi.ptr(iphi), capi.ptr(iphis), p)
+    np.testing.assert_array_equal(y, orc.negacyclic_backward_lazy(x2, n, iphi, iphis, p))
+    lib = capi.load()
+    # tables of another modulus are refused
+    assert lib.ya_negacylic_forward_lazy(capi.ptr(y), n, capi.ptr(iphi), capi.ptr(iphis), p) == 8
+
+
+@pytest.mark.parametrize("logn", [3, 11, 14, 16])
+def test_pointwise(logn):
+    n, L = 1 << logn, 3
+    rng = np.random.default_rng(300 + logn)
+    a = rand_bound(rng, n, L, 1, P)
+    b = rand_bound(rng, n, L, 1, P)
+    acc = rand_bound(rng, n, L, 1, P)
+    a[:5] = 0
+    say("pointwise", logn)
+    np.testing.assert_array_equal(run_binary("ya_poly_mul", a, b, n, L), orc.poly_mul(a, b, n, L))
+    np.testing.assert_array_equal(run_binary("ya_poly_add", a, b, n, L), orc.poly_add(a, b, n, L))
+    np.testing.assert_array_equal(run_binary("ya_poly_sub", a, b, n, L), orc.poly_sub(a, b, n, L))
+    np.testing.assert_array_equal(run_binary("ya_poly_add_product_of", a, b, n, L, out_init=acc),
+                                  orc.poly_add_product_of(acc, a, b, n, L))
+    neg = run_inplace("ya_poly_negate", a, n, L)
+    np.testing.assert_array_equal(neg, orc.poly_negate(a, n, L))
+    assert int(neg[0]) == P[0]
+
+
+def test_pointwise_in_place_alias():
+    device()
+    n, L = 1024, 2
+    rng = np.random.default_rng(3)
+    a = rand_bound(rng, n, L, 1, P)
+    b = rand_bound(rng, n, L, 1, P)
+    da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
+    capi.call("ya_poly_mul", da.ptr, da.ptr, db.ptr, n, L, None)    # operator*=
+    np.testing.assert_array_equal(da.download(), orc.poly_mul(a, b, n, L))
+
+
+@pytest.mark.parametrize("logn", LOGNS)
+def test_polymul(logn):
+    n, L = 1 << logn, (3 if logn <= 14 else 2)
+    rng = np.random.default_rng(400 + logn)
+    a = rand_bound(rng, n, L, 1, P)
+    b = rand_bound(rng, n, L, 1, P)
+    say("polymul", logn, L)
+    np.testing.assert_array_equal(polymul(a, b, n, L), orc.polymul(a, b, n, L))
+
+
+def test_polymul_c2_shape_uniform_times_gaussian():
+    n, L = 1 << 14, 8
+    a = capi.sample_uniform(n, L, 2026)
+    b = capi.sample_gaussian(n, L, 3.2, 2027)
+    say("polymul C2", n, L)
+    np.testing.assert_array_equal(polymul(a, b, n, L), orc.polymul(a, b, n, L))
+
+
+def test_polymul_c1_schoolbook():
+    n = 4096
+    a = capi.sample_uniform(n, 1, 11)
+    b = capi.sample_uniform(n, 1, 12)
+    say("polymul C1 schoolbook")
+    np.testing.assert_array_equal(polymul(a, b, n, 1), orc.schoolbook(a, b, n, 0))
+
+
+def test_golden_vectors():
+    for n in (8, 64, 1024):
+        for cm in (0, 7):
+            k = "n%d_cm%d" % (n, cm)
+            say("golden", k)
+            np.testing.assert_array_equal(run_inplace("ya_ntt_forward_lazy", GOLD["fwd_in_" + k], n, cm),
+                                          GOLD["fwd_lazy_out_" + k])
+            np.testing.assert_array_equal(run_inplace("ya_ntt_forward", GOLD["fwd_in_" + k], n, cm),
+                                          GOLD["fwd_out_" + k])
+            np.testing.assert_array_equal(run_inplace("ya_ntt_backward", GOLD["bwd_in_" + k], n, cm),
+                                          GOLD["bwd_out_" + k])
+    for n, L in ((4096, 1), (1024, 3)):
+        k = "n%d_L%d" % (n, L)
+        np.testing.assert_array_equal(polymul(GOLD["pm_a_" + k], GOLD["pm_b_" + k], n, L),
+                                      GOLD["pm_c_" + k])
+
+
+@pytest.mark.parametrize("logn", [3, 14, 16])
+def test_extreme_values(logn):
+    n, L = 1 << logn, 2
+    say("extremes", logn)
+    top4 = np.concatenate([np.full(n, 4 * P[r] - 1, dtype=np.uint64) for r in range(L)])
+    np.testing.assert_array_equal(run_inplace("ya_poly_forward_lazy", top4, n, L),
+                                  orc.poly_forward_lazy(top4, n, L))
+    top2 = np.concatenate([np.full(n, 2 * P[r] - 1, dtype=np.uint64) for r in range(L)])
+    np.testing.assert_array_equal(run_inplace("ya_poly_backward", top2, n, L),
+                                  orc.poly_backward(top2, n, L))
+    top1 = np.concatenate([np.full(n, P[r] - 1, dtype=np.uint64) for r in range(L)])
+    np.testing.assert_array_equal(polymul(top1, top1, n, L), orc.polymul(top1, top1, n, L))
+    z = np.zeros(n * L, dtype=np.uint64)
+    np.testing.assert_array_equal(polymul(z, top1, n, L), z)

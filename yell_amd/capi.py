@@ -65,9 +65,23 @@ _SIGS = {
     "ya_poly_negate": (i32, [vp, sz, sz, vp]),
     "ya_poly_add_product_of": (i32, [vp, vp, vp, sz, sz, vp]),
     "ya_poly_multiply": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya_poly_multiply_many": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_sample_uniform": (i32, [vp, sz, sz, u64]),
     "ya_sample_gaussian": (i32, [vp, sz, sz, ctypes.c_double, u64]),
+    "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
+    "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),
+    "ya_batch_plan_run": (i32, [vp, vp]),
+    "ya_batch_plan_run_graph": (i32, [vp, vp]),
+    "ya_batch_plan_info": (i32, [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+    "ya_batch_plan_destroy": (i32, [vp]),
+    "ya_batch_polymul": (i32, [vp, sz, vp]),
 }
+
+
+class PolymulJob(ctypes.Structure):
+    """struct ya_polymul_job."""
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("degree", ctypes.c_uint32), ("nmoduli", ctypes.c_uint32)]
 
 
 class YaError(RuntimeError):

@@ -1,0 +1,329 @@
+// ya_batch.hip -- batched poly-multiplication over jobs of uneven degree and
+// modulus count: one persistent kernel drains a queue of work items.
+//
+// A work item is one degree class (2^LB words per residue) and up to
+// 2^(10+LE-LB) residues (for LB <= 14) that one 1024-thread workgroup
+// transforms side by side: each residue gets a sub-group of 2^(LB-LE)
+// threads and its own slice of the workgroup's LDS, so a 2^10 residue and a
+// 2^14 residue keep every lane busy alike.  A 2^15 residue is one item: its
+// top layer runs over global memory and its two 2^14 halves through LDS.
+// Arithmetic and layer order are those of the plain kernels (the reference's
+// butterflies, lib/ntt.cpp:47-83), so each result equals ya_poly_multiply's.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/yell_amd.h"
+#include "ya_blocks.hpp"
+#include "ya_internal.hpp"
+
+namespace ya {
+
+constexpr int kBatchThreads = 1024;
+constexpr int kBatchMinLog = 7;
+constexpr int kBatchMaxLog = 15;
+
+struct BatchItem {
+  uint32_t logn;   // degree class
+  uint32_t le;     // log2 words per thread in a pass (3 or 4)
+  uint32_t first;  // first residue (index into the residue table)
+  uint32_t count;  // residues in this item
+};
+
+struct BatchRes {
+  const uint64_t *a;
+  const uint64_t *b;
+  uint64_t *c;
+  const ModDesc *d;
+};
+
+struct BatchQueue {
+  uint32_t next;  // next item to claim
+  uint32_t done;  // workgroups that have drained the queue
+};
+
+// Sub-grouped fused poly-multiplication of up to 2^(10+LE-LB) residues.
+template <int LB, int LE>
+__device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes *__restrict__ res,
+                                            uint64_t *lds, uint32_t t) {
+  constexpr uint32_t TS = 1u << (LB - LE);  // threads per residue
+  const uint32_t sub = t / TS, tl = t % TS;
+  const bool active = sub < it.count;
+  const BatchRes r = res[it.first + (active ? sub : 0u)];
+  const ModDesc d = *r.d;
+  uint64_t *my_lds = lds + sub * LdsWords<LB>::value;
+  uint64_t x[1 << LE], y[1 << LE];
+  fwd_block<LB, LE>(x, r.a, my_lds, d.fwd, LB, 0u, d.p, tl);
+  if (active) gmem_store<0, LE>(r.c, x, tl);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) y[e] = x[e];
+  fwd_block<LB, LE>(x, r.b, my_lds, d.fwd, LB, 0u, d.p, tl);
+  if (active) gmem_load<0, LE>(r.c, y, tl);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+  inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, d.inv, LB, 0u, d, tl);
+}
+
+// Top layer (index bit 14) of a 2^15 residue over global memory, src -> dst.
+__device__ __forceinline__ void top15_fwd(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+                                          const ModDesc &d, uint32_t t) {
+  const Tw w = load_tw(d.fwd + 1);  // (N >> 15) + 0
+  const uint64_t p2 = d.p << 1;
+#pragma unroll 4
+  for (uint32_t q = 0; q < 16; ++q) {
+    const uint32_t i = t + q * kBatchThreads;
+    uint64_t x0 = src[i], x1 = src[i + (1u << 14)];
+    ct_butterfly(x0, x1, w, d.p, p2);
+    dst[i] = x0;
+    dst[i + (1u << 14)] = x1;
+  }
+}
+
+__device__ __forceinline__ void top15_inv_merged(uint64_t *__restrict__ c, const ModDesc &d,
+                                                 uint32_t t) {
+  const uint64_t p = d.p, p2 = p << 1;
+#pragma unroll 4
+  for (uint32_t q = 0; q < 16; ++q) {
+    const uint32_t i = t + q * kBatchThreads;
+    const uint64_t u = c[i], v = c[i + (1u << 14)];
+    c[i] = mulmod_shoup(u + v, d.inv_n, d.inv_n_s, p);
+    c[i + (1u << 14)] = mulmod_shoup(u - v + p2, d.inv_n_phi, d.inv_n_phi_s, p);
+  }
+}
+
+// One 2^15 residue; `scr` is this workgroup's private 2^15-word scratch.
+__device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__restrict__ res,
+                                         uint64_t *lds, uint64_t *__restrict__ scr, uint32_t t) {
+  constexpr int LE = 4;
+  const BatchRes r = res[it.first];
+  const ModDesc d = *r.d;
+  uint64_t x[1 << LE], y[1 << LE];
+  top15_fwd(r.a, scr, d, t);  // forward(a): bit 14, then each half
+  __syncthreads();
+  for (uint32_t h = 0; h < 2; ++h) {
+    fwd_block<14, LE>(x, scr + (h << 14), lds, d.fwd, 15, h << 14, d.p, t);
+    gmem_store<0, LE>(scr + (h << 14), x, t);
+  }
+  top15_fwd(r.b, r.c, d, t);  // forward(b) into c
+  __syncthreads();
+  for (uint32_t h = 0; h < 2; ++h) {
+    fwd_block<14, LE>(x, r.c + (h << 14), lds, d.fwd, 15, h << 14, d.p, t);
+    gmem_load<0, LE>(scr + (h << 14), y, t);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+    inv_block<14, LE, TOP_GS>(x, r.c + (h << 14), lds, d.inv, 15, h << 14, d, t);
+  }
+  __syncthreads();
+  top15_inv_merged(r.c, d, t);
+}
+
+__global__ void __launch_bounds__(kBatchThreads)
+k_batch_polymul(const BatchItem *__restrict__ items, uint32_t nitems,
+                const BatchRes *__restrict__ res, BatchQueue *q, uint64_t *__restrict__ scratch) {
+  __shared__ uint64_t lds[LdsWords<14>::value];
+  __shared__ uint32_t s_item;
+  const uint32_t t = threadIdx.x;
+  for (;;) {
+    if (t == 0) s_item = atomicAdd(&q->next, 1u);
+    __syncthreads();
+    const uint32_t k = s_item;
+    __syncthreads();
+    if (k >= nitems) break;
+    const BatchItem it = items[k];
+    switch (it.logn * 8 + it.le) {
+      case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, t); break;
+      case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, t); break;
+      case 8 * 8 + 3: batch_small<8, 3>(it, res, lds, t); break;
+      case 8 * 8 + 4: batch_small<8, 4>(it, res, lds, t); break;
+      case 9 * 8 + 3: batch_small<9, 3>(it, res, lds, t); break;
+      case 9 * 8 + 4: batch_small<9, 4>(it, res, lds, t); break;
+      case 10 * 8 + 3: batch_small<10, 3>(it, res, lds, t); break;
+      case 10 * 8 + 4: batch_small<10, 4>(it, res, lds, t); break;
+      case 11 * 8 + 3: batch_small<11, 3>(it, res, lds, t); break;
+      case 11 * 8 + 4: batch_small<11, 4>(it, res, lds, t); break;
+      case 12 * 8 + 3: batch_small<12, 3>(it, res, lds, t); break;
+      case 12 * 8 + 4: batch_small<12, 4>(it, res, lds, t); break;
+      case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, t); break;
+      case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, t); break;
+      case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, t); break;
+      case 15 * 8 + 4: batch_15(it, res, lds, scratch + ((size_t)blockIdx.x << 15), t); break;
+      default: break;  // rejected by the host
+    }
+    __syncthreads();  // LDS of this item is dead before the next item's first store
+  }
+  // The last workgroup to leave resets the queue for the next launch.
+  if (t == 0) {
+    __threadfence();
+    if (atomicAdd(&q->done, 1u) == gridDim.x - 1) {
+      q->next = 0;
+      q->done = 0;
+      __threadfence();
+    }
+  }
+}
+
+}  // namespace ya
+
+// ===========================================================================
+using namespace ya;
+
+struct ya_batch_plan {
+  int device = 0;
+  size_t nitems = 0, nres = 0;
+  unsigned grid = 0;
+  BatchItem *d_items = nullptr;
+  BatchRes *d_res = nullptr;
+  BatchQueue *d_q = nullptr;
+  uint64_t *d_scratch = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double total_polymuls = 0;
+};
+
+namespace {
+
+int plan_free(ya_batch_plan *pl) {
+  if (!pl) return YA_OK;
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  (void)hipFree(pl->d_items);
+  (void)hipFree(pl->d_res);
+  (void)hipFree(pl->d_q);
+  (void)hipFree(pl->d_scratch);
+  delete pl;
+  return YA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t njobs) {
+  if (!out || (!jobs && njobs)) return fail(YA_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (njobs == 0) return fail(YA_ERR_INVALID, "empty batch");
+  DeviceState *st;
+  int rc = current_device(&st);
+  if (rc) return rc;
+  // Validate and group residues by degree class.
+  std::vector<std::vector<BatchRes>> by_class(kBatchMaxLog + 1);
+  bool has15 = false;
+  for (size_t j = 0; j < njobs; ++j) {
+    const ya_polymul_job &jb = jobs[j];
+    int logn;
+    if ((rc = check_shape(jb.degree, 0, jb.nmoduli, &logn))) return rc;
+    if (logn < kBatchMinLog || logn > kBatchMaxLog)
+      return fail(YA_ERR_UNSUPPORTED, "job %zu: degree 2^%d outside the batched range 2^%d..2^%d",
+                  j, logn, kBatchMinLog, kBatchMaxLog);
+    if (!jb.a || !jb.b || !jb.c) return fail(YA_ERR_INVALID, "job %zu: null buffer", j);
+    const size_t words = (size_t)jb.degree * jb.nmoduli;
+    if ((jb.c < jb.a + words && jb.a < jb.c + words) || (jb.c < jb.b + words && jb.b < jb.c + words))
+      return fail(YA_ERR_INVALID, "job %zu: c overlaps a or b", j);
+    if ((rc = ensure_tables(st, logn, 0, jb.nmoduli))) return rc;
+    for (uint32_t r = 0; r < jb.nmoduli; ++r) {
+      const size_t off = (size_t)r * jb.degree;
+      by_class[logn].push_back(BatchRes{jb.a + off, jb.b + off, jb.c + off, desc_ptr(st, logn, r)});
+    }
+    has15 |= logn == 15;
+  }
+  // Items: largest class first (longest items start first).
+  std::vector<BatchItem> items;
+  std::vector<BatchRes> res;
+  for (int lg = kBatchMaxLog; lg >= kBatchMinLog; --lg) {
+    const auto &v = by_class[lg];
+    if (v.empty()) continue;
+    int le = lg >= 14 ? 4 : tuned_shape(lg, 2).le;
+    le = (le == 3) ? 3 : 4;
+    const uint32_t per = lg >= 14 ? 1u : (1u << (10 + le - lg));
+    for (size_t i = 0; i < v.size(); i += per) {
+      const uint32_t cnt = (uint32_t)std::min<size_t>(per, v.size() - i);
+      items.push_back(BatchItem{(uint32_t)lg, (uint32_t)le, (uint32_t)res.size(), cnt});
+      res.insert(res.end(), v.begin() + i, v.begin() + i + cnt);
+    }
+  }
+  auto *pl = new ya_batch_plan();
+  pl->device = 0;
+  (void)hipGetDevice(&pl->device);
+  pl->nitems = items.size();
+  pl->nres = res.size();
+  int cus = 0;
+  hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
+  if (e != hipSuccess || cus <= 0) cus = 256;
+  pl->grid = (unsigned)std::min<size_t>((size_t)cus, items.size());
+  e = hipMalloc(&pl->d_items, items.size() * sizeof(BatchItem));
+  if (e == hipSuccess) e = hipMalloc(&pl->d_res, res.size() * sizeof(BatchRes));
+  if (e == hipSuccess) e = hipMalloc(&pl->d_q, sizeof(BatchQueue));
+  if (e == hipSuccess && has15) e = hipMalloc(&pl->d_scratch, ((size_t)pl->grid << 15) * 8);
+  if (e != hipSuccess) {
+    plan_free(pl);
+    return fail(YA_ERR_ALLOC, "batch plan allocation: %s", hipGetErrorString(e));
+  }
+  BatchQueue q0{0, 0};
+  e = hipMemcpy(pl->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(pl->d_res, res.data(), res.size() * sizeof(BatchRes), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(pl->d_q, &q0, sizeof q0, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    plan_free(pl);
+    return hip_fail(e, "batch plan upload");
+  }
+  pl->total_polymuls = (double)njobs;
+  *out = pl;
+  return YA_OK;
+}
+
+int ya_batch_plan_run(ya_batch_plan *pl, void *stream) {
+  if (!pl) return fail(YA_ERR_INVALID, "null plan");
+  hipLaunchKernelGGL(k_batch_polymul, dim3(pl->grid), dim3(kBatchThreads), 0, (hipStream_t)stream,
+                     pl->d_items, (uint32_t)pl->nitems, pl->d_res, pl->d_q, pl->d_scratch);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? YA_OK : hip_fail(e, "batch launch");
+}
+
+int ya_batch_plan_run_graph(ya_batch_plan *pl, void *stream) {
+  if (!pl) return fail(YA_ERR_INVALID, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (!pl->exec) {
+    hipStream_t cap;
+    e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "capture stream");
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_batch_polymul, dim3(pl->grid), dim3(kBatchThreads), 0, cap, pl->d_items,
+                         (uint32_t)pl->nitems, pl->d_res, pl->d_q, pl->d_scratch);
+      e = hipGetLastError();
+      hipError_t e2 = hipStreamEndCapture(cap, &pl->graph);
+      if (e == hipSuccess) e = e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cap);
+    if (e != hipSuccess) return hip_fail(e, "graph capture");
+  }
+  e = hipGraphLaunch(pl->exec, s);
+  return e == hipSuccess ? YA_OK : hip_fail(e, "graph launch");
+}
+
+int ya_batch_plan_destroy(ya_batch_plan *pl) { return plan_free(pl); }
+
+int ya_batch_plan_info(const ya_batch_plan *pl, size_t *items, size_t *residues, size_t *workgroups) {
+  if (!pl) return fail(YA_ERR_INVALID, "null plan");
+  if (items) *items = pl->nitems;
+  if (residues) *residues = pl->nres;
+  if (workgroups) *workgroups = pl->grid;
+  return YA_OK;
+}
+
+int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream) {
+  ya_batch_plan *pl = nullptr;
+  int rc = ya_batch_plan_create(&pl, jobs, njobs);
+  if (rc) return rc;
+  rc = ya_batch_plan_run(pl, stream);
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  plan_free(pl);
+  if (rc) return rc;
+  return e == hipSuccess ? YA_OK : hip_fail(e, "batch sync");
+}
+
+}  // extern "C"

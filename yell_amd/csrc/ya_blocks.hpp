@@ -87,7 +87,7 @@ __device__ __forceinline__ void inv_block(uint64_t (&x)[1 << LE], uint64_t *__re
       __syncthreads();
       lds_store<WLO, LE>(lds, x, t);
       __syncthreads();
-    } else {
+    } else if (dst != nullptr) {  // null: a sub-group with no residue of its own
       gmem_store<WLO, LE>(dst, x, t);
     }
   });

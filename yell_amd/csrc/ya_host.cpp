@@ -570,42 +570,58 @@ int ya_poly_add_product_of(uint64_t *acc, const uint64_t *a, const uint64_t *b, 
   return pw_common(PW_MULADD, acc, a, b, degree, nmoduli, stream);
 }
 
-int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
-                     size_t nmoduli, void *stream) {
+static int polymul_impl(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                        size_t nmoduli, size_t count, void *stream) {
   int logn;
   int rc = check_shape(degree, 0, nmoduli, &logn);
   if (rc) return rc;
   if (!a || !b || !c) return fail(YA_ERR_INVALID, "null buffer");
-  if (c == a || c == b) return fail(YA_ERR_INVALID, "c must not alias a or b");
+  if (count == 0 || count * nmoduli > (1u << 30)) return fail(YA_ERR_INVALID, "bad count %zu", count);
+  const size_t words = degree * nmoduli * count;
+  if ((c < a + words && a < c + words) || (c < b + words && b < c + words))
+    return fail(YA_ERR_INVALID, "c must not overlap a or b");
   DeviceState *st;
   if ((rc = current_device(&st))) return rc;
   if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
   const ModDesc *d = desc_ptr(st, logn, 0);
   hipStream_t s = (hipStream_t)stream;
   if (logn <= kMaxBlockLog) {
-    hipError_t e = launch_polymul(c, a, b, d, logn, nmoduli, tuned_shape(logn, 1), s);
+    hipError_t e = launch_polymul(c, a, b, d, logn, nmoduli * count, nmoduli, tuned_shape(logn, 1), s);
     return e == hipSuccess ? YA_OK : hip_fail(e, "polymul launch");
   }
   // Large rings: forward(a) into c, forward(b) into stream-ordered scratch,
-  // c = c * scratch, backward(c).
-  const size_t words = degree * nmoduli;
+  // c = c * scratch, backward(c); polynomial by polynomial.
+  const size_t pw = degree * nmoduli;
   uint64_t *tmp = nullptr;
-  hipError_t e = hipMallocAsync((void **)&tmp, words * 8, s);
+  hipError_t e = hipMallocAsync((void **)&tmp, pw * 8, s);
   if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMallocAsync: %s", hipGetErrorString(e));
   const NttShape sh = tuned_shape(logn, 0);
-  e = hipMemcpyAsync(c, a, words * 8, hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(tmp, b, words * 8, hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess) e = launch_forward(c, d, logn, nmoduli, true, sh, s);
-  if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, true, sh, s);
-  for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
-    const size_t nr = std::min<size_t>(65535, nmoduli - r0);
-    e = launch_pointwise(PW_MUL, c + r0 * degree, c + r0 * degree, tmp + r0 * degree, d + r0,
-                         degree, nr, s);
+  for (size_t k = 0; e == hipSuccess && k < count; ++k) {
+    uint64_t *ck = c + k * pw;
+    e = hipMemcpyAsync(ck, a + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp, b + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = launch_forward(ck, d, logn, nmoduli, true, sh, s);
+    if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, true, sh, s);
+    for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
+      const size_t nr = std::min<size_t>(65535, nmoduli - r0);
+      e = launch_pointwise(PW_MUL, ck + r0 * degree, ck + r0 * degree, tmp + r0 * degree, d + r0,
+                           degree, nr, s);
+    }
+    if (e == hipSuccess) e = launch_backward(ck, d, logn, nmoduli, sh, s);
   }
-  if (e == hipSuccess) e = launch_backward(c, d, logn, nmoduli, sh, s);
   hipError_t ef = hipFreeAsync(tmp, s);
   if (e == hipSuccess) e = ef;
   return e == hipSuccess ? YA_OK : hip_fail(e, "polymul (large ring)");
+}
+
+int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                     size_t nmoduli, void *stream) {
+  return polymul_impl(c, a, b, degree, nmoduli, 1, stream);
+}
+
+int ya_poly_multiply_many(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                          size_t nmoduli, size_t count, void *stream) {
+  return polymul_impl(c, a, b, degree, nmoduli, count, stream);
 }
 
 }  // extern "C"

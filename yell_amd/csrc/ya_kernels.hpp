@@ -33,9 +33,11 @@ hipError_t launch_inverse_lazy(uint64_t *data, const ModDesc *descs, int logn, s
                                hipStream_t st);
 
 // c = backward(forward(a) (*) forward(b)) for `nres` residues (a, b, c share
-// the residue-major layout); c may not alias a or b.
+// the residue-major layout); residue r uses descs[r % nmod] (nres is a
+// multiple of nmod: nres / nmod polynomials of nmod residues each); c may not
+// alias a or b.  Rings up to 2^14 only.
 hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
-                          int logn, size_t nres, NttShape shape, hipStream_t st);
+                          int logn, size_t nres, size_t nmod, NttShape shape, hipStream_t st);
 
 enum PwOp { PW_MUL = 0, PW_ADD = 1, PW_SUB = 2, PW_NEGATE = 3, PW_MULADD = 4, PW_CORRECT = 5 };
 

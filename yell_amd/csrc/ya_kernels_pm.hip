@@ -9,10 +9,10 @@ namespace ya {
 template <int LB, int LE>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
-          const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs) {
+          const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
   const uint32_t res = blockIdx.x;
-  const ModDesc d = descs[res];
+  const ModDesc d = descs[res % nmod];
   const size_t off = (size_t)res << LB;
   const uint32_t t = threadIdx.x;
   uint64_t x[1 << LE], y[1 << LE];
@@ -29,20 +29,21 @@ namespace {
 
 template <int LB, int LE>
 hipError_t polymul_lb(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
-                      size_t nres, hipStream_t st) {
+                      size_t nres, size_t nmod, hipStream_t st) {
   hipLaunchKernelGGL((k_polymul<LB, LE>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c, a,
-                     b, descs);
+                     b, descs, (uint32_t)nmod);
   return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
-                          int logn, size_t nres, NttShape shape, hipStream_t st) {
-  if (logn < kMinLogN || logn > kMaxLogN || nres == 0) return hipErrorInvalidValue;
+                          int logn, size_t nres, size_t nmod, NttShape shape, hipStream_t st) {
+  if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0 || nres % nmod)
+    return hipErrorInvalidValue;
   if (logn > kMaxBlockLog) return hipErrorNotSupported;  // composed by the caller
   const int le = clamp_le(logn, shape.le);
-#define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, st)
+#define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, nmod, st)
   YA_LB_SWITCH(logn, le, YA_PM_CALL)
 #undef YA_PM_CALL
 }

@@ -40,6 +40,24 @@ struct ModDesc {
 
 __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
 
+// Twiddle load through a global-address-space pointer (the descriptor's
+// pointers are generic; without the cast the loads become FLAT and also
+// count against the LDS queue).
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Tw load_tw(const Tw *p) {
+  const u64x2 v = *(const __attribute__((address_space(1))) u64x2 *)p;
+  return Tw{v.x, v.y};
+}
+
+// Twiddle base for a pass whose window starts at WLO: the bits of gbase are
+// thread bits >= WLO, so for WLO >= 6 they are equal across a 64-lane wave and
+// the value is made explicitly wave-uniform (scalar twiddle loads).
+template <int WLO>
+__device__ __forceinline__ uint32_t uniform_if_wave(uint32_t v) {
+  if constexpr (WLO >= 6) return __builtin_amdgcn_readfirstlane(v);
+  return v;
+}
+
 template <int LB>
 struct LdsWords {
   static constexpr uint32_t value = (1u << LB) + ((1u << LB) >> 4);
@@ -95,10 +113,10 @@ __device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__r
 #pragma unroll
   for (int b = HI - 1; b >= LO; --b) {
     const int j = b - WLO;
-    const uint32_t toff = ((1u << nlog) >> (b + 1)) + (gbase >> (b + 1));
+    const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
 #pragma unroll
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
-      const Tw t = tw[toff + g];
+      const Tw t = load_tw(tw + toff + g);
 #pragma unroll
       for (int k = 0; k < (1 << j); ++k) {
         const int e0 = (g << (j + 1)) | k;
@@ -116,10 +134,10 @@ __device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__r
 #pragma unroll
   for (int b = LO; b < HI; ++b) {
     const int j = b - WLO;
-    const uint32_t toff = ((1u << nlog) >> (b + 1)) + (gbase >> (b + 1));
+    const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
 #pragma unroll
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
-      const Tw t = tw[toff + g];
+      const Tw t = load_tw(tw + toff + g);
 #pragma unroll
       for (int k = 0; k < (1 << j); ++k) {
         const int e0 = (g << (j + 1)) | k;
