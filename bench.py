@@ -114,6 +114,60 @@ def cpu_baseline(a0, b0, n, L, gpu_c0, seconds):
             "host_cpus": os.cpu_count()}, bool(np.array_equal(want, gpu_c0))
 
 
+def make_plan(jobs_spec):
+    """jobs_spec: list of (a_ptr, b_ptr, c_ptr, n, L) -> plan handle."""
+    arr = (capi.PolymulJob * len(jobs_spec))()
+    for k, (a, b, c, n, L) in enumerate(jobs_spec):
+        arr[k] = capi.PolymulJob(a, b, c, n, L)
+    plan = ctypes.c_void_p()
+    capi.call("ya_batch_plan_create", ctypes.byref(plan), ctypes.cast(arr, ctypes.c_void_p),
+              len(jobs_spec))
+    return plan
+
+
+def timed(ev, stream, fn, steps):
+    capi.call("ya_event_record", ev.a, stream)
+    for _ in range(steps):
+        fn()
+    capi.call("ya_event_record", ev.b, stream)
+    return ev.elapsed_ms() / steps
+
+
+def c3_bench(stream, ev, steps, seed=2026):
+    """1024 uneven jobs (degrees 2^10..2^15, 1..12 moduli) in one persistent
+    launch (hipGraph), against one ya_poly_multiply launch per job."""
+    rng = np.random.default_rng(seed)
+    specs = [(1 << int(rng.integers(10, 16)), int(rng.integers(1, 13))) for _ in range(1024)]
+    total = sum(n * L for n, L in specs)
+    A, B, C = (capi.DeviceBuffer(total) for _ in range(3))
+    host = np.empty(total, dtype=np.uint64)
+    off, jobs = 0, []
+    for k, (n, L) in enumerate(specs):
+        host[off:off + n * L] = capi.sample_uniform(n, L, seed + k)
+        jobs.append((A.ptr + 8 * off, B.ptr + 8 * off, C.ptr + 8 * off, n, L))
+        off += n * L
+    A.upload(host)
+    B.upload(host[::-1].copy())
+    for n in sorted({n for n, _ in specs}):
+        capi.call("ya_ntt_init_tables", n, 12)
+    plan = make_plan(jobs)
+    run = lambda: capi.call("ya_batch_plan_run_graph", plan, stream)
+    run()
+    capi.call("ya_stream_synchronize", stream)
+    ms = timed(ev, stream, run, steps)
+    plain = lambda: [capi.call("ya_poly_multiply", c, a, b, n, L, stream) for a, b, c, n, L in jobs]
+    plain()
+    ms_plain = timed(ev, stream, plain, 1)
+    capi.load().ya_batch_plan_destroy(plan)
+    for d in (A, B, C):
+        d.close()
+    return {"jobs": len(specs), "words_per_operand": total,
+            "value": len(specs) / (ms / 1e3), "unit": "poly-mults/s", "ms_per_batch": ms,
+            "plain_value": len(specs) / (ms_plain / 1e3), "ms_plain": ms_plain,
+            "batched_over_plain": ms_plain / ms,
+            "hbm_GBps_algorithmic": 3 * 8 * total / (ms / 1e3) / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +177,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch, probes and tuner")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and this path does not "
@@ -133,15 +190,44 @@ def main():
     w = n * L
     capi.load()
     capi.call("ya_init", 0)
+    stream = ctypes.c_void_p()
+    capi.call("ya_stream_create", ctypes.byref(stream))
+    ev = Events()
+    extra = {}
+    if args.copy_calib_mb:
+        nw = args.copy_calib_mb * (1 << 20) // 8
+        s_, d_ = capi.DeviceBuffer(nw), capi.DeviceBuffer(nw)
+        capi.call("ya_memcpy_dtod", d_.ptr, s_.ptr, nw * 8, stream)
+        capi.call("ya_stream_synchronize", stream)
+        s_.close()
+        d_.close()
+    if not args.no_extra:
+        log("autotune 2^10..2^15")
+        rep = (ctypes.c_double * 24)()
+        capi.call("ya_autotune", 10, 15, rep)
+        extra["autotune"] = {"classes": list(range(10, 16)),
+                             "ms_batched_radix8_radix16_fused_radix8_radix16": [
+                                 [round(rep[4 * k + j], 4) for j in range(4)] for k in range(6)],
+                             "chosen_batched_le": [capi.load().ya_tune_get(2, lg) for lg in range(10, 16)]}
+        probes = {}
+        for kind, name in ((0, "mad_u64_u32_per_s"), (1, "mul_lo_u32_per_s"), (2, "ct_butterflies_per_s")):
+            v = ctypes.c_double()
+            capi.call("ya_probe_valu", kind, ctypes.byref(v))
+            probes[name] = v.value
+        extra["valu_probe"] = probes
+
     log("inputs: batch", B, "N", n, "L", L)
     a, b = make_inputs(n, L, B, seed=1234)
     da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
     dc = capi.DeviceBuffer(B * w)
-    stream = ctypes.c_void_p()
-    capi.call("ya_stream_create", ctypes.byref(stream))
     capi.call("ya_ntt_init_tables", n, L)
+    plan = make_plan([(da.ptr + k * w * 8, db.ptr + k * w * 8, dc.ptr + k * w * 8, n, L)
+                      for k in range(B)])
 
     def batched():
+        capi.call("ya_batch_plan_run_graph", plan, stream)
+
+    def grid_form():
         capi.call("ya_poly_multiply_many", dc.ptr, da.ptr, db.ptr, n, L, B, stream)
 
     def plain():
@@ -149,30 +235,22 @@ def main():
             off = k * w * 8
             capi.call("ya_poly_multiply", dc.ptr + off, da.ptr + off, db.ptr + off, n, L, stream)
 
-    ev = Events()
     for _ in range(args.warmup):
         batched()
     capi.call("ya_stream_synchronize", stream)
     log("timing batched:", args.steps, "steps")
     t0 = time.perf_counter()
-    capi.call("ya_event_record", ev.a, stream)
-    for _ in range(args.steps):
-        batched()
-    capi.call("ya_event_record", ev.b, stream)
-    ms = ev.elapsed_ms()
+    ms_step = timed(ev, stream, batched, args.steps)
     wall = time.perf_counter() - t0
-    ms_step = ms / args.steps
     value = B / (ms_step / 1e3)
     c0 = dc.download(w)
-
+    grid_form()
+    ms_grid = timed(ev, stream, grid_form, max(1, args.steps // 2))
+    c0_grid = dc.download(w)
     plain()
     capi.call("ya_stream_synchronize", stream)
     log("timing plain:", args.plain_steps, "steps of", B, "launches")
-    capi.call("ya_event_record", ev.a, stream)
-    for _ in range(args.plain_steps):
-        plain()
-    capi.call("ya_event_record", ev.b, stream)
-    ms_plain = ev.elapsed_ms() / args.plain_steps
+    ms_plain = timed(ev, stream, plain, args.plain_steps)
     plain_value = B / (ms_plain / 1e3)
     c0_plain = dc.download(w)
 
@@ -185,24 +263,39 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (seeded uniform x Gaussian sigma=3.2 operands)",
         "config": {"workload": "RNS poly-multiplication N=16384 L=8 (P[0..7]), a uniform, "
-                               "b Gaussian sigma=3.2; %d polys per batched launch" % B,
+                               "b Gaussian sigma=3.2; %d polys per batched launch "
+                               "(persistent kernel replayed as a hipGraph)" % B,
                    "degree": n, "nmoduli": L, "batch": B},
         "plain": {"value": plain_value, "unit": "poly-mults/s", "ms_per_step": ms_plain,
                   "launches_per_step": B},
         "batched_over_plain": value / plain_value,
+        "grid_form": {"value": B / (ms_grid / 1e3), "ms_per_step": ms_grid,
+                      "kernel": "k_polymul<14,4>, one workgroup per residue"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_polymul<14,4>", "bytes_per_polymul": bytes_per},
+                     "kernel": "k_batch_polymul", "bytes_per_polymul": bytes_per,
+                     "polymuls_per_launch": B},
         "modmul_rate": {"modmuls_per_polymul": mm,
                         "achieved_T_per_s": mm * B / (ms_step / 1e3) / 1e12},
         "timed_wall_s": wall,
     }
+    out.update(extra)
+    if "valu_probe" in extra:
+        pk = extra["valu_probe"]["ct_butterflies_per_s"]
+        out["modmul_rate"]["butterfly_probe_T_per_s"] = pk / 1e12
+        out["modmul_rate"]["frac_of_probe"] = mm * B / (ms_step / 1e3) / pk
+    if not args.no_extra:
+        log("C3 uneven batch")
+        out["c3_uneven_batch"] = c3_bench(stream, ev, max(2, args.steps // 4))
+    same = bool(np.array_equal(c0, c0_plain) and np.array_equal(c0, c0_grid))
+    out["batched_equals_plain"] = same
     if not args.no_cpu:
         log("cpu baseline (~%.0f s)" % args.cpu_seconds)
         cb, exact = cpu_baseline(a[:w].copy(), b[:w].copy(), n, L, c0, args.cpu_seconds)
         out["cpu_baseline"] = cb
-        out["bit_exact_vs_cpu"] = exact and bool(np.array_equal(c0, c0_plain))
+        out["bit_exact_vs_cpu"] = exact and same
         out["gpu_over_cpu"] = value / cb["value"]
+    capi.load().ya_batch_plan_destroy(plan)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,94 @@
+// yell/ntt.hpp -- drop-in for the reference's yell::ntt<N>
+// (include/yell/ntt.hpp:6-66): the same static interface, with the tables
+// resident on the GPU and the transforms run by libyell_amd (yell_amd.h).
+// op may be device memory (a poly's ptr_at) -- transformed in place on the
+// device -- or host memory, which is staged through a device buffer.
+// Failures throw std::runtime_error (the reference asserts).
+#pragma once
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "yell/params.hpp"
+#include "yell_amd.h"
+
+namespace yell {
+
+inline void ya_check(int rc, const char *what) {
+  if (rc != YA_OK)
+    throw std::runtime_error(std::string(what) + ": " + ya_status_string(rc) + ": " + ya_last_error());
+}
+
+inline bool is_device_ptr(const void *p) {
+  int d = 0;
+  ya_check(ya_pointer_is_device(p, &d), "ya_pointer_is_device");
+  return d != 0;
+}
+
+template <size_t degree_>
+class ntt {
+  static constexpr size_t degree = degree_;
+  using T = params::value_type;
+
+ public:
+  // The reference's ntt_precomputed (details/ntt.hpp:27-77), a host copy of
+  // the device-resident table of (degree, cm).
+  struct ntt_precomputed {
+    std::vector<T> phiTbl, shoupPhiTbl, invphiTbl, shoupInvphiTbl;
+    T invDegree, shoupInvDegree, invphiInvDegree, shoupInvphiInvDegree;
+  };
+
+  static const ntt_precomputed *init_table(size_t cm) {
+    static std::mutex mu;
+    static std::map<size_t, std::unique_ptr<ntt_precomputed>> cache;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(cm);
+    if (it != cache.end()) return it->second.get();
+    std::unique_ptr<ntt_precomputed> t(new ntt_precomputed());
+    t->phiTbl.resize(degree);
+    t->shoupPhiTbl.resize(degree);
+    t->invphiTbl.resize(degree);
+    t->shoupInvphiTbl.resize(degree);
+    T sc[4];
+    ya_check(ya_ntt_table_download(degree, cm, t->phiTbl.data(), t->shoupPhiTbl.data(),
+                                   t->invphiTbl.data(), t->shoupInvphiTbl.data(), sc),
+             "ya_ntt_table_download");
+    t->invDegree = sc[0];
+    t->shoupInvDegree = sc[1];
+    t->invphiInvDegree = sc[2];
+    t->shoupInvphiInvDegree = sc[3];
+    auto *raw = t.get();
+    cache.emplace(cm, std::move(t));
+    return raw;
+  }
+
+  static void init_ntt_tables(size_t K) { ya_check(ya_ntt_init_tables(degree, K), "ya_ntt_init_tables"); }
+
+  static void forward(T *op, size_t cm) { run(op, cm, 0); }
+  static void forward_lazy(T *op, size_t cm) { run(op, cm, 1); }
+  static void backward(T *op, size_t cm) { run(op, cm, 2); }
+
+ private:
+  static void run(T *op, size_t cm, int dir) {
+    auto call = [&](T *x) {
+      if (dir == 0) ya_check(ya_ntt_forward(x, degree, cm, nullptr), "ya_ntt_forward");
+      else if (dir == 1) ya_check(ya_ntt_forward_lazy(x, degree, cm, nullptr), "ya_ntt_forward_lazy");
+      else ya_check(ya_ntt_backward(x, degree, cm, nullptr), "ya_ntt_backward");
+    };
+    if (is_device_ptr(op)) {
+      call(op);
+      return;
+    }
+    void *d = nullptr;
+    ya_check(ya_malloc(&d, degree * sizeof(T)), "ya_malloc");
+    std::unique_ptr<void, int (*)(void *)> guard(d, ya_free);
+    ya_check(ya_memcpy_htod(d, op, degree * sizeof(T), nullptr), "ya_memcpy_htod");
+    call(static_cast<T *>(d));
+    ya_check(ya_memcpy_dtoh(op, d, degree * sizeof(T), nullptr), "ya_memcpy_dtoh");
+  }
+};
+
+}  // namespace yell

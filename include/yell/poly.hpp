@@ -1,0 +1,256 @@
+// yell/poly.hpp -- drop-in for the reference's yell::poly<N>
+// (include/yell/poly.hpp:43-168, include/yell/details/poly.hpp) whose
+// residues live in GPU memory.
+//
+// Same constructors, samplers, NTT methods, operators and accessors as the
+// reference; the differences a caller sees:
+//   * storage is one residue-major device buffer [L][N] (ptr_at(cm) returns a
+//     device pointer; data() the base); raw_data() returns the residue
+//     pointers as a vector instead of T**;
+//   * element access operator()(cm, i) reads / writes one word across PCIe;
+//     to_host() / from_host() move whole polynomials;
+//   * samplers are seeded (yell::set_seed); gaussian takes sigma directly
+//     (the reference wraps an MPFR-built FastGaussianNoise table);
+//   * poly2mpz (GMP CRT lift) is not provided;
+//   * failures throw std::runtime_error (the reference asserts).
+// Every arithmetic result is bit-identical to the reference's.
+#pragma once
+#include <atomic>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "yell/ntt.hpp"
+#include "yell/ops.hpp"
+#include "yell/params.hpp"
+#include "yell_amd.h"
+
+namespace yell {
+
+struct uniform {};
+struct hwt_dist {
+  uint32_t hwt;
+  explicit hwt_dist(uint32_t h) : hwt(h) {}
+};
+struct ZO_dist {
+  uint8_t rho;
+  explicit ZO_dist(uint8_t r = 0x7F) : rho(r) {}
+};
+struct gaussian {
+  double sigma;
+  uint64_t amplifier;
+  explicit gaussian(double s = 3.2, uint64_t amp = 1) : sigma(s), amplifier(amp) {}
+};
+
+namespace detail {
+inline std::atomic<uint64_t> &seed_counter() {
+  static std::atomic<uint64_t> s{0x5eed5eed5eedULL};
+  return s;
+}
+}  // namespace detail
+
+// Seeds the samplers used by poly constructors / set(); each sampling call
+// draws the next seed, so a program's inputs are reproducible.
+inline void set_seed(uint64_t s) { detail::seed_counter().store(s); }
+inline uint64_t next_seed() { return detail::seed_counter().fetch_add(0x9E3779B97F4A7C15ULL); }
+
+template <size_t degree_>
+class poly {
+  static_assert(degree_ >= 8 && degree_ <= (1u << 17) && (degree_ & (degree_ - 1)) == 0,
+                "degree must be a power of two in [2^3, 2^17]");
+
+ public:
+  using value_type = params::value_type;
+  using gt_value_type = params::gt_value_type;
+  using signed_type = params::signed_type;
+  using pointer_type = value_type *;
+  using const_pointer_type = const value_type *;
+  using iterator = pointer_type;
+  using const_iterator = const_pointer_type;
+  static constexpr size_t degree = degree_;
+
+  explicit poly(size_t nmoduli) : nmoduli_(0), d_(nullptr) { resize_moduli_count(nmoduli); clear(); }
+  poly(size_t nmoduli, const uniform &m) : poly(nmoduli) { set(m); }
+  poly(size_t nmoduli, const hwt_dist &m) : poly(nmoduli) { set(m); }
+  poly(size_t nmoduli, const ZO_dist &m) : poly(nmoduli) { set(m); }
+  poly(size_t nmoduli, const gaussian &m) : poly(nmoduli) { set(m); }
+  poly(const poly &o) : poly(o.nmoduli_) { copy_from(o); }
+  poly(poly &&o) noexcept : nmoduli_(o.nmoduli_), d_(o.d_) { o.d_ = nullptr; o.nmoduli_ = 0; }
+  poly &operator=(const poly &o) {
+    if (this != &o) {
+      resize_moduli_count(o.nmoduli_);
+      copy_from(o);
+    }
+    return *this;
+  }
+  ~poly() { if (d_) ya_free(d_); }
+
+  friend void swap(poly &a, poly &b) {
+    std::swap(a.d_, b.d_);
+    std::swap(a.nmoduli_, b.nmoduli_);
+  }
+
+  // ---- samplers (details/poly.hpp:372-501), drawn on the host, uploaded.
+  void set(const uniform &) { sample([&](value_type *h, uint64_t s) { return ya_sample_uniform(h, degree, nmoduli_, s); }); }
+  void set(const ZO_dist &m) { sample([&](value_type *h, uint64_t s) { return ya_sample_zo(h, degree, nmoduli_, m.rho, s); }); }
+  void set(const hwt_dist &m) { sample([&](value_type *h, uint64_t s) { return ya_sample_hwt(h, degree, nmoduli_, m.hwt, s); }); }
+  void set(const gaussian &m) {
+    std::vector<value_type> h(degree * nmoduli_);
+    ya_check(ya_sample_gaussian(h.data(), degree, nmoduli_, m.sigma, next_seed()), "ya_sample_gaussian");
+    if (m.amplifier != 1) {  // r *= amplifier on the signed sample (details/poly.hpp:419-422)
+      for (size_t cm = 0; cm < nmoduli_; ++cm) {
+        const value_type p = params::P[cm];
+        for (size_t i = 0; i < degree; ++i) {
+          value_type &v = h[cm * degree + i];
+          const signed_type r = v > p / 2 ? -(signed_type)(p - v) : (signed_type)v;
+          const signed_type a = r * (signed_type)m.amplifier;
+          v = a < 0 ? p + a : (value_type)a;
+        }
+      }
+    }
+    from_host(h);
+  }
+
+  // ---- access
+  iterator ptr_at(size_t cm) { check_cm(cm); return d_ + cm * degree; }
+  iterator ptr_end(size_t cm) { return ptr_at(cm) + degree; }
+  const_iterator cptr_at(size_t cm) const { check_cm(cm); return d_ + cm * degree; }
+  const_iterator cptr_end(size_t cm) const { return cptr_at(cm) + degree; }
+  value_type *data() { return d_; }
+  const value_type *data() const { return d_; }
+  std::vector<value_type *> raw_data() {
+    std::vector<value_type *> v(nmoduli_);
+    for (size_t cm = 0; cm < nmoduli_; ++cm) v[cm] = d_ + cm * degree;
+    return v;
+  }
+  size_t moduli_count() const { return nmoduli_; }
+  value_type get_modulus(size_t cm) const { return params::P[cm]; }
+
+  value_type operator()(size_t cm, size_t i) const {
+    value_type v;
+    ya_check(ya_memcpy_dtoh(&v, cptr_at(cm) + i, sizeof v, nullptr), "read word");
+    return v;
+  }
+  void set_word(size_t cm, size_t i, value_type v) {
+    ya_check(ya_memcpy_htod(ptr_at(cm) + i, &v, sizeof v, nullptr), "write word");
+  }
+  std::vector<value_type> to_host() const {
+    std::vector<value_type> h(degree * nmoduli_);
+    if (!h.empty()) ya_check(ya_memcpy_dtoh(h.data(), d_, h.size() * sizeof(value_type), nullptr), "to_host");
+    return h;
+  }
+  void from_host(const std::vector<value_type> &h) {
+    if (h.size() != degree * nmoduli_) throw std::runtime_error("from_host: size mismatch");
+    if (!h.empty()) ya_check(ya_memcpy_htod(d_, h.data(), h.size() * sizeof(value_type), nullptr), "from_host");
+  }
+
+  bool operator==(const poly &o) const { return nmoduli_ == o.nmoduli_ && to_host() == o.to_host(); }
+  bool operator!=(const poly &o) const { return !(*this == o); }
+
+  // ---- NTT (details/poly.hpp:341-357)
+  void forward() { if (nmoduli_) ya_check(ya_poly_forward(d_, degree, nmoduli_, nullptr), "forward"); }
+  void forward_lazy() { if (nmoduli_) ya_check(ya_poly_forward_lazy(d_, degree, nmoduli_, nullptr), "forward_lazy"); }
+  void backward() { if (nmoduli_) ya_check(ya_poly_backward(d_, degree, nmoduli_, nullptr), "backward"); }
+
+  // ---- arithmetic (details/poly.hpp:289-328)
+  poly &add_product_of(const poly &a, const poly &b) {
+    same(a);
+    same(b);
+    ya_check(ya_poly_add_product_of(d_, a.d_, b.d_, degree, nmoduli_, nullptr), "add_product_of");
+    return *this;
+  }
+  void negate() { if (nmoduli_) ya_check(ya_poly_negate(d_, degree, nmoduli_, nullptr), "negate"); }
+  // a * s computed as a * r + a * (s - r) with a fresh uniform r (details/poly.hpp:305-320).
+  poly &mul_sensitive_poly(const poly &s) {
+    same(s);
+    poly r0(nmoduli_, uniform{});
+    poly r1(s);
+    r1 -= r0;
+    r1 *= *this;
+    *this *= r0;
+    *this += r1;
+    return *this;
+  }
+  void clear() { if (nmoduli_) ya_check(ya_memset_zero(d_, bytes(), nullptr), "clear"); }
+  void resize_moduli_count(size_t n) {
+    if (n > params::kMaxNbModuli) throw std::runtime_error("too many moduli");
+    if (n == nmoduli_) return;
+    value_type *nd = nullptr;
+    if (n) {
+      void *p = nullptr;
+      if (ya_malloc(&p, n * degree * sizeof(value_type)) != YA_OK)
+        throw std::runtime_error("Memory allocation failed.");
+      nd = static_cast<value_type *>(p);
+      const size_t keep = std::min(n, nmoduli_);
+      if (keep) ya_check(ya_memcpy_dtod(nd, d_, keep * degree * sizeof(value_type), nullptr), "resize copy");
+      if (n > keep) ya_check(ya_memset_zero(nd + keep * degree, (n - keep) * degree * sizeof(value_type), nullptr), "resize zero");
+    }
+    if (d_) ya_free(d_);
+    d_ = nd;
+    nmoduli_ = n;
+  }
+  void check_vaild_range() const {
+    const auto h = to_host();
+    for (size_t cm = 0; cm < nmoduli_; ++cm)
+      for (size_t i = 0; i < degree; ++i)
+        if (h[cm * degree + i] >= params::P[cm]) throw std::runtime_error("coefficient out of range");
+  }
+
+ private:
+  template <class F>
+  void sample(F f) {
+    std::vector<value_type> h(degree * nmoduli_);
+    if (nmoduli_) ya_check(f(h.data(), next_seed()), "sampler");
+    from_host(h);
+  }
+  void copy_from(const poly &o) {
+    if (nmoduli_) ya_check(ya_memcpy_dtod(d_, o.d_, bytes(), nullptr), "copy");
+  }
+  size_t bytes() const { return nmoduli_ * degree * sizeof(value_type); }
+  void check_cm(size_t cm) const { if (cm >= nmoduli_) throw std::out_of_range("modulus index"); }
+  void same(const poly &o) const {
+    if (o.nmoduli_ != nmoduli_) throw std::runtime_error("moduli counts differ");
+  }
+
+  size_t nmoduli_;
+  value_type *d_;
+};
+
+// Element-wise operators (include/yell/ops.hpp:7-41, poly.hpp:163-168): the
+// binary forms allocate the result, the compound forms work in place; both
+// use min(moduli counts) residues as the reference does.
+#define YELL_AMD_BINARY(OP, FN)                                                       \
+  template <size_t N>                                                                 \
+  poly<N> OP(const poly<N> &a, const poly<N> &b) {                                    \
+    const size_t L = std::min(a.moduli_count(), b.moduli_count());                    \
+    poly<N> r(L);                                                                     \
+    if (L) ya_check(FN(r.data(), a.data(), b.data(), N, L, nullptr), #FN);            \
+    return r;                                                                         \
+  }
+#define YELL_AMD_SELF(OP, FN)                                                         \
+  template <size_t N>                                                                 \
+  poly<N> &OP(poly<N> &a, const poly<N> &b) {                                         \
+    const size_t L = std::min(a.moduli_count(), b.moduli_count());                    \
+    if (L) ya_check(FN(a.data(), a.data(), b.data(), N, L, nullptr), #FN);            \
+    return a;                                                                         \
+  }
+YELL_AMD_BINARY(operator*, ya_poly_mul)
+YELL_AMD_BINARY(operator+, ya_poly_add)
+YELL_AMD_BINARY(operator-, ya_poly_sub)
+YELL_AMD_SELF(operator*=, ya_poly_mul)
+YELL_AMD_SELF(operator+=, ya_poly_add)
+YELL_AMD_SELF(operator-=, ya_poly_sub)
+#undef YELL_AMD_BINARY
+#undef YELL_AMD_SELF
+
+// No counterpart in the reference: the whole poly-multiplication
+// backward(forward(a) * forward(b)) in one fused launch, a and b untouched.
+template <size_t N>
+poly<N> multiply(const poly<N> &a, const poly<N> &b) {
+  const size_t L = std::min(a.moduli_count(), b.moduli_count());
+  poly<N> r(L);
+  if (L) ya_check(ya_poly_multiply(r.data(), a.data(), b.data(), N, L, nullptr), "ya_poly_multiply");
+  return r;
+}
+
+}  // namespace yell

@@ -86,6 +86,8 @@ int ya_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
 int ya_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
 int ya_host_alloc_pinned(void **ptr, size_t bytes);
 int ya_host_free_pinned(void *ptr);
+/* *is_device = 1 if ptr is device memory of the current HIP device, else 0. */
+int ya_pointer_is_device(const void *ptr, int *is_device);
 int ya_stream_create(void **stream);
 int ya_stream_destroy(void *stream);
 int ya_stream_synchronize(void *stream);
@@ -174,6 +176,21 @@ int ya_batch_plan_destroy(ya_batch_plan *plan);
 /* One-shot: create, run on `stream`, wait, destroy. */
 int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
 
+/* Autotuner (no counterpart in the reference).  For every degree class
+ * 2^lo .. 2^hi (within 2^7 .. 2^15) times the instantiated pass shapes
+ * (radix-8 or radix-16 register passes; the radix also fixes how many
+ * residues share a 1024-thread workgroup tile) for the batched kernel and
+ * the fused single-launch kernel on the device, and keeps the faster shape
+ * for later plans / launches.  report (optional) receives 4 doubles per
+ * class: batched ms at radix 8, radix 16, fused ms at radix 8, radix 16
+ * (-1 where not applicable).  Results are identical for every shape. */
+int ya_autotune(int logn_lo, int logn_hi, double *report);
+/* Read / set one entry of the shape table: kind 1 = fused kernel, 2 =
+ * batched kernel, 0 = standalone NTT kernels; le = log2 of the register-pass
+ * radix (3 or 4).  ya_tune_get returns le, or -1 for a bad kind/logn. */
+int ya_tune_get(int kind, int logn);
+int ya_tune_set(int kind, int logn, int le);
+
 /* ---- host samplers (input source; details/poly.hpp:201-260) ------------- */
 /* Seeded replacements of poly::set(uniform) and poly::set(gaussian): fill a
  * HOST residue-major buffer.  uniform: per residue, 64-bit word & (2^62-1),
@@ -182,6 +199,12 @@ int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
  * shared by all residues and mapped r < 0 ? p + r : r. */
 int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed);
 int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed);
+/* poly::set(ZO_dist) (details/poly.hpp:438-451): a byte r per coefficient,
+ * r <= rho ? ((r & 2) ? p-1 : 1) : 0, shared by all residues. */
+int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed);
+/* poly::set(hwt_dist) (details/poly.hpp:458-501): exactly hwt nonzero
+ * coefficients (uniform positions), each +1 or p-1, shared by all residues. */
+int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Device throughput of the VALU work this

@@ -1,0 +1,125 @@
+// test_shim.cpp -- exercises the C++ drop-in headers (include/yell/*.hpp)
+// on the GPU the way the reference's own users call yell::poly / ntt / ops
+// (benchmark_ntt.cc, example.cpp).  Exit status 0 = every check passed.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "yell/ntt.hpp"
+#include "yell/ops.hpp"
+#include "yell/poly.hpp"
+
+static int g_fail = 0;
+#define CHECK(c, ...)                                   \
+  do {                                                  \
+    if (!(c)) {                                         \
+      std::fprintf(stderr, "FAIL %s:%d: ", __FILE__, __LINE__); \
+      std::fprintf(stderr, __VA_ARGS__);                \
+      std::fprintf(stderr, "\n");                       \
+      ++g_fail;                                         \
+    }                                                   \
+  } while (0)
+
+using T = yell::params::value_type;
+
+// Poly-multiplication through the reference's call sequence equals the fused
+// entry point and a schoolbook negacyclic convolution on sampled coefficients.
+static void test_polymul() {
+  constexpr size_t N = 4096;
+  const size_t L = 3;
+  yell::poly<N> a(L, yell::uniform{}), b(L, yell::gaussian(3.2));
+  yell::poly<N> fa(a), fb(b);
+  fa.forward();
+  fb.forward();
+  yell::poly<N> c = fa * fb;
+  c.backward();
+  yell::poly<N> fused = yell::multiply(a, b);
+  CHECK(c == fused, "sequence vs fused poly-multiplication differ");
+  const auto ha = a.to_host(), hb = b.to_host(), hc = c.to_host();
+  yell::ops::mulmod mul;
+  yell::ops::addmod add;
+  yell::ops::submod sub;
+  for (size_t cm = 0; cm < L; ++cm) {
+    for (size_t k : {size_t(0), size_t(1), size_t(777), N - 1}) {
+      T pos = 0, neg = 0;
+      for (size_t i = 0; i < N; ++i) {
+        const T prod = mul(ha[cm * N + i], hb[cm * N + (i <= k ? k - i : N + k - i)], cm);
+        if (i <= k) pos = add(pos, prod, cm);
+        else neg = add(neg, prod, cm);
+      }
+      CHECK(hc[cm * N + k] == sub(pos, neg, cm), "schoolbook mismatch cm=%zu k=%zu", cm, k);
+    }
+  }
+}
+
+// ntt<N> on host memory (staged) equals the poly's device transform; tables
+// satisfy the reference's identities.
+static void test_ntt_host_pointer_and_tables() {
+  constexpr size_t N = 1024;
+  yell::poly<N> a(2, yell::uniform{});
+  auto h = a.to_host();
+  std::vector<T> r0(h.begin(), h.begin() + N);
+  yell::ntt<N>::forward_lazy(r0.data(), 0);
+  a.forward_lazy();
+  const auto d = a.to_host();
+  CHECK(std::equal(r0.begin(), r0.end(), d.begin()), "host-staged forward_lazy differs");
+  yell::ntt<N>::backward(a.ptr_at(1), 1);  // device pointer path, in place
+  const auto t = yell::ntt<N>::init_table(1);
+  yell::ops::mulmod mul;
+  CHECK(mul(t->invDegree, N, 1) == 1, "invDegree * N != 1");
+  CHECK(t->shoupPhiTbl[5] == yell::ops::shoupify(t->phiTbl[5], 1), "Shoup companion");
+  CHECK(mul(t->phiTbl[1], t->phiTbl[1], 1) == yell::params::P[1] - 1, "phi^(N/2)^2 != -1");
+}
+
+// The toy RLWE scheme of the reference's example.cpp:12-58 at N = 8192,
+// L = 4: decryption returns the message up to small noise.
+static void test_rlwe_example() {
+  constexpr size_t Deg = 8192, NdM = 4;
+  yell::poly<Deg> sx(NdM, yell::hwt_dist(64));  // SK
+  sx.forward();
+  yell::poly<Deg> pbx(NdM), pax(NdM, yell::uniform{});  // PK
+  pbx.set(yell::gaussian(3.2));
+  pbx.forward();
+  pbx.add_product_of(pax, sx);
+  pbx.negate();
+  yell::poly<Deg> msg(NdM, yell::uniform{});
+  yell::poly<Deg> cbx(NdM), cax(NdM);  // CTXT
+  yell::poly<Deg> u(NdM, yell::ZO_dist{});
+  u.forward();
+  cbx.set(yell::gaussian(3.2));
+  cbx += msg;
+  cbx.forward_lazy();
+  cbx.add_product_of(u, pbx);
+  cax.set(yell::gaussian(3.2));
+  cax.forward_lazy();
+  cax.add_product_of(u, pax);
+  yell::poly<Deg> plain(NdM);  // decrypt
+  plain = cbx;
+  plain.add_product_of(sx, cax);
+  plain.backward();
+  const auto hm = msg.to_host(), hp = plain.to_host();
+  long long worst = 0;
+  for (size_t cm = 0; cm < NdM; ++cm) {
+    const T p = yell::params::P[cm];
+    for (size_t i = 0; i < Deg; ++i) {
+      const T d = yell::ops::submod{}(hp[cm * Deg + i], hm[cm * Deg + i], cm);
+      const long long e = d > p / 2 ? -(long long)(p - d) : (long long)d;
+      if (std::llabs(e) > worst) worst = std::llabs(e);
+    }
+  }
+  CHECK(worst < (1ll << 20), "decryption noise %lld too large", worst);
+  std::printf("rlwe: max |noise| = %lld\n", worst);
+}
+
+int main() {
+  if (ya_init(0) != YA_OK) {
+    std::fprintf(stderr, "no device: %s\n", ya_last_error());
+    return 2;
+  }
+  yell::set_seed(42);
+  test_polymul();
+  test_ntt_host_pointer_and_tables();
+  test_rlwe_example();
+  std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "ok", g_fail);
+  return g_fail ? 1 : 0;
+}

@@ -1,0 +1,149 @@
+"""Batched form: uneven jobs in one persistent launch (and as a hipGraph),
+bit-exact against the oracle and against the plain per-poly path."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from _oracle import Oracle
+from gpu_util import device, say
+from yell_amd import capi
+
+pytestmark = pytest.mark.gpu
+orc = Oracle()
+
+
+def _jobs(rng, njobs, logs, lmax):
+    specs = [(1 << int(rng.choice(logs)), int(rng.integers(1, lmax + 1))) for _ in range(njobs)]
+    host = []
+    for k, (n, L) in enumerate(specs):
+        host.append((n, L, capi.sample_uniform(n, L, 1000 + 2 * k),
+                     capi.sample_uniform(n, L, 1001 + 2 * k)))
+    return host
+
+
+class Batch:
+    def __init__(self, host):
+        device()
+        self.host = host
+        self.bufs = []
+        arr = (capi.PolymulJob * len(host))()
+        for k, (n, L, a, b) in enumerate(host):
+            da, db, dc = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b), capi.DeviceBuffer(n * L)
+            self.bufs.append((da, db, dc))
+            arr[k] = capi.PolymulJob(da.ptr, db.ptr, dc.ptr, n, L)
+        self.jobs = arr
+        self.plan = ctypes.c_void_p()
+        capi.call("ya_batch_plan_create", ctypes.byref(self.plan), ctypes.cast(arr, ctypes.c_void_p), len(host))
+
+    def results(self):
+        return [dc.download() for _, _, dc in self.bufs]
+
+    def close(self):
+        capi.load().ya_batch_plan_destroy(self.plan)
+
+
+def test_uneven_batch_vs_oracle():
+    rng = np.random.default_rng(1)
+    host = _jobs(rng, 30, [7, 10, 11, 12, 13, 14, 15], 12)
+    say("batch uneven", [(n, L) for n, L, _, _ in host])
+    bt = Batch(host)
+    items, res, wgs = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+    capi.call("ya_batch_plan_info", bt.plan, ctypes.byref(items), ctypes.byref(res), ctypes.byref(wgs))
+    assert res.value == sum(L for _, L, _, _ in host)
+    for rep in range(2):   # the queue resets itself between launches
+        capi.call("ya_batch_plan_run", bt.plan, None)
+        capi.call("ya_device_synchronize")
+        for (n, L, a, b), got in zip(host, bt.results()):
+            np.testing.assert_array_equal(got, orc.polymul(a, b, n, L))
+    bt.close()
+
+
+def test_graph_replays_with_new_data():
+    rng = np.random.default_rng(2)
+    host = _jobs(rng, 12, [10, 12, 14, 15], 6)
+    bt = Batch(host)
+    for rep in range(3):
+        say("graph replay", rep)
+        new = []
+        for k, ((n, L, _, _), (da, db, _)) in enumerate(zip(host, bt.bufs)):
+            a = capi.sample_uniform(n, L, 7000 + 10 * rep + k)
+            b = capi.sample_gaussian(n, L, 3.2, 8000 + 10 * rep + k)
+            da.upload(a)
+            db.upload(b)
+            new.append((n, L, a, b))
+        capi.call("ya_batch_plan_run_graph", bt.plan, None)
+        capi.call("ya_device_synchronize")
+        for (n, L, a, b), got in zip(new, bt.results()):
+            np.testing.assert_array_equal(got, orc.polymul(a, b, n, L))
+    bt.close()
+
+
+def test_full_c3_batch_matches_plain_path():
+    """1024 jobs, degrees 2^10..2^15, 1..12 moduli: every job's result equals
+    the plain ya_poly_multiply result (itself pinned to the oracle)."""
+    device()
+    rng = np.random.default_rng(2026)
+    njobs = 1024
+    specs = [(1 << int(rng.integers(10, 16)), int(rng.integers(1, 13))) for _ in range(njobs)]
+    say("C3 full batch", sum(n * L for n, L in specs), "words per operand")
+    arr = (capi.PolymulJob * njobs)()
+    bufs = []
+    for k, (n, L) in enumerate(specs):
+        a = capi.sample_uniform(n, L, 50000 + k)
+        b = capi.sample_uniform(n, L, 90000 + k)
+        da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
+        dc, dp = capi.DeviceBuffer(n * L), capi.DeviceBuffer(n * L)
+        bufs.append((da, db, dc, dp))
+        arr[k] = capi.PolymulJob(da.ptr, db.ptr, dc.ptr, n, L)
+    capi.call("ya_batch_polymul", ctypes.cast(arr, ctypes.c_void_p), njobs, None)
+    for (n, L), (da, db, dc, dp) in zip(specs, bufs):
+        capi.call("ya_poly_multiply", dp.ptr, da.ptr, db.ptr, n, L, None)
+    capi.call("ya_device_synchronize")
+    for k, ((n, L), (da, db, dc, dp)) in enumerate(zip(specs, bufs)):
+        np.testing.assert_array_equal(dc.download(), dp.download(), err_msg="job %d" % k)
+    for t in bufs:
+        for d in t:
+            d.close()
+
+
+def test_rejects_unsupported_degree():
+    device()
+    n = 1 << 16
+    d = capi.DeviceBuffer(3 * n)
+    arr = (capi.PolymulJob * 1)(capi.PolymulJob(d.ptr, d.ptr + 8 * n, d.ptr + 16 * n, n, 1))
+    plan = ctypes.c_void_p()
+    rc = capi.load().ya_batch_plan_create(ctypes.byref(plan), ctypes.cast(arr, ctypes.c_void_p), 1)
+    assert rc == 5   # YA_ERR_UNSUPPORTED
+    arr[0] = capi.PolymulJob(d.ptr, d.ptr + 8 * 1024, d.ptr, 1024, 1)   # c aliases a
+    assert capi.load().ya_batch_plan_create(ctypes.byref(plan), ctypes.cast(arr, ctypes.c_void_p), 1) == 1
+
+
+def test_every_pass_shape_gives_the_same_words():
+    """The autotuner may pick radix-8 or radix-16 passes: both give the oracle's words."""
+    lib = device()
+    rng = np.random.default_rng(5)
+    host = _jobs(rng, 10, [9, 10, 11, 12, 13], 5)
+    saved = {lg: lib.ya_tune_get(2, lg) for lg in range(7, 14)}
+    try:
+        for le in (3, 4):
+            for lg in range(7, 14):
+                capi.call("ya_tune_set", 2, lg, le)
+            say("pass shape le", le)
+            bt = Batch(host)
+            capi.call("ya_batch_plan_run", bt.plan, None)
+            capi.call("ya_device_synchronize")
+            for (n, L, a, b), got in zip(host, bt.results()):
+                np.testing.assert_array_equal(got, orc.polymul(a, b, n, L))
+            bt.close()
+    finally:
+        for lg, le in saved.items():
+            capi.call("ya_tune_set", 2, lg, le)
+
+
+def test_autotune_runs_and_fills_table():
+    lib = device()
+    rep = (ctypes.c_double * 8)()
+    capi.call("ya_autotune", 12, 13, rep)
+    assert all(rep[k] > 0 for k in range(8))
+    assert lib.ya_tune_get(2, 12) in (3, 4) and lib.ya_tune_get(1, 13) in (3, 4)

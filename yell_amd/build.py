@@ -80,6 +80,21 @@ def build_lib(changed_only=False, jobs=None):
     return LIB
 
 
+def build_cpp_tests():
+    """tests/cpp/*.cpp -> tests/cpp/build/<name>, linked against libyell_amd.so."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "build")
+    os.makedirs(out_dir, exist_ok=True)
+    for src in sorted(glob.glob(os.path.join(src_dir, "*.cpp"))):
+        exe = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+               "-o", exe, "-L", os.path.dirname(LIB), "-lyell_amd",
+               "-Wl,-rpath,$ORIGIN/../../../yell_amd/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("C++ test build failed:\n" + r.stdout + r.stderr)
+
+
 def build_oracle():
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], capture_output=True,
                        text=True)
@@ -94,6 +109,7 @@ def main():
     ap.add_argument("-j", type=int, default=None)
     a = ap.parse_args()
     build_lib(a.changed, a.j)
+    build_cpp_tests()
     if not a.no_oracle:
         build_oracle()
     print(LIB)

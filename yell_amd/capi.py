@@ -68,6 +68,9 @@ _SIGS = {
     "ya_poly_multiply_many": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_sample_uniform": (i32, [vp, sz, sz, u64]),
     "ya_sample_gaussian": (i32, [vp, sz, sz, ctypes.c_double, u64]),
+    "ya_sample_zo": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
+    "ya_sample_hwt": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
+    "ya_pointer_is_device": (i32, [vp, ctypes.POINTER(i32)]),
     "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
     "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),
     "ya_batch_plan_run": (i32, [vp, vp]),
@@ -75,6 +78,9 @@ _SIGS = {
     "ya_batch_plan_info": (i32, [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]),
     "ya_batch_plan_destroy": (i32, [vp]),
     "ya_batch_polymul": (i32, [vp, sz, vp]),
+    "ya_autotune": (i32, [i32, i32, ctypes.POINTER(ctypes.c_double)]),
+    "ya_tune_get": (i32, [i32, i32]),
+    "ya_tune_set": (i32, [i32, i32, i32]),
 }
 
 

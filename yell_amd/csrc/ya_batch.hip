@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <vector>
 
@@ -51,19 +52,34 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
   constexpr uint32_t TS = 1u << (LB - LE);  // threads per residue
   const uint32_t sub = t / TS, tl = t % TS;
   const bool active = sub < it.count;
-  const BatchRes r = res[it.first + (active ? sub : 0u)];
-  const ModDesc d = *r.d;
+  BatchRes r;
+  ModDesc d;
+  if constexpr (LB - LE >= 6) {
+    // a sub-group spans whole waves: its residue is wave-uniform, so keep the
+    // pointers and constants in scalar registers
+    const uint32_t k = __builtin_amdgcn_readfirstlane(it.first + (active ? sub : 0u));
+    r = res[k];
+    d = *r.d;
+  } else {
+    r = res[it.first + (active ? sub : 0u)];
+    d = *r.d;
+  }
   uint64_t *my_lds = lds + sub * LdsWords<LB>::value;
   uint64_t x[1 << LE], y[1 << LE];
   fwd_block<LB, LE>(x, r.a, my_lds, d.fwd, LB, 0u, d.p, tl);
   if (active) gmem_store<0, LE>(r.c, x, tl);
+  const uint32_t t2 = phase_v(tl);
+  fwd_block<LB, LE>(x, r.b, my_lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+  if (active) {
+    gmem_load<0, LE>(r.c, y, t2);
+  } else {  // no residue: any defined words will do, nothing is stored
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) y[e] = x[e];
-  fwd_block<LB, LE>(x, r.b, my_lds, d.fwd, LB, 0u, d.p, tl);
-  if (active) gmem_load<0, LE>(r.c, y, tl);
+    for (int e = 0; e < (1 << LE); ++e) y[e] = x[e];
+  }
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
-  inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, d.inv, LB, 0u, d, tl);
+  const uint32_t t3 = phase_v(tl);
+  inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, d.inv, LB, 0u, d, t3);
 }
 
 // Top layer (index bit 14) of a 2^15 residue over global memory, src -> dst.
@@ -103,17 +119,20 @@ __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__
   top15_fwd(r.a, scr, d, t);  // forward(a): bit 14, then each half
   __syncthreads();
   for (uint32_t h = 0; h < 2; ++h) {
-    fwd_block<14, LE>(x, scr + (h << 14), lds, d.fwd, 15, h << 14, d.p, t);
-    gmem_store<0, LE>(scr + (h << 14), x, t);
+    const uint32_t th = phase_v(t);
+    fwd_block<14, LE>(x, scr + (h << 14), lds, d.fwd + phase_zero_s(), 15, h << 14, d.p, th);
+    gmem_store<0, LE>(scr + (h << 14), x, th);
   }
   top15_fwd(r.b, r.c, d, t);  // forward(b) into c
   __syncthreads();
   for (uint32_t h = 0; h < 2; ++h) {
-    fwd_block<14, LE>(x, r.c + (h << 14), lds, d.fwd, 15, h << 14, d.p, t);
-    gmem_load<0, LE>(scr + (h << 14), y, t);
+    const uint32_t th = phase_v(t);
+    fwd_block<14, LE>(x, r.c + (h << 14), lds, d.fwd + phase_zero_s(), 15, h << 14, d.p, th);
+    gmem_load<0, LE>(scr + (h << 14), y, th);
 #pragma unroll
     for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
-    inv_block<14, LE, TOP_GS>(x, r.c + (h << 14), lds, d.inv, 15, h << 14, d, t);
+    const uint32_t ti = phase_v(t);
+    inv_block<14, LE, TOP_GS>(x, r.c + (h << 14), lds, d.inv + phase_zero_s(), 15, h << 14, d, ti);
   }
   __syncthreads();
   top15_inv_merged(r.c, d, t);
@@ -124,14 +143,16 @@ k_batch_polymul(const BatchItem *__restrict__ items, uint32_t nitems,
                 const BatchRes *__restrict__ res, BatchQueue *q, uint64_t *__restrict__ scratch) {
   __shared__ uint64_t lds[LdsWords<14>::value];
   __shared__ uint32_t s_item;
-  const uint32_t t = threadIdx.x;
   for (;;) {
-    if (t == 0) s_item = atomicAdd(&q->next, 1u);
+    if (threadIdx.x == 0) s_item = atomicAdd(&q->next, 1u);
     __syncthreads();
     const uint32_t k = s_item;
     __syncthreads();
     if (k >= nitems) break;
     const BatchItem it = items[k];
+    // per item, so that nothing derived from the thread index is hoisted out
+    // of the loop (and kept live across every item)
+    const uint32_t t = phase_v(threadIdx.x);
     switch (it.logn * 8 + it.le) {
       case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, t); break;
       case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, t); break;
@@ -154,7 +175,7 @@ k_batch_polymul(const BatchItem *__restrict__ items, uint32_t nitems,
     __syncthreads();  // LDS of this item is dead before the next item's first store
   }
   // The last workgroup to leave resets the queue for the next launch.
-  if (t == 0) {
+  if (threadIdx.x == 0) {
     __threadfence();
     if (atomicAdd(&q->done, 1u) == gridDim.x - 1) {
       q->next = 0;
@@ -327,3 +348,109 @@ int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Autotuner.  For each degree class it times every instantiated pass shape
+// (LE = 3: radix-8 register passes and twice as many residues per workgroup
+// tile; LE = 4: radix-16 passes) on zero-filled synthetic operands sized to
+// keep every CU busy for several items, and keeps the faster one in the
+// launch-shape table used by later plans and fused launches.  Timing is
+// data-independent (the butterflies are branch-free).
+namespace {
+
+int time_ms(hipStream_t s, int reps, float *best, const std::function<int()> &fn) {
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  *best = 1e30f;
+  int rc = fn();  // warm-up
+  for (int r = 0; rc == YA_OK && r < reps; ++r) {
+    (void)hipEventRecord(a, s);
+    rc = fn();
+    (void)hipEventRecord(b, s);
+    if (hipEventSynchronize(b) != hipSuccess) rc = fail(YA_ERR_LAUNCH, "tune sync");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (ms < *best) *best = ms;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
+  DeviceState *st;
+  int rc = current_device(&st);
+  if (rc) return rc;
+  if (logn_lo < kBatchMinLog || logn_hi > kBatchMaxLog || logn_lo > logn_hi)
+    return fail(YA_ERR_INVALID, "tune range 2^%d..2^%d outside 2^%d..2^%d", logn_lo, logn_hi,
+                kBatchMinLog, kBatchMaxLog);
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t L = 8;
+  if ((rc = ensure_tables(st, 14, 0, L))) return rc;
+  const size_t words = (size_t)cus * 4 * (1u << 14);  // 4 full items per CU at 2^14
+  uint64_t *buf = nullptr;
+  hipStream_t s = nullptr;
+  hipError_t e = hipMalloc(&buf, 3 * words * 8);
+  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "tune buffers: %s", hipGetErrorString(e));
+  e = hipMemset(buf, 0, 3 * words * 8);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)hipFree(buf);
+    return hip_fail(e, "tune set-up");
+  }
+  uint64_t *a = buf, *b = buf + words, *c = buf + 2 * words;
+  for (int lg = logn_lo; lg <= logn_hi && rc == YA_OK; ++lg) {
+    double *rep = report ? report + 4 * (lg - logn_lo) : nullptr;
+    if (rep) rep[0] = rep[1] = rep[2] = rep[3] = -1.0;
+    if ((rc = ensure_tables(st, lg, 0, L))) break;
+    const size_t n = (size_t)1 << lg;
+    const size_t njobs = words / (n * L);
+    std::vector<ya_polymul_job> jobs(njobs);
+    for (size_t j = 0; j < njobs; ++j)
+      jobs[j] = ya_polymul_job{a + j * n * L, b + j * n * L, c + j * n * L, (uint32_t)n, (uint32_t)L};
+    const int lemax = 4, lemin = (lg >= 14) ? 4 : 3;
+    float best[2] = {1e30f, 1e30f};
+    int arg[2] = {4, 4};
+    for (int le = lemin; le <= lemax && rc == YA_OK; ++le) {
+      set_tuned_shape(lg, 2, NttShape{le});
+      ya_batch_plan *pl = nullptr;
+      if ((rc = ya_batch_plan_create(&pl, jobs.data(), jobs.size()))) break;
+      float ms;
+      rc = time_ms(s, 3, &ms, [&]() { return ya_batch_plan_run(pl, s); });
+      plan_free(pl);
+      if (rep) rep[le - 3] = ms;
+      if (ms < best[0]) best[0] = ms, arg[0] = le;
+      if (lg <= 14 && rc == YA_OK) {
+        const NttShape sh{le};
+        rc = time_ms(s, 3, &ms, [&]() {
+          hipError_t ee = launch_polymul(c, a, b, desc_ptr(st, lg, 0), lg, njobs * L, L, sh, s);
+          return ee == hipSuccess ? YA_OK : hip_fail(ee, "tune fused launch");
+        });
+        if (rep) rep[2 + le - 3] = ms;
+        if (ms < best[1]) best[1] = ms, arg[1] = le;
+      }
+    }
+    set_tuned_shape(lg, 2, NttShape{arg[0]});
+    set_tuned_shape(lg, 1, NttShape{arg[1]});
+  }
+  (void)hipStreamDestroy(s);
+  (void)hipFree(buf);
+  return rc;
+}
+
+extern "C" int ya_tune_get(int kind, int logn) {
+  if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN) return -1;
+  return tuned_shape(logn, kind).le;
+}
+
+extern "C" int ya_tune_set(int kind, int logn, int le) {
+  if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN || (le != 3 && le != 4))
+    return fail(YA_ERR_INVALID, "bad tune entry kind=%d logn=%d le=%d", kind, logn, le);
+  set_tuned_shape(logn, kind, NttShape{le});
+  return YA_OK;
+}

@@ -411,6 +411,20 @@ int ya_host_free_pinned(void *ptr) {
   if (ptr) YA_HIP(hipHostFree(ptr), "hipHostFree");
   return YA_OK;
 }
+int ya_pointer_is_device(const void *ptr, int *is_device) {
+  if (!is_device) return fail(YA_ERR_INVALID, "null output");
+  *is_device = 0;
+  if (!ptr) return YA_OK;
+  hipPointerAttribute_t at;
+  hipError_t e = hipPointerGetAttributes(&at, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // unregistered host memory: not an error
+    return YA_OK;
+  }
+  *is_device = at.type == hipMemoryTypeDevice ? 1 : 0;
+  return YA_OK;
+}
+
 int ya_stream_create(void **stream) {
   if (!stream) return fail(YA_ERR_INVALID, "null stream");
   hipStream_t s;

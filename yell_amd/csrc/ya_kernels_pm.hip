@@ -18,11 +18,13 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
   uint64_t x[1 << LE], y[1 << LE];
   fwd_block<LB, LE>(x, a + off, lds, d.fwd, LB, 0u, d.p, t);
   gmem_store<0, LE>(c + off, x, t);
-  fwd_block<LB, LE>(x, b + off, lds, d.fwd, LB, 0u, d.p, t);
-  gmem_load<0, LE>(c + off, y, t);
+  const uint32_t t2 = phase_v(t);
+  fwd_block<LB, LE>(x, b + off, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+  gmem_load<0, LE>(c + off, y, t2);
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
-  inv_block<LB, LE, TOP_MERGED>(x, c + off, lds, d.inv, LB, 0u, d, t);
+  const uint32_t t3 = phase_v(t);
+  inv_block<LB, LE, TOP_MERGED>(x, c + off, lds, d.inv, LB, 0u, d, t3);
 }
 
 namespace {

@@ -58,6 +58,21 @@ __device__ __forceinline__ uint32_t uniform_if_wave(uint32_t v) {
   return v;
 }
 
+// Phase isolation.  A fused kernel runs several transforms that load the
+// same twiddles and compute the same LDS / global addresses; the optimizer
+// would keep those values live from one transform to the next (exceeding the
+// 128 registers of a 1024-thread workgroup).  Passing the thread index and a
+// zero offset through an empty asm statement makes each phase recompute them.
+__device__ __forceinline__ uint32_t phase_v(uint32_t v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ uint32_t phase_zero_s() {
+  uint32_t z = 0;
+  __asm__ volatile("" : "+s"(z));
+  return z;
+}
+
 template <int LB>
 struct LdsWords {
   static constexpr uint32_t value = (1u << LB) + ((1u << LB) >> 4);
@@ -86,20 +101,27 @@ __device__ __forceinline__ void lds_load(const uint64_t *s, uint64_t (&x)[1 << L
   for (int e = 0; e < (1 << LE); ++e) x[e] = s[lds_pad(base | ((uint32_t)e << WLO))];
 }
 
+// Global-memory word accesses of a window.  The pointers are cast to the
+// global address space: residue pointers read from a descriptor are generic,
+// and generic (FLAT) accesses need a 64-bit address register pair per access.
+typedef __attribute__((address_space(1))) uint64_t gword;
+
 template <int WLO, int LE>
 __device__ __forceinline__ void gmem_load(const uint64_t *__restrict__ g, uint64_t (&x)[1 << LE],
                                           uint32_t t) {
+  const gword *gg = (const gword *)g;
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) x[e] = g[base | ((uint32_t)e << WLO)];
+  for (int e = 0; e < (1 << LE); ++e) x[e] = gg[base | ((uint32_t)e << WLO)];
 }
 
 template <int WLO, int LE>
 __device__ __forceinline__ void gmem_store(uint64_t *__restrict__ g, const uint64_t (&x)[1 << LE],
                                            uint32_t t) {
+  gword *gg = (gword *)g;
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) g[base | ((uint32_t)e << WLO)] = x[e];
+  for (int e = 0; e < (1 << LE); ++e) gg[base | ((uint32_t)e << WLO)] = x[e];
 }
 
 // Forward (CT) layers b = HI-1 ... LO of one pass.  `gbase` is the residue

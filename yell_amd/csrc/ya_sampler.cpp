@@ -12,6 +12,7 @@
 // |x| <= ceil(13 sigma) (tail mass below 2^-120).
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/yell_amd.h"
@@ -101,6 +102,58 @@ extern "C" int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, 
     const uint64_t p = ya::prime(cm);
     uint64_t *dst = out + cm * degree;
     for (size_t i = 0; i < degree; ++i) dst[i] = r[i] < 0 ? p + r[i] : (uint64_t)r[i];
+  }
+  return YA_OK;
+}
+
+// poly::set(ZO_dist) (include/yell/details/poly.hpp:438-451): one random byte
+// r per coefficient shared by all residues; r <= rho ? ((r & 2) ? p-1 : 1) : 0.
+extern "C" int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32_t rho,
+                            uint64_t seed) {
+  int logn;
+  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!out || rho > 255) return ya::fail(YA_ERR_INVALID, "null output or rho > 255");
+  Xoshiro g(seed);
+  std::vector<uint8_t> r(degree);
+  for (size_t i = 0; i < degree; i += 8) {
+    const uint64_t w = g.next();
+    for (size_t k = 0; k < 8 && i + k < degree; ++k) r[i + k] = (uint8_t)(w >> (8 * k));
+  }
+  for (size_t cm = 0; cm < nmoduli; ++cm) {
+    const uint64_t p = ya::prime(cm);
+    for (size_t i = 0; i < degree; ++i)
+      out[cm * degree + i] = r[i] <= rho ? ((r[i] & 2) ? p - 1 : 1) : 0;
+  }
+  return YA_OK;
+}
+
+// poly::set(hwt_dist) (include/yell/details/poly.hpp:458-501): exactly `hwt`
+// positions chosen uniformly (reservoir sampling), each +1 or -1 with equal
+// probability, shared by all residues (-1 stored as p-1).
+extern "C" int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint32_t hwt,
+                             uint64_t seed) {
+  int logn;
+  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!out || hwt == 0 || hwt > degree) return ya::fail(YA_ERR_INVALID, "hwt must be in [1, degree]");
+  Xoshiro g(seed);
+  std::vector<size_t> hit(hwt);
+  for (size_t k = 0; k < hwt; ++k) hit[k] = k;
+  for (size_t k = hwt; k < degree; ++k) {  // reservoir sampling, unbiased draw in [0, k]
+    const uint64_t lim = UINT64_MAX - UINT64_MAX % (k + 1);
+    uint64_t v;
+    do v = g.next(); while (v >= lim);
+    const size_t pos = (size_t)(v % (k + 1));
+    if (pos < hwt) hit[pos] = k;
+  }
+  std::vector<uint8_t> sign(hwt);
+  for (size_t k = 0; k < hwt; ++k) sign[k] = (uint8_t)(g.next() >> 63);
+  for (size_t cm = 0; cm < nmoduli; ++cm) {
+    const uint64_t p = ya::prime(cm);
+    uint64_t *dst = out + cm * degree;
+    std::fill(dst, dst + degree, 0);
+    for (size_t k = 0; k < hwt; ++k) dst[hit[k]] = sign[k] ? 1 : p - 1;
   }
   return YA_OK;
 }
