@@ -168,6 +168,30 @@ def c3_bench(stream, ev, steps, seed=2026):
             "hbm_GBps_algorithmic": 3 * 8 * total / (ms / 1e3) / 1e9}
 
 
+def served_bench(rates, duration, shapes):
+    """Poisson stream of host-operand requests (shape drawn from `shapes`):
+    continuous batching vs one request per launch, same seeded stream."""
+    out = {"shapes": [list(x) for x in shapes], "duration_s": duration, "runs": []}
+    degrees = (ctypes.c_uint32 * len(shapes))(*[n for n, _ in shapes])
+    mods = (ctypes.c_uint32 * len(shapes))(*[L for _, L in shapes])
+    for mode, cfgkw in (("batched", dict(max_batch_jobs=256, max_inflight=3)),
+                        ("plain", dict(max_batch_jobs=1, max_inflight=1))):
+        cfg = capi.ServerConfig(hbm_budget_bytes=2 << 30, max_pending=1024, max_moduli=16, **cfgkw)
+        srv = ctypes.c_void_p()
+        capi.call("ya_server_create", ctypes.byref(srv), ctypes.byref(cfg))
+        for rate in rates:
+            spec = capi.PoissonSpec(rate_per_s=rate, duration_s=duration, nshapes=len(shapes),
+                                    degrees=degrees, nmoduli=mods, pool=8, verify=1, seed=99)
+            rep = capi.ServeReport()
+            log("served", mode, "rate", rate)
+            capi.call("ya_server_run_poisson", srv, ctypes.byref(spec), ctypes.byref(rep))
+            d = rep.as_dict()
+            d.update({"mode": mode, "offered_rate_per_s": rate})
+            out["runs"].append(d)
+        capi.call("ya_server_destroy", srv)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +202,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch, probes and tuner")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[4000.0, 12000.0],
+                    help="offered loads (requests/s) of the served-form runs")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -287,6 +314,12 @@ def main():
     if not args.no_extra:
         log("C3 uneven batch")
         out["c3_uneven_batch"] = c3_bench(stream, ev, max(2, args.steps // 4))
+        sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
+        out["served"] = sv
+        for r in sv["runs"]:
+            if r["mode"] == "batched" and r["offered_rate_per_s"] == args.serve_rates[0]:
+                out["served_p50_us"], out["served_p99_us"] = r["p50_us"], r["p99_us"]
+                out["served_offered_rate_per_s"] = r["offered_rate_per_s"]
     same = bool(np.array_equal(c0, c0_plain) and np.array_equal(c0, c0_grid))
     out["batched_equals_plain"] = same
     if not args.no_cpu:

@@ -191,6 +191,59 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
 int ya_tune_get(int kind, int logn);
 int ya_tune_set(int kind, int logn, int le);
 
+/* ---- served form (request loop with continuous batching) --------------- *
+ * No counterpart in the reference.  Clients submit poly-multiplication
+ * requests with HOST operands a, b and a host result buffer c (pinned memory
+ * from ya_host_alloc_pinned gives asynchronous copies); the server admits
+ * pending requests while their device footprint (3 * degree * nmoduli words)
+ * fits in an arena of hbm_budget_bytes, batches whatever is admitted into one
+ * launch of the batched kernel on one of max_inflight streams, and retires
+ * batches as they finish.  Latency = submit -> result in c.  Results equal
+ * ya_poly_multiply's.  Buffers must stay valid until the request is waited
+ * for.  Degrees 2^7 .. 2^15, nmoduli <= max_moduli. */
+typedef struct ya_server ya_server;
+typedef struct ya_server_config {
+  size_t hbm_budget_bytes;  /* device arena for in-flight operands (default 1 GiB) */
+  uint32_t max_batch_jobs;  /* requests per launch (default 256) */
+  uint32_t max_inflight;    /* batches on the device at once, <= 8 (default 2) */
+  uint32_t max_pending;     /* queued requests before submit is rejected (default 4096) */
+  uint32_t max_moduli;      /* largest nmoduli accepted (default 16) */
+} ya_server_config;          /* zero fields take the defaults */
+typedef struct ya_server_stats {
+  uint64_t submitted, completed, rejected, batches, max_pending, arena_bytes_in_use;
+  double mean_batch_jobs, p50_us, p90_us, p99_us, max_us, mean_us;
+} ya_server_stats;
+int ya_server_create(ya_server **server, const ya_server_config *config);
+/* YA_ERR_REJECTED when the request can never fit or the queue is full. */
+int ya_server_submit(ya_server *server, const uint64_t *a, const uint64_t *b, uint64_t *c,
+                     uint32_t degree, uint32_t nmoduli, uint64_t *ticket);
+/* Blocks until the request is done (timeout_ms < 0: no limit); releases the ticket. */
+int ya_server_wait(ya_server *server, uint64_t ticket, int timeout_ms, double *latency_us);
+int ya_server_poll(ya_server *server, uint64_t ticket, int *done);
+int ya_server_get_stats(ya_server *server, ya_server_stats *stats);
+int ya_server_reset_stats(ya_server *server);
+/* Stops the loop; requests not yet dispatched fail. */
+int ya_server_destroy(ya_server *server);
+
+/* Poisson load generator over a server: exponential inter-arrival gaps at
+ * rate_per_s for duration_s; each request picks one of nshapes (degree,
+ * nmoduli) shapes uniformly and one of `pool` seeded operand pairs of that
+ * shape (a uniform, b Gaussian sigma 3.2).  verify != 0 compares every served
+ * result with the same pair multiplied alone before the run. */
+typedef struct ya_poisson_spec {
+  double rate_per_s, duration_s;
+  uint32_t nshapes;
+  const uint32_t *degrees, *nmoduli;
+  uint32_t pool, verify;
+  uint64_t seed;
+} ya_poisson_spec;
+typedef struct ya_serve_report {
+  uint64_t offered, completed, rejected, failed, mismatches, batches, max_pending;
+  uint64_t client_stalls;  /* arrivals delayed because every result buffer was in use */
+  double wall_s, throughput_per_s, p50_us, p90_us, p99_us, max_us, mean_us, mean_batch_jobs;
+} ya_serve_report;
+int ya_server_run_poisson(ya_server *server, const ya_poisson_spec *spec, ya_serve_report *report);
+
 /* ---- host samplers (input source; details/poly.hpp:201-260) ------------- */
 /* Seeded replacements of poly::set(uniform) and poly::set(gaussian): fill a
  * HOST residue-major buffer.  uniform: per residue, 64-bit word & (2^62-1),

@@ -81,7 +81,46 @@ _SIGS = {
     "ya_autotune": (i32, [i32, i32, ctypes.POINTER(ctypes.c_double)]),
     "ya_tune_get": (i32, [i32, i32]),
     "ya_tune_set": (i32, [i32, i32, i32]),
+    "ya_server_create": (i32, [ctypes.POINTER(vp), vp]),
+    "ya_server_submit": (i32, [vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(u64)]),
+    "ya_server_wait": (i32, [vp, u64, i32, ctypes.POINTER(ctypes.c_double)]),
+    "ya_server_poll": (i32, [vp, u64, ctypes.POINTER(i32)]),
+    "ya_server_get_stats": (i32, [vp, vp]),
+    "ya_server_reset_stats": (i32, [vp]),
+    "ya_server_destroy": (i32, [vp]),
+    "ya_server_run_poisson": (i32, [vp, vp, vp]),
 }
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_jobs", ctypes.c_uint32),
+                ("max_inflight", ctypes.c_uint32), ("max_pending", ctypes.c_uint32),
+                ("max_moduli", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "rejected", "batches",
+                                              "max_pending", "arena_bytes_in_use")] + \
+               [(k, ctypes.c_double) for k in ("mean_batch_jobs", "p50_us", "p90_us", "p99_us",
+                                              "max_us", "mean_us")]
+
+
+class PoissonSpec(ctypes.Structure):
+    _fields_ = [("rate_per_s", ctypes.c_double), ("duration_s", ctypes.c_double),
+                ("nshapes", ctypes.c_uint32), ("degrees", ctypes.POINTER(ctypes.c_uint32)),
+                ("nmoduli", ctypes.POINTER(ctypes.c_uint32)), ("pool", ctypes.c_uint32),
+                ("verify", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("offered", "completed", "rejected", "failed",
+                                              "mismatches", "batches", "max_pending",
+                                              "client_stalls")] + \
+               [(k, ctypes.c_double) for k in ("wall_s", "throughput_per_s", "p50_us", "p90_us",
+                                              "p99_us", "max_us", "mean_us", "mean_batch_jobs")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PolymulJob(ctypes.Structure):

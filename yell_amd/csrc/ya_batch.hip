@@ -13,37 +13,16 @@
 #include <chrono>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <vector>
 
 #include "../../include/yell_amd.h"
+#include "ya_batch.hpp"
 #include "ya_blocks.hpp"
 #include "ya_internal.hpp"
 
 namespace ya {
-
-constexpr int kBatchThreads = 1024;
-constexpr int kBatchMinLog = 7;
-constexpr int kBatchMaxLog = 15;
-
-struct BatchItem {
-  uint32_t logn;   // degree class
-  uint32_t le;     // log2 words per thread in a pass (3 or 4)
-  uint32_t first;  // first residue (index into the residue table)
-  uint32_t count;  // residues in this item
-};
-
-struct BatchRes {
-  const uint64_t *a;
-  const uint64_t *b;
-  uint64_t *c;
-  const ModDesc *d;
-};
-
-struct BatchQueue {
-  uint32_t next;  // next item to claim
-  uint32_t done;  // workgroups that have drained the queue
-};
 
 // Sub-grouped fused poly-multiplication of up to 2^(10+LE-LB) residues.
 template <int LB, int LE>
@@ -139,16 +118,33 @@ __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__
 }
 
 __global__ void __launch_bounds__(kBatchThreads)
-k_batch_polymul(const BatchItem *__restrict__ items, uint32_t nitems,
+k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
                 const BatchRes *__restrict__ res, BatchQueue *q, uint64_t *__restrict__ scratch) {
   __shared__ uint64_t lds[LdsWords<14>::value];
   __shared__ uint32_t s_item;
+  const uint32_t home = blockIdx.x % kSegs;
+  uint32_t seg = home, tried = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&q->next, 1u);
+    if (threadIdx.x == 0) {
+      uint32_t k = 0xffffffffu;
+      while (tried < kSegs) {  // own segment first, then the others in turn
+        const uint32_t n = segs.begin[seg + 1] - segs.begin[seg];
+        if (n != 0) {
+          const uint32_t j = atomicAdd(&q->next[seg], 1u);
+          if (j < n) {
+            k = segs.begin[seg] + j;
+            break;
+          }
+        }
+        seg = (seg + 1) % kSegs;
+        ++tried;
+      }
+      s_item = k;
+    }
     __syncthreads();
     const uint32_t k = s_item;
     __syncthreads();
-    if (k >= nitems) break;
+    if (k == 0xffffffffu) break;
     const BatchItem it = items[k];
     // per item, so that nothing derived from the thread index is hoisted out
     // of the loop (and kept live across every item)
@@ -178,7 +174,7 @@ k_batch_polymul(const BatchItem *__restrict__ items, uint32_t nitems,
   if (threadIdx.x == 0) {
     __threadfence();
     if (atomicAdd(&q->done, 1u) == gridDim.x - 1) {
-      q->next = 0;
+      for (int i = 0; i < kSegs; ++i) q->next[i] = 0;
       q->done = 0;
       __threadfence();
     }
@@ -193,6 +189,7 @@ using namespace ya;
 struct ya_batch_plan {
   int device = 0;
   size_t nitems = 0, nres = 0;
+  BatchSegs segs{};
   unsigned grid = 0;
   BatchItem *d_items = nullptr;
   BatchRes *d_res = nullptr;
@@ -219,18 +216,14 @@ int plan_free(ya_batch_plan *pl) {
 
 }  // namespace
 
-extern "C" {
+namespace ya {
 
-int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t njobs) {
-  if (!out || (!jobs && njobs)) return fail(YA_ERR_INVALID, "null argument");
-  *out = nullptr;
-  if (njobs == 0) return fail(YA_ERR_INVALID, "empty batch");
-  DeviceState *st;
-  int rc = current_device(&st);
-  if (rc) return rc;
-  // Validate and group residues by degree class.
+int build_batch(DeviceState *st, const ya_polymul_job *jobs, size_t njobs,
+                std::vector<BatchItem> &items, std::vector<BatchRes> &res, BatchSegs *segs,
+                bool *has15) {
   std::vector<std::vector<BatchRes>> by_class(kBatchMaxLog + 1);
-  bool has15 = false;
+  *has15 = false;
+  int rc;
   for (size_t j = 0; j < njobs; ++j) {
     const ya_polymul_job &jb = jobs[j];
     int logn;
@@ -247,27 +240,85 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
       const size_t off = (size_t)r * jb.degree;
       by_class[logn].push_back(BatchRes{jb.a + off, jb.b + off, jb.c + off, desc_ptr(st, logn, r)});
     }
-    has15 |= logn == 15;
+    *has15 |= logn == 15;
   }
-  // Items: largest class first (longest items start first).
-  std::vector<BatchItem> items;
-  std::vector<BatchRes> res;
+  // Items per (degree, prime) class; whole classes are dealt to the XCD
+  // segments by estimated work (N log N per residue), largest first to the
+  // least loaded segment; each segment is then ordered largest degree first.
+  struct Cls {
+    int lg;
+    const ModDesc *d;
+    std::vector<BatchRes> v;
+    double work;
+  };
+  std::vector<Cls> classes;
   for (int lg = kBatchMaxLog; lg >= kBatchMinLog; --lg) {
-    const auto &v = by_class[lg];
-    if (v.empty()) continue;
-    int le = lg >= 14 ? 4 : tuned_shape(lg, 2).le;
-    le = (le == 3) ? 3 : 4;
-    const uint32_t per = lg >= 14 ? 1u : (1u << (10 + le - lg));
-    for (size_t i = 0; i < v.size(); i += per) {
-      const uint32_t cnt = (uint32_t)std::min<size_t>(per, v.size() - i);
-      items.push_back(BatchItem{(uint32_t)lg, (uint32_t)le, (uint32_t)res.size(), cnt});
-      res.insert(res.end(), v.begin() + i, v.begin() + i + cnt);
+    std::map<const ModDesc *, std::vector<BatchRes>> byd;
+    for (const auto &r : by_class[lg]) byd[r.d].push_back(r);
+    for (auto &kv : byd)
+      classes.push_back(Cls{lg, kv.first, std::move(kv.second), 0.0});
+  }
+  for (auto &c : classes) c.work = (double)c.v.size() * (double)(1u << c.lg) * c.lg;
+  std::stable_sort(classes.begin(), classes.end(), [](const Cls &x, const Cls &y) { return x.work > y.work; });
+  double load[kSegs] = {0};
+  std::vector<std::vector<const Cls *>> seg_cls(kSegs);
+  for (const auto &c : classes) {
+    int best = 0;
+    for (int k = 1; k < kSegs; ++k)
+      if (load[k] < load[best]) best = k;
+    load[best] += c.work;
+    seg_cls[best].push_back(&c);
+  }
+  items.clear();
+  res.clear();
+  for (int sgi = 0; sgi < kSegs; ++sgi) {
+    segs->begin[sgi] = (uint32_t)items.size();
+    auto &lst = seg_cls[sgi];
+    std::stable_sort(lst.begin(), lst.end(), [](const Cls *x, const Cls *y) { return x->lg > y->lg; });
+    for (const Cls *c : lst) {
+      const int lg = c->lg;
+      int le = lg >= 14 ? 4 : tuned_shape(lg, 2).le;
+      le = (le == 3) ? 3 : 4;
+      const uint32_t per = lg >= 14 ? 1u : (1u << (10 + le - lg));
+      for (size_t i = 0; i < c->v.size(); i += per) {
+        const uint32_t cnt = (uint32_t)std::min<size_t>(per, c->v.size() - i);
+        items.push_back(BatchItem{(uint32_t)lg, (uint32_t)le, (uint32_t)res.size(), cnt});
+        res.insert(res.end(), c->v.begin() + i, c->v.begin() + i + cnt);
+      }
     }
   }
+  segs->begin[kSegs] = (uint32_t)items.size();
+  return YA_OK;
+}
+
+hipError_t launch_batch(const BatchItem *items, const BatchSegs &segs, const BatchRes *res,
+                        BatchQueue *q, uint64_t *scratch, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_polymul, dim3(grid), dim3(kBatchThreads), 0, s, items, segs, res, q,
+                     scratch);
+  return hipGetLastError();
+}
+
+}  // namespace ya
+
+extern "C" {
+
+int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t njobs) {
+  if (!out || (!jobs && njobs)) return fail(YA_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (njobs == 0) return fail(YA_ERR_INVALID, "empty batch");
+  DeviceState *st;
+  int rc = current_device(&st);
+  if (rc) return rc;
+  std::vector<BatchItem> items;
+  std::vector<BatchRes> res;
+  bool has15 = false;
+  BatchSegs segs{};
+  if ((rc = build_batch(st, jobs, njobs, items, res, &segs, &has15))) return rc;
   auto *pl = new ya_batch_plan();
   pl->device = 0;
   (void)hipGetDevice(&pl->device);
   pl->nitems = items.size();
+  pl->segs = segs;
   pl->nres = res.size();
   int cus = 0;
   hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
@@ -281,7 +332,7 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
     plan_free(pl);
     return fail(YA_ERR_ALLOC, "batch plan allocation: %s", hipGetErrorString(e));
   }
-  BatchQueue q0{0, 0};
+  BatchQueue q0{};
   e = hipMemcpy(pl->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(pl->d_res, res.data(), res.size() * sizeof(BatchRes), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(pl->d_q, &q0, sizeof q0, hipMemcpyHostToDevice);
@@ -296,9 +347,8 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
 
 int ya_batch_plan_run(ya_batch_plan *pl, void *stream) {
   if (!pl) return fail(YA_ERR_INVALID, "null plan");
-  hipLaunchKernelGGL(k_batch_polymul, dim3(pl->grid), dim3(kBatchThreads), 0, (hipStream_t)stream,
-                     pl->d_items, (uint32_t)pl->nitems, pl->d_res, pl->d_q, pl->d_scratch);
-  hipError_t e = hipGetLastError();
+  hipError_t e = launch_batch(pl->d_items, pl->segs, pl->d_res, pl->d_q, pl->d_scratch,
+                              pl->grid, (hipStream_t)stream);
   return e == hipSuccess ? YA_OK : hip_fail(e, "batch launch");
 }
 
@@ -313,7 +363,7 @@ int ya_batch_plan_run_graph(ya_batch_plan *pl, void *stream) {
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_batch_polymul, dim3(pl->grid), dim3(kBatchThreads), 0, cap, pl->d_items,
-                         (uint32_t)pl->nitems, pl->d_res, pl->d_q, pl->d_scratch);
+                         pl->segs, pl->d_res, pl->d_q, pl->d_scratch);
       e = hipGetLastError();
       hipError_t e2 = hipStreamEndCapture(cap, &pl->graph);
       if (e == hipSuccess) e = e2;

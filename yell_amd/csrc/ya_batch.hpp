@@ -1,0 +1,60 @@
+// ya_batch.hpp -- work-item layout of the batched poly-multiplication kernel
+// and the host helpers shared by batch plans and the request server.
+#pragma once
+
+#include <vector>
+
+#include "../../include/yell_amd.h"
+#include "ya_internal.hpp"
+
+namespace ya {
+
+constexpr int kBatchThreads = 1024;
+constexpr int kBatchMinLog = 7;
+constexpr int kBatchMaxLog = 15;
+
+struct BatchItem {
+  uint32_t logn;   // degree class
+  uint32_t le;     // log2 words per thread in a pass (3 or 4)
+  uint32_t first;  // first residue (index into the residue table)
+  uint32_t count;  // residues in this item
+};
+
+struct BatchRes {
+  const uint64_t *a;
+  const uint64_t *b;
+  uint64_t *c;
+  const ModDesc *d;
+};
+
+// Work items are split into kSegs segments, one per XCD (8 on MI355X):
+// workgroup w drains segment w % kSegs first -- the hardware places
+// consecutive workgroups on consecutive XCDs, and each segment holds whole
+// (degree, prime) classes, so an XCD's L2 keeps few twiddle tables -- then
+// takes items from the other segments.
+constexpr int kSegs = 8;
+
+struct BatchQueue {
+  uint32_t next[kSegs];  // next item to claim in each segment (relative)
+  uint32_t done;         // workgroups that have drained every segment
+};
+
+struct BatchSegs {
+  uint32_t begin[kSegs + 1];  // segment s = items [begin[s], begin[s+1])
+};
+
+// Groups the residues of `jobs` into work items (sub-grouped by the tuned
+// pass shape), deals whole (degree, prime) classes to the XCD segments by
+// estimated work (largest first, to the least loaded segment) and orders
+// each segment largest degree first; validates every job.
+int build_batch(DeviceState *st, const ya_polymul_job *jobs, size_t njobs,
+                std::vector<BatchItem> &items, std::vector<BatchRes> &res, BatchSegs *segs,
+                bool *has15);
+
+// Launches the persistent kernel over device copies of items / res.  `q`
+// must be zeroed before the first launch (the kernel leaves it zeroed);
+// `scratch` needs grid * 2^15 words when any item is of degree 2^15.
+hipError_t launch_batch(const BatchItem *items, const BatchSegs &segs, const BatchRes *res,
+                        BatchQueue *q, uint64_t *scratch, unsigned grid, hipStream_t s);
+
+}  // namespace ya

@@ -11,8 +11,16 @@ __global__ void __launch_bounds__(1 << (LB - LE))
 k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
           const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
-  const uint32_t res = blockIdx.x;
-  const ModDesc d = descs[res % nmod];
+  // XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (the
+  // hardware deals consecutive workgroups to consecutive XCDs); give each XCD
+  // a contiguous range of the residues in prime-major order, so its L2 holds
+  // the twiddles of few primes.
+  const uint32_t g = gridDim.x, xcd = blockIdx.x % 8u, pos = blockIdx.x / 8u;
+  const uint32_t v = xcd * (g / 8u) + min(xcd, g % 8u) + pos;
+  const uint32_t count = g / nmod;             // polynomials in the launch
+  const uint32_t cm = v / count, poly = v % count;
+  const uint32_t res = poly * nmod + cm;
+  const ModDesc d = descs[cm];
   const size_t off = (size_t)res << LB;
   const uint32_t t = threadIdx.x;
   uint64_t x[1 << LE], y[1 << LE];
