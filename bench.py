@@ -230,12 +230,13 @@ def main():
         d_.close()
     if not args.no_extra:
         log("autotune 2^10..2^15")
-        rep = (ctypes.c_double * 24)()
+        rep = (ctypes.c_double * 36)()
         capi.call("ya_autotune", 10, 15, rep)
         extra["autotune"] = {"classes": list(range(10, 16)),
-                             "ms_batched_radix8_radix16_fused_radix8_radix16": [
-                                 [round(rep[4 * k + j], 4) for j in range(4)] for k in range(6)],
-                             "chosen_batched_le": [capi.load().ya_tune_get(2, lg) for lg in range(10, 16)]}
+                             "ms_batched_r8_r16_fused_r8_r16_r32": [
+                                 [round(rep[6 * k + j], 4) for j in (0, 1, 3, 4, 5)] for k in range(6)],
+                             "chosen_batched_le": [capi.load().ya_tune_get(2, lg) for lg in range(10, 16)],
+                             "chosen_fused_le": [capi.load().ya_tune_get(1, lg) for lg in range(10, 16)]}
         probes = {}
         for kind, name in ((0, "mad_u64_u32_per_s"), (1, "mul_lo_u32_per_s"), (2, "ct_butterflies_per_s")):
             v = ctypes.c_double()

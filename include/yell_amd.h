@@ -187,7 +187,7 @@ int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
 int ya_autotune(int logn_lo, int logn_hi, double *report);
 /* Read / set one entry of the shape table: kind 1 = fused kernel, 2 =
  * batched kernel, 0 = standalone NTT kernels; le = log2 of the register-pass
- * radix (3 or 4).  ya_tune_get returns le, or -1 for a bad kind/logn. */
+ * radix (3, 4, or 5 for kinds 0 and 1).  ya_tune_get returns le, or -1. */
 int ya_tune_get(int kind, int logn);
 int ya_tune_set(int kind, int logn, int le);
 

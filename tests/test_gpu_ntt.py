@@ -173,3 +173,31 @@ def test_extreme_values(logn):
     np.testing.assert_array_equal(polymul(top1, top1, n, L), orc.polymul(top1, top1, n, L))
     z = np.zeros(n * L, dtype=np.uint64)
     np.testing.assert_array_equal(polymul(z, top1, n, L), z)
+
+
+@pytest.mark.parametrize("le", [3, 4, 5])
+def test_every_pass_shape_of_fused_and_standalone_kernels(le):
+    """Radix-8/16/32 register passes (autotuner choices) give the oracle's words."""
+    lib = device()
+    saved = {(k, lg): lib.ya_tune_get(k, lg) for k in (0, 1) for lg in (12, 13, 14)}
+    try:
+        for lg in (12, 13, 14):
+            if le == 3 and lg == 14:
+                continue
+            for k in (0, 1):
+                capi.call("ya_tune_set", k, lg, le)
+            n, L = 1 << lg, 2
+            rng = np.random.default_rng(500 + lg + le)
+            a = rand_bound(rng, n, L, 1, P)
+            b = rand_bound(rng, n, L, 1, P)
+            say("shape", le, lg)
+            np.testing.assert_array_equal(polymul(a, b, n, L), orc.polymul(a, b, n, L))
+            x = rand_bound(rng, n, L, 4, P)
+            np.testing.assert_array_equal(run_inplace("ya_poly_forward_lazy", x, n, L),
+                                          orc.poly_forward_lazy(x, n, L))
+            y = rand_bound(rng, n, L, 2, P)
+            np.testing.assert_array_equal(run_inplace("ya_poly_backward", y, n, L),
+                                          orc.poly_backward(y, n, L))
+    finally:
+        for (k, lg), v in saved.items():
+            capi.call("ya_tune_set", k, lg, v)

@@ -455,38 +455,41 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
   }
   uint64_t *a = buf, *b = buf + words, *c = buf + 2 * words;
   for (int lg = logn_lo; lg <= logn_hi && rc == YA_OK; ++lg) {
-    double *rep = report ? report + 4 * (lg - logn_lo) : nullptr;
-    if (rep) rep[0] = rep[1] = rep[2] = rep[3] = -1.0;
+    double *rep = report ? report + 6 * (lg - logn_lo) : nullptr;
+    if (rep) rep[0] = rep[1] = rep[2] = rep[3] = rep[4] = rep[5] = -1.0;
     if ((rc = ensure_tables(st, lg, 0, L))) break;
     const size_t n = (size_t)1 << lg;
     const size_t njobs = words / (n * L);
     std::vector<ya_polymul_job> jobs(njobs);
     for (size_t j = 0; j < njobs; ++j)
       jobs[j] = ya_polymul_job{a + j * n * L, b + j * n * L, c + j * n * L, (uint32_t)n, (uint32_t)L};
-    const int lemax = 4, lemin = (lg >= 14) ? 4 : 3;
+    const int lemin = (lg >= 14) ? 4 : 3, lemax = (lg >= 12 && lg <= 14) ? 5 : 4;
     float best[2] = {1e30f, 1e30f};
     int arg[2] = {4, 4};
     for (int le = lemin; le <= lemax && rc == YA_OK; ++le) {
-      set_tuned_shape(lg, 2, NttShape{le});
-      ya_batch_plan *pl = nullptr;
-      if ((rc = ya_batch_plan_create(&pl, jobs.data(), jobs.size()))) break;
       float ms;
-      rc = time_ms(s, 3, &ms, [&]() { return ya_batch_plan_run(pl, s); });
-      plan_free(pl);
-      if (rep) rep[le - 3] = ms;
-      if (ms < best[0]) best[0] = ms, arg[0] = le;
+      if (le <= 4) {  // the batched kernel's 1024-thread tiles take radix 8 or 16
+        set_tuned_shape(lg, 2, NttShape{le});
+        ya_batch_plan *pl = nullptr;
+        if ((rc = ya_batch_plan_create(&pl, jobs.data(), jobs.size()))) break;
+        rc = time_ms(s, 3, &ms, [&]() { return ya_batch_plan_run(pl, s); });
+        plan_free(pl);
+        if (rep) rep[le - 3] = ms;
+        if (ms < best[0]) best[0] = ms, arg[0] = le;
+      }
       if (lg <= 14 && rc == YA_OK) {
         const NttShape sh{le};
         rc = time_ms(s, 3, &ms, [&]() {
           hipError_t ee = launch_polymul(c, a, b, desc_ptr(st, lg, 0), lg, njobs * L, L, sh, s);
           return ee == hipSuccess ? YA_OK : hip_fail(ee, "tune fused launch");
         });
-        if (rep) rep[2 + le - 3] = ms;
+        if (rep) rep[3 + le - 3] = ms;
         if (ms < best[1]) best[1] = ms, arg[1] = le;
       }
     }
     set_tuned_shape(lg, 2, NttShape{arg[0]});
     set_tuned_shape(lg, 1, NttShape{arg[1]});
+    set_tuned_shape(lg, 0, NttShape{arg[1]});
   }
   (void)hipStreamDestroy(s);
   (void)hipFree(buf);
@@ -499,7 +502,7 @@ extern "C" int ya_tune_get(int kind, int logn) {
 }
 
 extern "C" int ya_tune_set(int kind, int logn, int le) {
-  if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN || (le != 3 && le != 4))
+  if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN || le < 3 || le > 5 || (kind == 2 && le == 5))
     return fail(YA_ERR_INVALID, "bad tune entry kind=%d logn=%d le=%d", kind, logn, le);
   set_tuned_shape(logn, kind, NttShape{le});
   return YA_OK;

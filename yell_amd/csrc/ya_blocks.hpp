@@ -25,10 +25,17 @@ __device__ __forceinline__ void static_for(F &&f) {
 }
 
 // ---------------------------------------------------------------------------
-// Forward transform of index bits [0, LB) of one block, words loaded from
-// global `src` (block base).  On return x holds the words of window [0, LE)
-// for thread t (i = e | t << LE), in [0, 4p).
+// Window start of the first forward pass of a 2^LB block.
 template <int LB, int LE>
+struct FirstWindow {
+  static constexpr int value = LB - LE > 0 ? LB - LE : 0;
+};
+
+// Forward transform of index bits [0, LB) of one block, words loaded from
+// global `src` (block base), or already in x (PRELOADED: loaded with
+// gmem_load<FirstWindow<LB,LE>::value, LE>).  On return x holds the words of
+// window [0, LE) for thread t (i = e | t << LE), in [0, 4p).
+template <int LB, int LE, bool PRELOADED = false>
 __device__ __forceinline__ void fwd_block(uint64_t (&x)[1 << LE], const uint64_t *__restrict__ src,
                                           uint64_t *lds, const Tw *__restrict__ tw, uint32_t nlog,
                                           uint32_t boff, uint64_t p, uint32_t t) {
@@ -39,7 +46,7 @@ __device__ __forceinline__ void fwd_block(uint64_t (&x)[1 << LE], const uint64_t
     constexpr int LO = HI - LE > 0 ? HI - LE : 0;
     constexpr int WLO = LO;
     if constexpr (P == 0) {
-      gmem_load<WLO, LE>(src, x, t);
+      if constexpr (!PRELOADED) gmem_load<WLO, LE>(src, x, t);
     } else {
       lds_load<WLO, LE>(lds, x, t);
     }
@@ -94,8 +101,9 @@ __device__ __forceinline__ void inv_block(uint64_t (&x)[1 << LE], uint64_t *__re
 }
 
 // Runtime (lb, le) -> template instance.  Instantiated shapes: le = 3 for
-// lb = 3, le = 4 for lb in [4, 14], and le = 3 as the alternative for lb in
-// [10, 13]; any other request falls back to le = 4 (clamp_le).
+// lb = 3, le = 4 for lb in [4, 14], le = 3 as an alternative for lb in
+// [10, 13] and le = 5 (radix-32 passes, 512-thread workgroups at 2^14) for lb
+// in [12, 14]; any other request falls back to le = 4 (clamp_le).
 #define YA_LB_SWITCH(lb, le, CALL)                         \
   switch ((lb) * 8 + (le)) {                               \
     case 3 * 8 + 3: return CALL(3, 3);                     \
@@ -111,15 +119,19 @@ __device__ __forceinline__ void inv_block(uint64_t (&x)[1 << LE], uint64_t *__re
     case 11 * 8 + 4: return CALL(11, 4);                   \
     case 12 * 8 + 3: return CALL(12, 3);                   \
     case 12 * 8 + 4: return CALL(12, 4);                   \
+    case 12 * 8 + 5: return CALL(12, 5);                   \
     case 13 * 8 + 3: return CALL(13, 3);                   \
     case 13 * 8 + 4: return CALL(13, 4);                   \
+    case 13 * 8 + 5: return CALL(13, 5);                   \
     case 14 * 8 + 4: return CALL(14, 4);                   \
+    case 14 * 8 + 5: return CALL(14, 5);                   \
     default: return hipErrorInvalidValue;                  \
   }
 
 inline int clamp_le(int lb, int le) {
   if (lb <= 3) return 3;
   if (le == 3 && lb >= 10 && lb <= 13) return 3;
+  if (le == 5 && lb >= 12) return 5;
   return 4;
 }
 

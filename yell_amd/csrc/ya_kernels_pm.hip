@@ -24,11 +24,26 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
   const size_t off = (size_t)res << LB;
   const uint32_t t = threadIdx.x;
   uint64_t x[1 << LE], y[1 << LE];
-  fwd_block<LB, LE>(x, a + off, lds, d.fwd, LB, 0u, d.p, t);
-  gmem_store<0, LE>(c + off, x, t);
-  const uint32_t t2 = phase_v(t);
-  fwd_block<LB, LE>(x, b + off, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
-  gmem_load<0, LE>(c + off, y, t2);
+  if constexpr (LE <= 4) {
+    // both operands are requested at once (b waits in y during forward(a)),
+    // so the second transform does not stall on HBM latency again
+    constexpr int W0 = FirstWindow<LB, LE>::value;
+    gmem_load<W0, LE>(a + off, x, t);
+    gmem_load<W0, LE>(b + off, y, t);
+    fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd, LB, 0u, d.p, t);
+    gmem_store<0, LE>(c + off, x, t);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) x[e] = y[e];
+    const uint32_t t2 = phase_v(t);
+    fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+    gmem_load<0, LE>(c + off, y, t2);
+  } else {
+    fwd_block<LB, LE>(x, a + off, lds, d.fwd, LB, 0u, d.p, t);
+    gmem_store<0, LE>(c + off, x, t);
+    const uint32_t t2 = phase_v(t);
+    fwd_block<LB, LE>(x, b + off, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+    gmem_load<0, LE>(c + off, y, t2);
+  }
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);
