@@ -168,6 +168,39 @@ def c3_bench(stream, ev, steps, seed=2026):
             "hbm_GBps_algorithmic": 3 * 8 * total / (ms / 1e3) / 1e9}
 
 
+def c4_bench(stream, ev, steps):
+    """N = 2^16, L = 24: forward NTT of 16 operands, sum of 8 pair products in
+    the NTT domain, one inverse NTT (17 * N * L * 8 algorithmic bytes)."""
+    n, L, pairs = 1 << 16, 24, 8
+    w = n * L
+    ops = np.concatenate([capi.sample_uniform(n, L, 4000 + k) for k in range(2 * pairs)])
+    dall, dacc = capi.DeviceBuffer.from_host(ops), capi.DeviceBuffer(w)
+    capi.call("ya_ntt_init_tables", n, L)
+    pa = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j) for j in range(pairs)])
+    pb = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j + 1) for j in range(pairs)])
+    e2 = Events()
+
+    def step():
+        capi.call("ya_poly_forward_many", dall.ptr, n, L, 2 * pairs, stream)
+        capi.call("ya_event_record", e2.a, stream)
+        capi.call("ya_memset_zero", dacc.ptr, w * 8, stream)
+        capi.call("ya_poly_add_products", dacc.ptr, pa, pb, pairs, n, L, stream)
+        capi.call("ya_event_record", e2.b, stream)
+        capi.call("ya_poly_backward", dacc.ptr, n, L, stream)
+
+    step()
+    capi.call("ya_stream_synchronize", stream)
+    ms = timed(ev, stream, step, steps)
+    ms_mac = e2.elapsed_ms()
+    dall.close()
+    dacc.close()
+    alg = 17 * w * 8
+    return {"degree": n, "nmoduli": L, "pairs": pairs, "ms_per_job": ms,
+            "jobs_per_s": 1e3 / ms, "algorithmic_bytes": alg,
+            "hbm_GBps_algorithmic": alg / (ms / 1e3) / 1e9,
+            "mac_ms": ms_mac, "mac_GBps": (2 * pairs + 2) * w * 8 / (ms_mac / 1e3) / 1e9}
+
+
 def served_bench(rates, duration, shapes):
     """Poisson stream of host-operand requests (shape drawn from `shapes`):
     continuous batching vs one request per launch, same seeded stream."""
@@ -315,6 +348,8 @@ def main():
     if not args.no_extra:
         log("C3 uneven batch")
         out["c3_uneven_batch"] = c3_bench(stream, ev, max(2, args.steps // 4))
+        log("C4 large-ring MAC")
+        out["c4_large_ring_mac"] = c4_bench(stream, ev, 3)
         sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
         out["served"] = sv
         for r in sv["runs"]:

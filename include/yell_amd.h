@@ -135,6 +135,18 @@ int ya_poly_negate(uint64_t *x, size_t degree, size_t nmoduli, void *stream);
 int ya_poly_add_product_of(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t degree,
                            size_t nmoduli, void *stream);
 
+/* The same three transforms over `count` polynomials stored back to back
+ * (polynomial k at words [k*L*N, (k+1)*L*N), residue r of each using P[r]). */
+int ya_poly_forward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream);
+int ya_poly_forward_lazy_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count,
+                              void *stream);
+int ya_poly_backward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream);
+/* acc.add_product_of(a[0], b[0]); acc.add_product_of(a[1], b[1]); ... in one
+ * pass (details/poly.hpp:119-132 per pair, same words): a and b are HOST
+ * arrays of npairs DEVICE pointers to polys of the acc's shape. */
+int ya_poly_add_products(uint64_t *acc, const uint64_t *const *a, const uint64_t *const *b,
+                         size_t npairs, size_t degree, size_t nmoduli, void *stream);
+
 /* One RNS poly-multiplication, fused: c = backward(forward(a) * forward(b)),
  * the call sequence a.forward(); b.forward(); c = a * b; c.backward() of the
  * reference (details/poly.hpp:171-186, ops.hpp:7-24) without modifying a, b.

@@ -1,0 +1,83 @@
+"""Large-ring multiply-accumulate (N = 2^16, L = 24, 8 pairs) and the
+many-polynomial transforms, bit-exact against the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from _oracle import Oracle
+from gpu_util import device, rand_bound, say
+from yell_amd import capi
+
+pytestmark = pytest.mark.gpu
+orc = Oracle()
+P = [orc.prime(i) for i in range(32)]
+
+
+@pytest.mark.parametrize("logn", [10, 14, 15, 16])
+def test_many_transforms(logn):
+    device()
+    n, L, count = 1 << logn, 3, 3
+    rng = np.random.default_rng(logn)
+    x = np.concatenate([rand_bound(rng, n, L, 4, P) for _ in range(count)])
+    d = capi.DeviceBuffer.from_host(x)
+    say("forward_lazy_many", logn)
+    capi.call("ya_poly_forward_lazy_many", d.ptr, n, L, count, None)
+    got = d.download()
+    for k in range(count):
+        s = slice(k * n * L, (k + 1) * n * L)
+        np.testing.assert_array_equal(got[s], orc.poly_forward_lazy(x[s], n, L))
+    y = np.concatenate([rand_bound(rng, n, L, 2, P) for _ in range(count)])
+    d.upload(y)
+    capi.call("ya_poly_backward_many", d.ptr, n, L, count, None)
+    got = d.download()
+    for k in range(count):
+        s = slice(k * n * L, (k + 1) * n * L)
+        np.testing.assert_array_equal(got[s], orc.poly_backward(y[s], n, L))
+
+
+@pytest.mark.parametrize("npairs", [1, 3, 17])
+def test_add_products(npairs):
+    device()
+    n, L = 1 << 11, 4
+    rng = np.random.default_rng(npairs)
+    acc = rand_bound(rng, n, L, 1, P)
+    a = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+    b = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+    da = [capi.DeviceBuffer.from_host(v) for v in a]
+    db = [capi.DeviceBuffer.from_host(v) for v in b]
+    dacc = capi.DeviceBuffer.from_host(acc)
+    pa = (ctypes.c_void_p * npairs)(*[x.ptr for x in da])
+    pb = (ctypes.c_void_p * npairs)(*[x.ptr for x in db])
+    say("add_products", npairs)
+    capi.call("ya_poly_add_products", dacc.ptr, pa, pb, npairs, n, L, None)
+    want = acc
+    for j in range(npairs):
+        want = orc.poly_add_product_of(want, a[j], b[j], n, L)
+    np.testing.assert_array_equal(dacc.download(), want)
+
+
+def test_c4_large_ring_mac():
+    """sum_{j<8} a_j * b_j at N = 2^16, L = 24: 16 forward NTTs, the MAC, one
+    inverse NTT (the reference sequence: forward, add_product_of x 8, backward)."""
+    device()
+    n, L, pairs = 1 << 16, 24, 8
+    ops = [capi.sample_uniform(n, L, 4000 + k) for k in range(2 * pairs)]
+    dall = capi.DeviceBuffer.from_host(np.concatenate(ops))
+    w = n * L
+    dacc = capi.DeviceBuffer(w)
+    capi.call("ya_memset_zero", dacc.ptr, w * 8, None)
+    say("C4 forward x16")
+    capi.call("ya_poly_forward_many", dall.ptr, n, L, 2 * pairs, None)
+    pa = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j) for j in range(pairs)])
+    pb = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j + 1) for j in range(pairs)])
+    capi.call("ya_poly_add_products", dacc.ptr, pa, pb, pairs, n, L, None)
+    capi.call("ya_poly_backward", dacc.ptr, n, L, None)
+    got = dacc.download()
+    say("C4 oracle")
+    acc = np.zeros(w, dtype=np.uint64)
+    for j in range(pairs):
+        fa = orc.poly_forward(ops[2 * j], n, L)
+        fb = orc.poly_forward(ops[2 * j + 1], n, L)
+        acc = orc.poly_add_product_of(acc, fa, fb, n, L)
+    np.testing.assert_array_equal(got, orc.poly_backward(acc, n, L))

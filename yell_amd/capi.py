@@ -65,6 +65,10 @@ _SIGS = {
     "ya_poly_negate": (i32, [vp, sz, sz, vp]),
     "ya_poly_add_product_of": (i32, [vp, vp, vp, sz, sz, vp]),
     "ya_poly_multiply": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya_poly_forward_many": (i32, [vp, sz, sz, sz, vp]),
+    "ya_poly_forward_lazy_many": (i32, [vp, sz, sz, sz, vp]),
+    "ya_poly_backward_many": (i32, [vp, sz, sz, sz, vp]),
+    "ya_poly_add_products": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_poly_multiply_many": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_sample_uniform": (i32, [vp, sz, sz, u64]),
     "ya_sample_gaussian": (i32, [vp, sz, sz, ctypes.c_double, u64]),

@@ -65,12 +65,12 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
 __device__ __forceinline__ void top15_fwd(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
                                           const ModDesc &d, uint32_t t) {
   const Tw w = load_tw(d.fwd + 1);  // (N >> 15) + 0
-  const uint64_t p2 = d.p << 1;
+  const PrimeK k(d.p);
 #pragma unroll 4
   for (uint32_t q = 0; q < 16; ++q) {
     const uint32_t i = t + q * kBatchThreads;
     uint64_t x0 = src[i], x1 = src[i + (1u << 14)];
-    ct_butterfly(x0, x1, w, d.p, p2);
+    ct_butterfly<true>(x0, x1, w, k);  // first layer: the caller's words
     dst[i] = x0;
     dst[i + (1u << 14)] = x1;
   }

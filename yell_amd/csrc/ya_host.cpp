@@ -466,38 +466,84 @@ int ya_event_elapsed_ms(float *ms, void *start, void *end) {
 }
 
 // ---- NTT ------------------------------------------------------------------
-static int ntt_common(uint64_t *x, size_t degree, size_t cm0, size_t nres, int dir, void *stream) {
+static int ntt_common(uint64_t *x, size_t degree, size_t cm0, size_t nres, size_t count, int dir,
+                      void *stream) {
   int logn;
   int rc = check_shape(degree, cm0, nres, &logn);
   if (rc) return rc;
   if (!x) return fail(YA_ERR_INVALID, "null buffer");
+  if (count == 0 || count * nres > (1u << 30)) return fail(YA_ERR_INVALID, "bad count %zu", count);
   DeviceState *st;
   if ((rc = current_device(&st))) return rc;
   if ((rc = ensure_tables(st, logn, cm0, nres))) return rc;
   const ModDesc *d = desc_ptr(st, logn, cm0);
   hipError_t e;
-  if (dir == 2) e = launch_backward(x, d, logn, nres, tuned_shape(logn, 0), (hipStream_t)stream);
-  else e = launch_forward(x, d, logn, nres, dir == 1, tuned_shape(logn, 0), (hipStream_t)stream);
+  const size_t total = nres * count;
+  if (dir == 2) e = launch_backward(x, d, logn, total, nres, tuned_shape(logn, 0), (hipStream_t)stream);
+  else e = launch_forward(x, d, logn, total, nres, dir == 1, tuned_shape(logn, 0), (hipStream_t)stream);
   return e == hipSuccess ? YA_OK : hip_fail(e, "NTT launch");
 }
 
 int ya_ntt_forward(uint64_t *x, size_t degree, size_t cm, void *stream) {
-  return ntt_common(x, degree, cm, 1, 0, stream);
+  return ntt_common(x, degree, cm, 1, 1, 0, stream);
 }
 int ya_ntt_forward_lazy(uint64_t *x, size_t degree, size_t cm, void *stream) {
-  return ntt_common(x, degree, cm, 1, 1, stream);
+  return ntt_common(x, degree, cm, 1, 1, 1, stream);
 }
 int ya_ntt_backward(uint64_t *x, size_t degree, size_t cm, void *stream) {
-  return ntt_common(x, degree, cm, 1, 2, stream);
+  return ntt_common(x, degree, cm, 1, 1, 2, stream);
 }
 int ya_poly_forward(uint64_t *poly, size_t degree, size_t nmoduli, void *stream) {
-  return ntt_common(poly, degree, 0, nmoduli, 0, stream);
+  return ntt_common(poly, degree, 0, nmoduli, 1, 0, stream);
 }
 int ya_poly_forward_lazy(uint64_t *poly, size_t degree, size_t nmoduli, void *stream) {
-  return ntt_common(poly, degree, 0, nmoduli, 1, stream);
+  return ntt_common(poly, degree, 0, nmoduli, 1, 1, stream);
 }
 int ya_poly_backward(uint64_t *poly, size_t degree, size_t nmoduli, void *stream) {
-  return ntt_common(poly, degree, 0, nmoduli, 2, stream);
+  return ntt_common(poly, degree, 0, nmoduli, 1, 2, stream);
+}
+int ya_poly_forward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream) {
+  return ntt_common(polys, degree, 0, nmoduli, count, 0, stream);
+}
+int ya_poly_forward_lazy_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count,
+                              void *stream) {
+  return ntt_common(polys, degree, 0, nmoduli, count, 1, stream);
+}
+int ya_poly_backward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream) {
+  return ntt_common(polys, degree, 0, nmoduli, count, 2, stream);
+}
+
+int ya_poly_add_products(uint64_t *acc, const uint64_t *const *a, const uint64_t *const *b,
+                         size_t npairs, size_t degree, size_t nmoduli, void *stream) {
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!acc || (npairs && (!a || !b))) return fail(YA_ERR_INVALID, "null buffer");
+  for (size_t j = 0; j < npairs; ++j)
+    if (!a[j] || !b[j]) return fail(YA_ERR_INVALID, "null operand %zu", j);
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  for (size_t j0 = 0; j0 < npairs; j0 += kMacPairs) {
+    MacPairs mp{};
+    const int k = (int)std::min<size_t>(kMacPairs, npairs - j0);
+    for (int j = 0; j < k; ++j) {
+      mp.a[j] = a[j0 + j];
+      mp.b[j] = b[j0 + j];
+    }
+    for (size_t r0 = 0; r0 < nmoduli; r0 += 65535) {
+      const size_t nr = std::min<size_t>(65535, nmoduli - r0);
+      MacPairs sub = mp;
+      for (int j = 0; j < k; ++j) {
+        sub.a[j] += r0 * degree;
+        sub.b[j] += r0 * degree;
+      }
+      hipError_t e = launch_mac(acc + r0 * degree, sub, k, desc_ptr(st, logn, r0), degree, nr,
+                                (hipStream_t)stream);
+      if (e != hipSuccess) return hip_fail(e, "MAC launch");
+    }
+  }
+  return YA_OK;
 }
 
 static int seam_host(uint64_t *x, size_t degree, const uint64_t *w, const uint64_t *ws, uint64_t p,
@@ -528,7 +574,7 @@ static int seam_host(uint64_t *x, size_t degree, const uint64_t *w, const uint64
   const ModDesc *dd = desc_ptr(st, logn, cm);
   if (e == hipSuccess) {
     if (inverse) e = launch_inverse_lazy(d, dd, logn, 1, nullptr);
-    else e = launch_forward(d, dd, logn, 1, true, tuned_shape(logn, 0), nullptr);
+    else e = launch_forward(d, dd, logn, 1, 1, true, tuned_shape(logn, 0), nullptr);
   }
   if (e == hipSuccess) e = hipMemcpy(x, d, degree * 8, hipMemcpyDeviceToHost);
   (void)hipFree(d);
@@ -614,14 +660,14 @@ static int polymul_impl(uint64_t *c, const uint64_t *a, const uint64_t *b, size_
     uint64_t *ck = c + k * pw;
     e = hipMemcpyAsync(ck, a + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(tmp, b + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = launch_forward(ck, d, logn, nmoduli, true, sh, s);
-    if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, true, sh, s);
+    if (e == hipSuccess) e = launch_forward(ck, d, logn, nmoduli, nmoduli, true, sh, s);
+    if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, nmoduli, true, sh, s);
     for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
       const size_t nr = std::min<size_t>(65535, nmoduli - r0);
       e = launch_pointwise(PW_MUL, ck + r0 * degree, ck + r0 * degree, tmp + r0 * degree, d + r0,
                            degree, nr, s);
     }
-    if (e == hipSuccess) e = launch_backward(ck, d, logn, nmoduli, sh, s);
+    if (e == hipSuccess) e = launch_backward(ck, d, logn, nmoduli, nmoduli, sh, s);
   }
   hipError_t ef = hipFreeAsync(tmp, s);
   if (e == hipSuccess) e = ef;

@@ -20,11 +20,12 @@ struct NttShape {
   int le = 4;
 };
 
-// forward / forward_lazy over `nres` residues of 2^logn words, residue r at
-// data + r * 2^logn with descriptor descs[r].
-hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, bool lazy,
-                          NttShape shape, hipStream_t st);
-hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
+// forward / forward_lazy / backward over `nres` residues of 2^logn words,
+// residue r at data + r * 2^logn with descriptor descs[r % nmod] (nmod = nres
+// for one polynomial; nmod = L for nres / L polynomials of L residues).
+hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
+                          bool lazy, NttShape shape, hipStream_t st);
+hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                            NttShape shape, hipStream_t st);
 
 // negacylic_backward_lazy (lib/ntt.cpp:151-210): every inverse layer except
@@ -38,6 +39,17 @@ hipError_t launch_inverse_lazy(uint64_t *data, const ModDesc *descs, int logn, s
 // alias a or b.  Rings up to 2^14 only.
 hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
                           int logn, size_t nres, size_t nmod, NttShape shape, hipStream_t st);
+
+// acc = acc + sum_j a[j] * b[j] (mod p), the pairs applied in order j = 0..n-1
+// with one Barrett muladd each (details/poly.hpp:119-132 per pair); at most
+// kMacPairs pairs per launch.
+constexpr int kMacPairs = 16;
+struct MacPairs {
+  const uint64_t *a[kMacPairs];
+  const uint64_t *b[kMacPairs];
+};
+hipError_t launch_mac(uint64_t *acc, const MacPairs &pairs, int npairs, const ModDesc *descs,
+                      size_t n, size_t nres, hipStream_t st);
 
 enum PwOp { PW_MUL = 0, PW_ADD = 1, PW_SUB = 2, PW_NEGATE = 3, PW_MULADD = 4, PW_CORRECT = 5 };
 

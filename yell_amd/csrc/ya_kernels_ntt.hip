@@ -7,12 +7,13 @@ namespace ya {
 // ---------------------------------------------------------------------------
 template <int LB, int LE, bool LAZY>
 __global__ void __launch_bounds__(1 << (LB - LE))
-k_fwd_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nlog) {
+k_fwd_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nlog,
+            uint32_t nmod) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
   const uint32_t nblk_log = nlog - LB;
   const uint32_t res = blockIdx.x >> nblk_log;
   const uint32_t blk = blockIdx.x & ((1u << nblk_log) - 1u);
-  const ModDesc d = descs[res];
+  const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << nlog) + ((size_t)blk << LB);
   const uint32_t t = threadIdx.x;
   uint64_t x[1 << LE];
@@ -23,12 +24,13 @@ k_fwd_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint
 
 template <int LB, int LE, int TOP>
 __global__ void __launch_bounds__(1 << (LB - LE))
-k_inv_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nlog) {
+k_inv_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nlog,
+            uint32_t nmod) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
   const uint32_t nblk_log = nlog - LB;
   const uint32_t res = blockIdx.x >> nblk_log;
   const uint32_t blk = blockIdx.x & ((1u << nblk_log) - 1u);
-  const ModDesc d = descs[res];
+  const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << nlog) + ((size_t)blk << LB);
   const uint32_t t = threadIdx.x;
   uint64_t x[1 << LE];
@@ -42,27 +44,27 @@ constexpr int kTopThreads = 256;
 
 template <int K>
 __global__ void __launch_bounds__(kTopThreads)
-k_fwd_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs) {
+k_fwd_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod) {
   constexpr int NLOG = kMaxBlockLog + K;
   const uint32_t cols_per_res_log = kMaxBlockLog - 8;  // 2^14 / 256 workgroups per residue
   const uint32_t res = blockIdx.x >> cols_per_res_log;
   const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << 8) | threadIdx.x;
-  const ModDesc d = descs[res];
+  const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << NLOG);
   uint64_t x[1 << K];
   gmem_load<kMaxBlockLog, K>(base, x, t);
-  fwd_layers<kMaxBlockLog, K, NLOG, kMaxBlockLog>(x, d.fwd, NLOG, 0u, d.p);
+  fwd_layers<kMaxBlockLog, K, NLOG, kMaxBlockLog, true>(x, d.fwd, NLOG, 0u, d.p);
   gmem_store<kMaxBlockLog, K>(base, x, t);
 }
 
 template <int K, bool MERGED>
 __global__ void __launch_bounds__(kTopThreads)
-k_inv_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs) {
+k_inv_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod) {
   constexpr int NLOG = kMaxBlockLog + K;
   const uint32_t cols_per_res_log = kMaxBlockLog - 8;
   const uint32_t res = blockIdx.x >> cols_per_res_log;
   const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << 8) | threadIdx.x;
-  const ModDesc d = descs[res];
+  const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << NLOG);
   uint64_t x[1 << K];
   gmem_load<kMaxBlockLog, K>(base, x, t);
@@ -74,45 +76,45 @@ k_inv_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs) {
 namespace {
 
 template <int LB, int LE>
-hipError_t fwd_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, bool lazy,
-                  hipStream_t st) {
+hipError_t fwd_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
+                  bool lazy, hipStream_t st) {
   const dim3 grid((unsigned)(nres << (logn - LB)));
   const dim3 block(1u << (LB - LE));
   if (lazy) {
-    hipLaunchKernelGGL((k_fwd_block<LB, LE, true>), grid, block, 0, st, data, descs, (uint32_t)logn);
+    hipLaunchKernelGGL((k_fwd_block<LB, LE, true>), grid, block, 0, st, data, descs, (uint32_t)logn, (uint32_t)nmod);
   } else {
-    hipLaunchKernelGGL((k_fwd_block<LB, LE, false>), grid, block, 0, st, data, descs, (uint32_t)logn);
+    hipLaunchKernelGGL((k_fwd_block<LB, LE, false>), grid, block, 0, st, data, descs, (uint32_t)logn, (uint32_t)nmod);
   }
   return hipGetLastError();
 }
 
 template <int LB, int LE>
-hipError_t inv_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, bool lazy,
-                  hipStream_t st) {
+hipError_t inv_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
+                  bool lazy, hipStream_t st) {
   const dim3 grid((unsigned)(nres << (logn - LB)));
   const dim3 block(1u << (LB - LE));
   if (logn != LB) {
-    hipLaunchKernelGGL((k_inv_block<LB, LE, TOP_GS>), grid, block, 0, st, data, descs, (uint32_t)logn);
+    hipLaunchKernelGGL((k_inv_block<LB, LE, TOP_GS>), grid, block, 0, st, data, descs, (uint32_t)logn, (uint32_t)nmod);
   } else if (lazy) {
-    hipLaunchKernelGGL((k_inv_block<LB, LE, TOP_SKIP>), grid, block, 0, st, data, descs, (uint32_t)logn);
+    hipLaunchKernelGGL((k_inv_block<LB, LE, TOP_SKIP>), grid, block, 0, st, data, descs, (uint32_t)logn, (uint32_t)nmod);
   } else {
-    hipLaunchKernelGGL((k_inv_block<LB, LE, TOP_MERGED>), grid, block, 0, st, data, descs, (uint32_t)logn);
+    hipLaunchKernelGGL((k_inv_block<LB, LE, TOP_MERGED>), grid, block, 0, st, data, descs, (uint32_t)logn, (uint32_t)nmod);
   }
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, bool lazy,
-                          NttShape shape, hipStream_t st) {
-  if (logn < kMinLogN || logn > kMaxLogN || nres == 0) return hipErrorInvalidValue;
+hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
+                          bool lazy, NttShape shape, hipStream_t st) {
+  if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0) return hipErrorInvalidValue;
   if (logn > kMaxBlockLog) {
     const int k = logn - kMaxBlockLog;
     const dim3 grid((unsigned)(nres << (kMaxBlockLog - 8)));
     switch (k) {
-      case 1: hipLaunchKernelGGL(k_fwd_top<1>, grid, dim3(kTopThreads), 0, st, data, descs); break;
-      case 2: hipLaunchKernelGGL(k_fwd_top<2>, grid, dim3(kTopThreads), 0, st, data, descs); break;
-      case 3: hipLaunchKernelGGL(k_fwd_top<3>, grid, dim3(kTopThreads), 0, st, data, descs); break;
+      case 1: hipLaunchKernelGGL(k_fwd_top<1>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+      case 2: hipLaunchKernelGGL(k_fwd_top<2>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+      case 3: hipLaunchKernelGGL(k_fwd_top<3>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
       default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
@@ -120,19 +122,19 @@ hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t
   }
   const int lb = logn < kMaxBlockLog ? logn : kMaxBlockLog;
   const int le = clamp_le(lb, shape.le);
-#define YA_FWD_CALL(LB, LE) fwd_lb<LB, LE>(data, descs, logn, nres, lazy, st)
+#define YA_FWD_CALL(LB, LE) fwd_lb<LB, LE>(data, descs, logn, nres, nmod, lazy, st)
   YA_LB_SWITCH(lb, le, YA_FWD_CALL)
 #undef YA_FWD_CALL
 }
 
 static hipError_t backward_impl(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
-                                bool lazy, NttShape shape, hipStream_t st) {
-  if (logn < kMinLogN || logn > kMaxLogN || nres == 0) return hipErrorInvalidValue;
+                                size_t nmod, bool lazy, NttShape shape, hipStream_t st) {
+  if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0) return hipErrorInvalidValue;
   const int lb = logn < kMaxBlockLog ? logn : kMaxBlockLog;
   const int le = clamp_le(lb, shape.le);
   hipError_t e;
   {
-#define YA_INV_CALL(LB, LE) inv_lb<LB, LE>(data, descs, logn, nres, lazy, st)
+#define YA_INV_CALL(LB, LE) inv_lb<LB, LE>(data, descs, logn, nres, nmod, lazy, st)
     e = [&]() -> hipError_t { YA_LB_SWITCH(lb, le, YA_INV_CALL) }();
 #undef YA_INV_CALL
   }
@@ -141,25 +143,25 @@ static hipError_t backward_impl(uint64_t *data, const ModDesc *descs, int logn, 
   const dim3 grid((unsigned)(nres << (kMaxBlockLog - 8)));
   const dim3 blk(kTopThreads);
   switch (k * 2 + (lazy ? 0 : 1)) {
-    case 2: hipLaunchKernelGGL((k_inv_top<1, false>), grid, blk, 0, st, data, descs); break;
-    case 3: hipLaunchKernelGGL((k_inv_top<1, true>), grid, blk, 0, st, data, descs); break;
-    case 4: hipLaunchKernelGGL((k_inv_top<2, false>), grid, blk, 0, st, data, descs); break;
-    case 5: hipLaunchKernelGGL((k_inv_top<2, true>), grid, blk, 0, st, data, descs); break;
-    case 6: hipLaunchKernelGGL((k_inv_top<3, false>), grid, blk, 0, st, data, descs); break;
-    case 7: hipLaunchKernelGGL((k_inv_top<3, true>), grid, blk, 0, st, data, descs); break;
+    case 2: hipLaunchKernelGGL((k_inv_top<1, false>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
+    case 3: hipLaunchKernelGGL((k_inv_top<1, true>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
+    case 4: hipLaunchKernelGGL((k_inv_top<2, false>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
+    case 5: hipLaunchKernelGGL((k_inv_top<2, true>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
+    case 6: hipLaunchKernelGGL((k_inv_top<3, false>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
+    case 7: hipLaunchKernelGGL((k_inv_top<3, true>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
+hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                            NttShape shape, hipStream_t st) {
-  return backward_impl(data, descs, logn, nres, false, shape, st);
+  return backward_impl(data, descs, logn, nres, nmod, false, shape, st);
 }
 
 hipError_t launch_inverse_lazy(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
                                hipStream_t st) {
-  return backward_impl(data, descs, logn, nres, true, NttShape{}, st);
+  return backward_impl(data, descs, logn, nres, nres, true, NttShape{}, st);
 }
 
 }  // namespace ya

@@ -31,6 +31,36 @@ k_pointwise(uint64_t *__restrict__ out, const uint64_t *__restrict__ a,
   out[k + 1] = o[1];
 }
 
+// acc += a_j * b_j for j = 0..np-1, one Barrett muladd per pair (the
+// reference's add_product_of applied pair after pair), two words per thread.
+__global__ void __launch_bounds__(kPwThreads)
+k_mac(uint64_t *__restrict__ acc, MacPairs pairs, int np, const ModDesc *__restrict__ descs,
+      uint32_t n) {
+  const uint32_t r = blockIdx.y;
+  const uint32_t i = (blockIdx.x * kPwThreads + threadIdx.x) * 2u;
+  if (i >= n) return;
+  const uint64_t p = descs[r].p, pn = descs[r].pn;
+  const size_t k = (size_t)r * n + i;
+  uint64_t o0 = acc[k], o1 = acc[k + 1];
+  for (int j = 0; j < np; ++j) {
+    const uint64_t *a = pairs.a[j] + k, *b = pairs.b[j] + k;
+    o0 = muladd_barrett(o0, a[0], b[0], p, pn);
+    o1 = muladd_barrett(o1, a[1], b[1], p, pn);
+  }
+  acc[k] = o0;
+  acc[k + 1] = o1;
+}
+
+hipError_t launch_mac(uint64_t *acc, const MacPairs &pairs, int npairs, const ModDesc *descs,
+                      size_t n, size_t nres, hipStream_t st) {
+  if (n < 2 || (n & 1) || nres == 0 || nres > 65535 || npairs < 0 || npairs > kMacPairs)
+    return hipErrorInvalidValue;
+  if (npairs == 0) return hipSuccess;
+  const dim3 grid((unsigned)((n / 2 + kPwThreads - 1) / kPwThreads), (unsigned)nres);
+  hipLaunchKernelGGL(k_mac, grid, dim3(kPwThreads), 0, st, acc, pairs, npairs, descs, (uint32_t)n);
+  return hipGetLastError();
+}
+
 hipError_t launch_pointwise(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
                             const ModDesc *descs, size_t n, size_t nres, hipStream_t st) {
   if (n < 2 || (n & 1) || nres == 0 || nres > 65535) return hipErrorInvalidValue;

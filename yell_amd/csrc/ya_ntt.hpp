@@ -128,10 +128,12 @@ __device__ __forceinline__ void gmem_store(uint64_t *__restrict__ g, const uint6
 // index of word 0 of this thread with the window bits and the bits below the
 // window cleared (i.e. block offset | thi << (WLO+LE)); nlog = log2 N of the
 // whole residue.
-template <int WLO, int LE, int HI, int LO>
+// FIRST: layer HI-1 is the first layer applied to the caller's words (exact
+// reduction compare, see ct_butterfly).
+template <int WLO, int LE, int HI, int LO, bool FIRST = false>
 __device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
                                            uint32_t nlog, uint32_t gbase, uint64_t p) {
-  const uint64_t p2 = p << 1;
+  const PrimeK k(p);
 #pragma unroll
   for (int b = HI - 1; b >= LO; --b) {
     const int j = b - WLO;
@@ -140,19 +142,21 @@ __device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__r
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
       const Tw t = load_tw(tw + toff + g);
 #pragma unroll
-      for (int k = 0; k < (1 << j); ++k) {
-        const int e0 = (g << (j + 1)) | k;
-        ct_butterfly(x[e0], x[e0 | (1 << j)], t, p, p2);
+      for (int q = 0; q < (1 << j); ++q) {
+        const int e0 = (g << (j + 1)) | q;
+        if (FIRST && b == HI - 1) ct_butterfly<true>(x[e0], x[e0 | (1 << j)], t, k);
+        else ct_butterfly<false>(x[e0], x[e0 | (1 << j)], t, k);
       }
     }
   }
 }
 
 // Inverse (GS) layers b = LO ... HI-1 of one pass (same indexing).
-template <int WLO, int LE, int HI, int LO>
+// FIRST: layer LO is the first layer applied to the caller's words.
+template <int WLO, int LE, int HI, int LO, bool FIRST = false>
 __device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
                                            uint32_t nlog, uint32_t gbase, uint64_t p) {
-  const uint64_t p2 = p << 1;
+  const PrimeK k(p);
 #pragma unroll
   for (int b = LO; b < HI; ++b) {
     const int j = b - WLO;
@@ -161,9 +165,10 @@ __device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__r
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
       const Tw t = load_tw(tw + toff + g);
 #pragma unroll
-      for (int k = 0; k < (1 << j); ++k) {
-        const int e0 = (g << (j + 1)) | k;
-        gs_butterfly(x[e0], x[e0 | (1 << j)], t, p, p2);
+      for (int q = 0; q < (1 << j); ++q) {
+        const int e0 = (g << (j + 1)) | q;
+        if (FIRST && b == LO) gs_butterfly<true>(x[e0], x[e0 | (1 << j)], t, k);
+        else gs_butterfly<false>(x[e0], x[e0 | (1 << j)], t, k);
       }
     }
   }

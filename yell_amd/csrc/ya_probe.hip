@@ -36,13 +36,13 @@ k_probe(uint64_t *__restrict__ out, uint32_t iters, uint64_t seed, uint64_t p) {
 #pragma unroll
     for (int k = 0; k < kChains; ++k) v[k] = w[k];
   } else {  // CT butterflies over 16 words, 8 per iteration
-    const uint64_t p2 = p << 1;
+    const PrimeK pk(p);
     const Tw t{(seed % p), ((seed % p) * 3) | 1};
 #pragma unroll
     for (int k = 0; k < kChains * 2; ++k) v[k] %= (p << 2);
     for (uint32_t it = 0; it < iters; ++it) {
 #pragma unroll
-      for (int k = 0; k < kChains; ++k) ct_butterfly(v[k], v[k + kChains], t, p, p2);
+      for (int k = 0; k < kChains; ++k) ct_butterfly<false>(v[k], v[k + kChains], t, pk);
     }
   }
   uint64_t acc = 0;
