@@ -284,6 +284,8 @@ def main():
     capi.call("ya_ntt_init_tables", n, L)
     plan = make_plan([(da.ptr + k * w * 8, db.ptr + k * w * 8, dc.ptr + k * w * 8, n, L)
                       for k in range(B)])
+    form = ctypes.c_int()
+    capi.call("ya_batch_plan_info", plan, None, None, None, ctypes.byref(form))
 
     def batched():
         capi.call("ya_batch_plan_run_graph", plan, stream)
@@ -325,7 +327,8 @@ def main():
         "data": "synthetic (seeded uniform x Gaussian sigma=3.2 operands)",
         "config": {"workload": "RNS poly-multiplication N=16384 L=8 (P[0..7]), a uniform, "
                                "b Gaussian sigma=3.2; %d polys per batched launch "
-                               "(persistent kernel replayed as a hipGraph)" % B,
+                               "(batch plan replayed as a hipGraph)" % B,
+                   "launch_form": {1: "persistent", 2: "grid"}.get(form.value, "?"),
                    "degree": n, "nmoduli": L, "batch": B},
         "plain": {"value": plain_value, "unit": "poly-mults/s", "ms_per_step": ms_plain,
                   "launches_per_step": B},
@@ -334,7 +337,8 @@ def main():
                       "kernel": "k_polymul<14,4>, one workgroup per residue"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_batch_polymul", "bytes_per_polymul": bytes_per,
+                     "kernel": "k_polymul_tab<14,4>" if form.value == 2 else "k_batch_polymul",
+                     "bytes_per_polymul": bytes_per,
                      "polymuls_per_launch": B},
         "modmul_rate": {"modmuls_per_polymul": mm,
                         "achieved_T_per_s": mm * B / (ms_step / 1e3) / 1e12},

@@ -182,8 +182,16 @@ int ya_batch_plan_create(ya_batch_plan **plan, const ya_polymul_job *jobs, size_
 int ya_batch_plan_run(ya_batch_plan *plan, void *stream);
 /* Same work as a hipGraph: captured on first use, then replayed on `stream`. */
 int ya_batch_plan_run_graph(ya_batch_plan *plan, void *stream);
+/* Launch forms: a batch of one degree class <= 2^14 holding at least two
+ * residues per CU runs as a grid of the fused kernel over the plan's residue
+ * table (YA_BATCH_GRID); any other batch drains the persistent kernel's
+ * queue (YA_BATCH_PERSISTENT).  Both give the same words. */
+enum { YA_BATCH_PERSISTENT = 1, YA_BATCH_GRID = 2 };
 int ya_batch_plan_info(const ya_batch_plan *plan, size_t *items, size_t *residues,
-                       size_t *workgroups);
+                       size_t *workgroups, int *form);
+/* Forces the launch form of plans created afterwards (0 = automatic; GRID
+ * only where the batch allows it). */
+int ya_batch_force_form(int form);
 int ya_batch_plan_destroy(ya_batch_plan *plan);
 /* One-shot: create, run on `stream`, wait, destroy. */
 int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
@@ -224,6 +232,9 @@ typedef struct ya_server_config {
 typedef struct ya_server_stats {
   uint64_t submitted, completed, rejected, batches, max_pending, arena_bytes_in_use;
   double mean_batch_jobs, p50_us, p90_us, p99_us, max_us, mean_us;
+  /* split of the latency: queue = submit -> dispatched in a batch,
+   * service = dispatched -> result in c (copies + kernel + retirement) */
+  double p50_queue_us, p99_queue_us, p50_service_us, p99_service_us;
 } ya_server_stats;
 int ya_server_create(ya_server **server, const ya_server_config *config);
 /* YA_ERR_REJECTED when the request can never fit or the queue is full. */
@@ -253,6 +264,7 @@ typedef struct ya_serve_report {
   uint64_t offered, completed, rejected, failed, mismatches, batches, max_pending;
   uint64_t client_stalls;  /* arrivals delayed because every result buffer was in use */
   double wall_s, throughput_per_s, p50_us, p90_us, p99_us, max_us, mean_us, mean_batch_jobs;
+  double p50_queue_us, p99_queue_us, p50_service_us, p99_service_us;
 } ya_serve_report;
 int ya_server_run_poisson(ya_server *server, const ya_poisson_spec *spec, ya_serve_report *report);
 

@@ -49,7 +49,7 @@ def test_uneven_batch_vs_oracle():
     say("batch uneven", [(n, L) for n, L, _, _ in host])
     bt = Batch(host)
     items, res, wgs = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
-    capi.call("ya_batch_plan_info", bt.plan, ctypes.byref(items), ctypes.byref(res), ctypes.byref(wgs))
+    capi.call("ya_batch_plan_info", bt.plan, ctypes.byref(items), ctypes.byref(res), ctypes.byref(wgs), None)
     assert res.value == sum(L for _, L, _, _ in host)
     for rep in range(2):   # the queue resets itself between launches
         capi.call("ya_batch_plan_run", bt.plan, None)
@@ -147,3 +147,25 @@ def test_autotune_runs_and_fills_table():
     capi.call("ya_autotune", 12, 13, rep)
     assert all(rep[6 * c + k] > 0 for c in range(2) for k in (0, 1, 3, 4, 5))
     assert lib.ya_tune_get(2, 12) in (3, 4) and lib.ya_tune_get(1, 13) in (3, 4)
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_both_launch_forms_of_a_uniform_batch(form):
+    """A one-class batch as the persistent queue and as the residue-table grid."""
+    lib = device()
+    rng = np.random.default_rng(11)
+    host = _jobs(rng, 40, [14], 8)
+    capi.call("ya_batch_force_form", form)
+    try:
+        bt = Batch(host)
+        got_form = ctypes.c_int()
+        capi.call("ya_batch_plan_info", bt.plan, None, None, None, ctypes.byref(got_form))
+        assert got_form.value == form
+        say("launch form", form)
+        capi.call("ya_batch_plan_run_graph", bt.plan, None)
+        capi.call("ya_device_synchronize")
+        for (n, L, a, b), got in zip(host, bt.results()):
+            np.testing.assert_array_equal(got, orc.polymul(a, b, n, L))
+        bt.close()
+    finally:
+        capi.call("ya_batch_force_form", 0)

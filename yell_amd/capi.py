@@ -79,7 +79,9 @@ _SIGS = {
     "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),
     "ya_batch_plan_run": (i32, [vp, vp]),
     "ya_batch_plan_run_graph": (i32, [vp, vp]),
-    "ya_batch_plan_info": (i32, [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+    "ya_batch_plan_info": (i32, [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz),
+                                 ctypes.POINTER(i32)]),
+    "ya_batch_force_form": (i32, [i32]),
     "ya_batch_plan_destroy": (i32, [vp]),
     "ya_batch_polymul": (i32, [vp, sz, vp]),
     "ya_autotune": (i32, [i32, i32, ctypes.POINTER(ctypes.c_double)]),
@@ -106,7 +108,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "rejected", "batches",
                                               "max_pending", "arena_bytes_in_use")] + \
                [(k, ctypes.c_double) for k in ("mean_batch_jobs", "p50_us", "p90_us", "p99_us",
-                                              "max_us", "mean_us")]
+                                              "max_us", "mean_us", "p50_queue_us", "p99_queue_us",
+                                              "p50_service_us", "p99_service_us")]
 
 
 class PoissonSpec(ctypes.Structure):
@@ -121,7 +124,9 @@ class ServeReport(ctypes.Structure):
                                               "mismatches", "batches", "max_pending",
                                               "client_stalls")] + \
                [(k, ctypes.c_double) for k in ("wall_s", "throughput_per_s", "p50_us", "p90_us",
-                                              "p99_us", "max_us", "mean_us", "mean_batch_jobs")]
+                                              "p99_us", "max_us", "mean_us", "mean_batch_jobs",
+                                              "p50_queue_us", "p99_queue_us", "p50_service_us",
+                                              "p99_service_us")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

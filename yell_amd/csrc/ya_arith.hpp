@@ -99,72 +99,32 @@ __device__ __forceinline__ uint64_t mulmod_shoup(uint64_t x, uint64_t y, uint64_
 // Per-prime constants of the butterflies (wave-uniform, scalar registers).
 struct PrimeK {
   uint64_t p, p2;    // p, 2p
-  uint64_t negp;     // 2^64 - p
-  uint64_t neg2p;    // 2^64 - 2p
-  __device__ __forceinline__ explicit PrimeK(uint64_t q) : p(q), p2(q << 1), negp(0 - q), neg2p(0 - (q << 1)) {}
+  __device__ __forceinline__ explicit PrimeK(uint64_t q) : p(q), p2(q << 1) {}
 };
 
 // Shoup product without the final correction: x * w mod p in [0, 2p), the
-// word x*w - hi64(x*w')*p (mod 2^64) of the reference, computed as
-// lo64(x*w + q*(2^64 - p)) so the subtraction folds into the 64-bit addend of
-// v_mad_u64_u32.
+// word x*w - hi64(x*w')*p (mod 2^64) of the reference.
 __device__ __forceinline__ uint64_t mulmod_shoup_lazy(uint64_t x, const Tw &t, const PrimeK &k) {
   const uint64_t q = mulhi64(x, t.ws);
-  const uint64_t r = mullo64(q, k.negp);                       // -q*p mod 2^64
-  const uint32_t a0 = lo32(x), a1 = hi32(x);
-  const uint64_t v0 = (uint64_t)a0 * lo32(t.w) + r;            // one mad, 64-bit addend
-  const uint32_t cross = a0 * hi32(t.w) + a1 * lo32(t.w);
-  return v0 + ((uint64_t)cross << 32);
+  return mullo64(x, t.w) - mullo64(q, k.p);
 }
 
-__device__ __forceinline__ uint64_t mulmod_shoup_lazy(uint64_t x, const Tw &t, uint64_t p) {
-  return mulmod_shoup_lazy(x, t, PrimeK(p));
-}
-
-// v >= 2p ? v - 2p : v for v < 4p: d = v - 2p (mod 2^64) has its top bit set
-// exactly when v < 2p (|v - 2p| < 2^63 because p < 2^62); the select is a
-// per-bit blend with the sign mask (no compare, no carry chain).
-__device__ __forceinline__ uint64_t sub2p_if_ge(uint64_t v, const PrimeK &k) {
-  const uint64_t d = v + k.neg2p;
-  const uint32_t m = (uint32_t)((int32_t)hi32(d) >> 31);
-  const uint32_t lo = (m & lo32(v)) | (~m & lo32(d));
-  const uint32_t hi = (m & hi32(v)) | (~m & hi32(d));
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// lib/ntt.cpp:69-83.  EXACT: the first layer of a transform sees the
-// caller's words; its reduction uses a full compare so that even words
-// outside the documented [0, 4p) contract are treated as the reference does.
-template <bool EXACT = false>
+// lib/ntt.cpp:69-83
 __device__ __forceinline__ void ct_butterfly(uint64_t &x0, uint64_t &x1, const Tw &t, const PrimeK &k) {
-  uint64_t u0;
-  if constexpr (EXACT) u0 = x0 >= k.p2 ? x0 - k.p2 : x0;
-  else u0 = sub2p_if_ge(x0, k);
+  const uint64_t u0 = x0 >= k.p2 ? x0 - k.p2 : x0;
   const uint64_t v = mulmod_shoup_lazy(x1, t, k);
   x0 = u0 + v;
   x1 = u0 + k.p2 - v;
 }
 
-__device__ __forceinline__ void ct_butterfly(uint64_t &x0, uint64_t &x1, const Tw &t, uint64_t p,
-                                             uint64_t) {
-  ct_butterfly<true>(x0, x1, t, PrimeK(p));
-}
-
-// lib/ntt.cpp:47-63 (EXACT as above).
-template <bool EXACT = false>
+// lib/ntt.cpp:47-63
 __device__ __forceinline__ void gs_butterfly(uint64_t &x0, uint64_t &x1, const Tw &t, const PrimeK &k) {
   const uint64_t u0 = x0, u1 = x1;
   uint64_t t0 = u0 + u1;
-  if constexpr (EXACT) t0 = t0 >= k.p2 ? t0 - k.p2 : t0;
-  else t0 = sub2p_if_ge(t0, k);
+  t0 = t0 >= k.p2 ? t0 - k.p2 : t0;
   const uint64_t t1 = u0 + k.p2 - u1;
   x0 = t0;
   x1 = mulmod_shoup_lazy(t1, t, k);
-}
-
-__device__ __forceinline__ void gs_butterfly(uint64_t &x0, uint64_t &x1, const Tw &t, uint64_t p,
-                                             uint64_t) {
-  gs_butterfly<true>(x0, x1, t, PrimeK(p));
 }
 
 // include/yell/details/ntt.hpp:111-118: [0,4p) -> [0,p).

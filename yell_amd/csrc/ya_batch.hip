@@ -10,6 +10,7 @@
 // Arithmetic and layer order are those of the plain kernels (the reference's
 // butterflies, lib/ntt.cpp:47-83), so each result equals ya_poly_multiply's.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <functional>
@@ -70,7 +71,7 @@ __device__ __forceinline__ void top15_fwd(const uint64_t *__restrict__ src, uint
   for (uint32_t q = 0; q < 16; ++q) {
     const uint32_t i = t + q * kBatchThreads;
     uint64_t x0 = src[i], x1 = src[i + (1u << 14)];
-    ct_butterfly<true>(x0, x1, w, k);  // first layer: the caller's words
+    ct_butterfly(x0, x1, w, k);
     dst[i] = x0;
     dst[i + (1u << 14)] = x1;
   }
@@ -198,9 +199,13 @@ struct ya_batch_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double total_polymuls = 0;
+  int form = YA_BATCH_PERSISTENT;  // launch form chosen at creation
+  int grid_logn = 0;               // degree class of the grid form
 };
 
 namespace {
+
+std::atomic<int> g_form_override{0};
 
 int plan_free(ya_batch_plan *pl) {
   if (!pl) return YA_OK;
@@ -341,14 +346,32 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
     return hip_fail(e, "batch plan upload");
   }
   pl->total_polymuls = (double)njobs;
+  // Launch form: a batch of one degree class <= 2^14 with at least two
+  // residues per CU runs as a grid of the fused kernel over the residue table
+  // (no queue, fewer instructions per residue); mixed batches and 2^15 rings
+  // use the persistent kernel.  g_form_override forces one form (tests).
+  bool one_class = true;
+  for (const auto &it : items) one_class &= it.logn == items[0].logn;
+  const int lg = (int)items[0].logn;
+  const bool grid_ok = one_class && lg <= 14;
+  int form = (grid_ok && res.size() >= 2 * (size_t)cus) ? YA_BATCH_GRID : YA_BATCH_PERSISTENT;
+  const int ov = g_form_override.load();
+  if (ov == YA_BATCH_PERSISTENT || (ov == YA_BATCH_GRID && grid_ok)) form = ov;
+  pl->form = form;
+  pl->grid_logn = lg;
   *out = pl;
   return YA_OK;
 }
 
+static hipError_t plan_launch(ya_batch_plan *pl, hipStream_t s) {
+  if (pl->form == YA_BATCH_GRID)
+    return launch_polymul_table(pl->d_res, pl->nres, pl->grid_logn, tuned_shape(pl->grid_logn, 1), s);
+  return launch_batch(pl->d_items, pl->segs, pl->d_res, pl->d_q, pl->d_scratch, pl->grid, s);
+}
+
 int ya_batch_plan_run(ya_batch_plan *pl, void *stream) {
   if (!pl) return fail(YA_ERR_INVALID, "null plan");
-  hipError_t e = launch_batch(pl->d_items, pl->segs, pl->d_res, pl->d_q, pl->d_scratch,
-                              pl->grid, (hipStream_t)stream);
+  hipError_t e = plan_launch(pl, (hipStream_t)stream);
   return e == hipSuccess ? YA_OK : hip_fail(e, "batch launch");
 }
 
@@ -362,9 +385,7 @@ int ya_batch_plan_run_graph(ya_batch_plan *pl, void *stream) {
     if (e != hipSuccess) return hip_fail(e, "capture stream");
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_batch_polymul, dim3(pl->grid), dim3(kBatchThreads), 0, cap, pl->d_items,
-                         pl->segs, pl->d_res, pl->d_q, pl->d_scratch);
-      e = hipGetLastError();
+      e = plan_launch(pl, cap);
       hipError_t e2 = hipStreamEndCapture(cap, &pl->graph);
       if (e == hipSuccess) e = e2;
     }
@@ -378,11 +399,20 @@ int ya_batch_plan_run_graph(ya_batch_plan *pl, void *stream) {
 
 int ya_batch_plan_destroy(ya_batch_plan *pl) { return plan_free(pl); }
 
-int ya_batch_plan_info(const ya_batch_plan *pl, size_t *items, size_t *residues, size_t *workgroups) {
+int ya_batch_plan_info(const ya_batch_plan *pl, size_t *items, size_t *residues, size_t *workgroups,
+                       int *form) {
   if (!pl) return fail(YA_ERR_INVALID, "null plan");
   if (items) *items = pl->nitems;
   if (residues) *residues = pl->nres;
-  if (workgroups) *workgroups = pl->grid;
+  if (workgroups) *workgroups = pl->form == YA_BATCH_GRID ? pl->nres : pl->grid;
+  if (form) *form = pl->form;
+  return YA_OK;
+}
+
+int ya_batch_force_form(int form) {
+  if (form != 0 && form != YA_BATCH_PERSISTENT && form != YA_BATCH_GRID)
+    return fail(YA_ERR_INVALID, "bad launch form %d", form);
+  g_form_override.store(form);
   return YA_OK;
 }
 

@@ -57,4 +57,9 @@ int build_batch(DeviceState *st, const ya_polymul_job *jobs, size_t njobs,
 hipError_t launch_batch(const BatchItem *items, const BatchSegs &segs, const BatchRes *res,
                         BatchQueue *q, uint64_t *scratch, unsigned grid, hipStream_t s);
 
+// Grid launch of the fused poly-multiplication over a residue table whose
+// residues are all of degree 2^logn (<= 2^14), one workgroup per residue.
+hipError_t launch_polymul_table(const BatchRes *res, size_t nres, int logn, NttShape shape,
+                                hipStream_t st);
+
 }  // namespace ya

@@ -51,7 +51,7 @@ __device__ __forceinline__ void fwd_block(uint64_t (&x)[1 << LE], const uint64_t
       lds_load<WLO, LE>(lds, x, t);
     }
     const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
-    fwd_layers<WLO, LE, HI, LO, P == 0>(x, tw, nlog, gbase, p);
+    fwd_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, p);
     if constexpr (P + 1 < NP) {
       __syncthreads();
       lds_store<WLO, LE>(lds, x, t);
@@ -83,12 +83,12 @@ __device__ __forceinline__ void inv_block(uint64_t (&x)[1 << LE], uint64_t *__re
     if constexpr (Q > 0) lds_load<WLO, LE>(lds, x, t);
     const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
     if constexpr (LASTQ && TOP == TOP_MERGED) {
-      inv_layers<WLO, LE, HI - 1, LO, Q == 0>(x, tw, nlog, gbase, d.p);
+      inv_layers<WLO, LE, HI - 1, LO>(x, tw, nlog, gbase, d.p);
       inv_last_layer<LE, LB - 1 - WLO>(x, d);
     } else if constexpr (LASTQ && TOP == TOP_SKIP) {
-      inv_layers<WLO, LE, HI - 1, LO, Q == 0>(x, tw, nlog, gbase, d.p);
+      inv_layers<WLO, LE, HI - 1, LO>(x, tw, nlog, gbase, d.p);
     } else {
-      inv_layers<WLO, LE, HI, LO, Q == 0>(x, tw, nlog, gbase, d.p);
+      inv_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, d.p);
     }
     if constexpr (!LASTQ) {
       __syncthreads();

@@ -53,7 +53,7 @@ k_fwd_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32
   uint64_t *base = data + ((size_t)res << NLOG);
   uint64_t x[1 << K];
   gmem_load<kMaxBlockLog, K>(base, x, t);
-  fwd_layers<kMaxBlockLog, K, NLOG, kMaxBlockLog, true>(x, d.fwd, NLOG, 0u, d.p);
+  fwd_layers<kMaxBlockLog, K, NLOG, kMaxBlockLog>(x, d.fwd, NLOG, 0u, d.p);
   gmem_store<kMaxBlockLog, K>(base, x, t);
 }
 

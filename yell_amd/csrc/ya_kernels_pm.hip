@@ -1,53 +1,79 @@
 // ya_kernels_pm.hip -- fused RNS poly-multiplication kernel.
+#include "ya_batch.hpp"
 #include "ya_blocks.hpp"
 
 namespace ya {
 
-// Fused c = backward(forward(a) (*) forward(b)) for one residue of 2^LB words
-// per workgroup.  forward(a) is parked in c at the positions this thread
+// Fused c = backward(forward(a) (*) forward(b)) of one residue of 2^LB words
+// by one workgroup.  forward(a) is parked in c at the positions this thread
 // later reads back itself, so no other buffer is needed.
 template <int LB, int LE>
-__global__ void __launch_bounds__(1 << (LB - LE))
-k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
-          const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod) {
-  __shared__ uint64_t lds[LdsWords<LB>::value];
-  // XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (the
-  // hardware deals consecutive workgroups to consecutive XCDs); give each XCD
-  // a contiguous range of the residues in prime-major order, so its L2 holds
-  // the twiddles of few primes.
-  const uint32_t g = gridDim.x, xcd = blockIdx.x % 8u, pos = blockIdx.x / 8u;
-  const uint32_t v = xcd * (g / 8u) + min(xcd, g % 8u) + pos;
-  const uint32_t count = g / nmod;             // polynomials in the launch
-  const uint32_t cm = v / count, poly = v % count;
-  const uint32_t res = poly * nmod + cm;
-  const ModDesc d = descs[cm];
-  const size_t off = (size_t)res << LB;
-  const uint32_t t = threadIdx.x;
+__device__ __forceinline__ void polymul_residue(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
+                                                const uint64_t *__restrict__ b, const ModDesc &d,
+                                                uint64_t *lds, uint32_t t) {
   uint64_t x[1 << LE], y[1 << LE];
   if constexpr (LE <= 4) {
     // both operands are requested at once (b waits in y during forward(a)),
     // so the second transform does not stall on HBM latency again
     constexpr int W0 = FirstWindow<LB, LE>::value;
-    gmem_load<W0, LE>(a + off, x, t);
-    gmem_load<W0, LE>(b + off, y, t);
+    gmem_load<W0, LE>(a, x, t);
+    gmem_load<W0, LE>(b, y, t);
     fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd, LB, 0u, d.p, t);
-    gmem_store<0, LE>(c + off, x, t);
+    gmem_store<0, LE>(c, x, t);
 #pragma unroll
     for (int e = 0; e < (1 << LE); ++e) x[e] = y[e];
     const uint32_t t2 = phase_v(t);
     fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
-    gmem_load<0, LE>(c + off, y, t2);
+    gmem_load<0, LE>(c, y, t2);
   } else {
-    fwd_block<LB, LE>(x, a + off, lds, d.fwd, LB, 0u, d.p, t);
-    gmem_store<0, LE>(c + off, x, t);
+    fwd_block<LB, LE>(x, a, lds, d.fwd, LB, 0u, d.p, t);
+    gmem_store<0, LE>(c, x, t);
     const uint32_t t2 = phase_v(t);
-    fwd_block<LB, LE>(x, b + off, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
-    gmem_load<0, LE>(c + off, y, t2);
+    fwd_block<LB, LE>(x, b, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+    gmem_load<0, LE>(c, y, t2);
   }
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);
-  inv_block<LB, LE, TOP_MERGED>(x, c + off, lds, d.inv, LB, 0u, d, t3);
+  inv_block<LB, LE, TOP_MERGED>(x, c, lds, d.inv, LB, 0u, d, t3);
+}
+
+// XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (the hardware
+// deals consecutive workgroups to consecutive XCDs); give each XCD a
+// contiguous range [xcd*g/8, ...) of the g work units.
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g) {
+  const uint32_t xcd = b % 8u, pos = b / 8u;
+  return xcd * (g / 8u) + min(xcd, g % 8u) + pos;
+}
+
+// Residue-major polynomials: residue res of the launch uses descs[res % nmod];
+// the XCD ranges cover the residues in prime-major order, so an XCD's L2
+// holds the twiddles of few primes.
+template <int LB, int LE>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
+          const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod) {
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t g = gridDim.x;
+  const uint32_t v = xcd_order(blockIdx.x, g);
+  const uint32_t count = g / nmod;  // polynomials in the launch
+  const uint32_t cm = v / count, poly = v % count;
+  const uint32_t res = poly * nmod + cm;
+  const ModDesc d = descs[cm];
+  const size_t off = (size_t)res << LB;
+  polymul_residue<LB, LE>(c + off, a + off, b + off, d, lds, threadIdx.x);
+}
+
+// One residue per workgroup from a residue table (a batch plan of one degree
+// class); the table is ordered by prime, and the XCD ranges follow it.
+template <int LB, int LE>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_polymul_tab(const BatchRes *__restrict__ res, uint32_t nres) {
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t v = xcd_order(blockIdx.x, nres);
+  const BatchRes r = res[v];
+  const ModDesc d = *r.d;
+  polymul_residue<LB, LE>(r.c, r.a, r.b, d, lds, threadIdx.x);
 }
 
 namespace {
@@ -60,7 +86,24 @@ hipError_t polymul_lb(uint64_t *c, const uint64_t *a, const uint64_t *b, const M
   return hipGetLastError();
 }
 
+template <int LB, int LE>
+hipError_t polymul_tab_lb(const BatchRes *res, size_t nres, hipStream_t st) {
+  hipLaunchKernelGGL((k_polymul_tab<LB, LE>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, res,
+                     (uint32_t)nres);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_polymul_table(const BatchRes *res, size_t nres, int logn, NttShape shape,
+                                hipStream_t st) {
+  if (logn < kMinLogN || logn > kMaxBlockLog || nres == 0 || nres > 0x7fffffff)
+    return hipErrorInvalidValue;
+  const int le = clamp_le(logn, shape.le);
+#define YA_PMT_CALL(LB, LE) polymul_tab_lb<LB, LE>(res, nres, st)
+  YA_LB_SWITCH(logn, le, YA_PMT_CALL)
+#undef YA_PMT_CALL
+}
 
 hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
                           int logn, size_t nres, size_t nmod, NttShape shape, hipStream_t st) {

@@ -128,9 +128,7 @@ __device__ __forceinline__ void gmem_store(uint64_t *__restrict__ g, const uint6
 // index of word 0 of this thread with the window bits and the bits below the
 // window cleared (i.e. block offset | thi << (WLO+LE)); nlog = log2 N of the
 // whole residue.
-// FIRST: layer HI-1 is the first layer applied to the caller's words (exact
-// reduction compare, see ct_butterfly).
-template <int WLO, int LE, int HI, int LO, bool FIRST = false>
+template <int WLO, int LE, int HI, int LO>
 __device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
                                            uint32_t nlog, uint32_t gbase, uint64_t p) {
   const PrimeK k(p);
@@ -144,16 +142,14 @@ __device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__r
 #pragma unroll
       for (int q = 0; q < (1 << j); ++q) {
         const int e0 = (g << (j + 1)) | q;
-        if (FIRST && b == HI - 1) ct_butterfly<true>(x[e0], x[e0 | (1 << j)], t, k);
-        else ct_butterfly<false>(x[e0], x[e0 | (1 << j)], t, k);
+        ct_butterfly(x[e0], x[e0 | (1 << j)], t, k);
       }
     }
   }
 }
 
 // Inverse (GS) layers b = LO ... HI-1 of one pass (same indexing).
-// FIRST: layer LO is the first layer applied to the caller's words.
-template <int WLO, int LE, int HI, int LO, bool FIRST = false>
+template <int WLO, int LE, int HI, int LO>
 __device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
                                            uint32_t nlog, uint32_t gbase, uint64_t p) {
   const PrimeK k(p);
@@ -167,8 +163,7 @@ __device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__r
 #pragma unroll
       for (int q = 0; q < (1 << j); ++q) {
         const int e0 = (g << (j + 1)) | q;
-        if (FIRST && b == LO) gs_butterfly<true>(x[e0], x[e0 | (1 << j)], t, k);
-        else gs_butterfly<false>(x[e0], x[e0 | (1 << j)], t, k);
+        gs_butterfly(x[e0], x[e0 | (1 << j)], t, k);
       }
     }
   }

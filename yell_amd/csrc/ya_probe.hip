@@ -42,7 +42,7 @@ k_probe(uint64_t *__restrict__ out, uint32_t iters, uint64_t seed, uint64_t p) {
     for (int k = 0; k < kChains * 2; ++k) v[k] %= (p << 2);
     for (uint32_t it = 0; it < iters; ++it) {
 #pragma unroll
-      for (int k = 0; k < kChains; ++k) ct_butterfly<false>(v[k], v[k + kChains], t, pk);
+      for (int k = 0; k < kChains; ++k) ct_butterfly(v[k], v[k + kChains], t, pk);
     }
   }
   uint64_t acc = 0;

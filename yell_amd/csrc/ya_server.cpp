@@ -41,7 +41,7 @@ struct Request {
   uint32_t degree = 0, nmoduli = 0;
   size_t words = 0;   // N * L
   size_t off = 0;     // arena offset (words) of [a | b | c]
-  clk::time_point t_submit, t_done;
+  clk::time_point t_submit, t_dispatch, t_done;
   int state = 0;      // 0 pending, 1 on device, 2 done, 3 failed
 };
 
@@ -119,7 +119,7 @@ struct ya_server {
   int error = YA_OK;
 
   // statistics
-  std::vector<double> lat_us;
+  std::vector<double> lat_us, queue_us, service_us;
   uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_jobs = 0;
   uint64_t max_pending = 0;
 
@@ -233,6 +233,7 @@ void dispatch_loop(ya_server *s) {
       if (!s->arena->alloc(3 * r->words, &off)) break;  // wait for memory to free up
       r->off = off;
       r->state = 1;
+      r->t_dispatch = clk::now();
       nres += r->nmoduli;
       sl.reqs.push_back(r);
       s->pending.pop_front();
@@ -290,6 +291,8 @@ void complete_loop(ya_server *s) {
         if (code >= 0) {
           s->completed++;
           s->lat_us.push_back(std::chrono::duration<double, std::micro>(r->t_done - r->t_submit).count());
+          s->queue_us.push_back(std::chrono::duration<double, std::micro>(r->t_dispatch - r->t_submit).count());
+          s->service_us.push_back(std::chrono::duration<double, std::micro>(r->t_done - r->t_dispatch).count());
         }
       }
       sl.reqs.clear();
@@ -460,6 +463,10 @@ int ya_server_get_stats(ya_server *s, ya_server_stats *out) {
   double sum = 0;
   for (double v : s->lat_us) sum += v;
   out->mean_us = s->lat_us.empty() ? 0 : sum / (double)s->lat_us.size();
+  out->p50_queue_us = percentile(s->queue_us, 0.50);
+  out->p99_queue_us = percentile(s->queue_us, 0.99);
+  out->p50_service_us = percentile(s->service_us, 0.50);
+  out->p99_service_us = percentile(s->service_us, 0.99);
   return YA_OK;
 }
 
@@ -467,6 +474,8 @@ int ya_server_reset_stats(ya_server *s) {
   if (!s) return fail(YA_ERR_INVALID, "null server");
   std::lock_guard<std::mutex> g(s->mu);
   s->lat_us.clear();
+  s->queue_us.clear();
+  s->service_us.clear();
   s->submitted = s->completed = s->rejected = s->batches = s->batched_jobs = s->max_pending = 0;
   return YA_OK;
 }
@@ -626,6 +635,10 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   rep->mean_batch_jobs = st.mean_batch_jobs;
   rep->max_pending = st.max_pending;
   rep->client_stalls = backpressure;
+  rep->p50_queue_us = st.p50_queue_us;
+  rep->p99_queue_us = st.p99_queue_us;
+  rep->p50_service_us = st.p50_service_us;
+  rep->p99_service_us = st.p99_service_us;
   (void)hipHostFree(results);
   cleanup();
   return YA_OK;
