@@ -39,7 +39,7 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
     // pointers and constants in scalar registers
     const uint32_t k = __builtin_amdgcn_readfirstlane(it.first + (active ? sub : 0u));
     r = res[k];
-    d = *r.d;
+    d = load_desc(r.d);
   } else {
     r = res[it.first + (active ? sub : 0u)];
     d = *r.d;
@@ -94,7 +94,7 @@ __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__
                                          uint64_t *lds, uint64_t *__restrict__ scr, uint32_t t) {
   constexpr int LE = 4;
   const BatchRes r = res[it.first];
-  const ModDesc d = *r.d;
+  const ModDesc d = load_desc(r.d);
   uint64_t x[1 << LE], y[1 << LE];
   top15_fwd(r.a, scr, d, t);  // forward(a): bit 14, then each half
   __syncthreads();

@@ -72,7 +72,7 @@ k_polymul_tab(const BatchRes *__restrict__ res, uint32_t nres) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
   const uint32_t v = xcd_order(blockIdx.x, nres);
   const BatchRes r = res[v];
-  const ModDesc d = *r.d;
+  const ModDesc d = load_desc(r.d);
   polymul_residue<LB, LE>(r.c, r.a, r.b, d, lds, threadIdx.x);
 }
 

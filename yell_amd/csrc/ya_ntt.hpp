@@ -102,26 +102,47 @@ __device__ __forceinline__ void lds_load(const uint64_t *s, uint64_t (&x)[1 << L
 }
 
 // Global-memory word accesses of a window.  The pointers are cast to the
-// global address space: residue pointers read from a descriptor are generic,
-// and generic (FLAT) accesses need a 64-bit address register pair per access.
+// global address space (residue pointers read from a descriptor are generic,
+// and generic FLAT accesses need a 64-bit address pair per access), and the
+// byte offset is formed in 32 bits (a block is at most 2^17 words), so every
+// access is a base-register + 32-bit-offset global load/store.
 typedef __attribute__((address_space(1))) uint64_t gword;
+typedef __attribute__((address_space(1))) unsigned char gbyte;
 
 template <int WLO, int LE>
 __device__ __forceinline__ void gmem_load(const uint64_t *__restrict__ g, uint64_t (&x)[1 << LE],
                                           uint32_t t) {
-  const gword *gg = (const gword *)g;
+  const gbyte *gb = (const gbyte *)g;
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) x[e] = gg[base | ((uint32_t)e << WLO)];
+  for (int e = 0; e < (1 << LE); ++e)
+    x[e] = *(const gword *)(gb + ((base | ((uint32_t)e << WLO)) << 3));
 }
 
 template <int WLO, int LE>
 __device__ __forceinline__ void gmem_store(uint64_t *__restrict__ g, const uint64_t (&x)[1 << LE],
                                            uint32_t t) {
-  gword *gg = (gword *)g;
+  gbyte *gb = (gbyte *)g;
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) gg[base | ((uint32_t)e << WLO)] = x[e];
+  for (int e = 0; e < (1 << LE); ++e) *(gword *)(gb + ((base | ((uint32_t)e << WLO)) << 3)) = x[e];
+}
+
+// Descriptor read through the constant address space: when the address is
+// wave-uniform the fields land in scalar registers (s_load).
+__device__ __forceinline__ ModDesc load_desc(const ModDesc *p) {
+  const __attribute__((address_space(4))) u64x2 *q = (const __attribute__((address_space(4))) u64x2 *)p;
+  const u64x2 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+  ModDesc d;
+  d.fwd = (const Tw *)v0.x;
+  d.inv = (const Tw *)v0.y;
+  d.p = v1.x;
+  d.pn = v1.y;
+  d.inv_n = v2.x;
+  d.inv_n_s = v2.y;
+  d.inv_n_phi = v3.x;
+  d.inv_n_phi_s = v3.y;
+  return d;
 }
 
 // Forward (CT) layers b = HI-1 ... LO of one pass.  `gbase` is the residue
