@@ -28,6 +28,11 @@ from yell_amd import capi  # noqa: E402
 
 METRIC = "bit-exact poly-mults/s at N=2^14,L=8 batched vs plain; p50/p99 served latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one poly-multiplication of the batched step moves, from the
+# FETCH_SIZE / WRITE_SIZE counter passes (calibrated against a 256 MiB copy)
+# in profiles/k_polymul_n16384_l8_trace_and_traffic.md: 1.714e9 B read +
+# 1.074e9 B written per launch of 512 poly-mults.
+TRAFFIC_BYTES_PER_POLYMUL = 5.444e6
 
 
 def log(*a):
@@ -336,7 +341,8 @@ def main():
         "grid_form": {"value": B / (ms_grid / 1e3), "ms_per_step": ms_grid,
                       "kernel": "k_polymul<14,4>, one workgroup per residue"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_POLYMUL * B,
+                     "traffic_source": "profiles/k_polymul_n16384_l8_trace_and_traffic.md",
                      "kernel": "k_polymul_tab<14,4>" if form.value == 2 else "k_batch_polymul",
                      "bytes_per_polymul": bytes_per,
                      "polymuls_per_launch": B},
