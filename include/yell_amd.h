@@ -235,6 +235,10 @@ typedef struct ya_server_stats {
   /* split of the latency: queue = submit -> dispatched in a batch,
    * service = dispatched -> result in c (copies + kernel + retirement) */
   double p50_queue_us, p99_queue_us, p50_service_us, p99_service_us;
+  /* per batch: device time of copies-in + kernel, device time of copies-out,
+   * host time issuing the batch, host time dispatch -> retired */
+  double p50_batch_gpu_us, p99_batch_gpu_us, p99_batch_d2h_us, p99_batch_launch_us;
+  double p50_batch_host_us, p99_batch_host_us;
 } ya_server_stats;
 int ya_server_create(ya_server **server, const ya_server_config *config);
 /* YA_ERR_REJECTED when the request can never fit or the queue is full. */
@@ -265,6 +269,8 @@ typedef struct ya_serve_report {
   uint64_t client_stalls;  /* arrivals delayed because every result buffer was in use */
   double wall_s, throughput_per_s, p50_us, p90_us, p99_us, max_us, mean_us, mean_batch_jobs;
   double p50_queue_us, p99_queue_us, p50_service_us, p99_service_us;
+  double p50_batch_gpu_us, p99_batch_gpu_us, p99_batch_d2h_us, p99_batch_launch_us;
+  double p50_batch_host_us, p99_batch_host_us;
 } ya_serve_report;
 int ya_server_run_poisson(ya_server *server, const ya_poisson_spec *spec, ya_serve_report *report);
 

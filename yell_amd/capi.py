@@ -109,7 +109,9 @@ class ServerStats(ctypes.Structure):
                                               "max_pending", "arena_bytes_in_use")] + \
                [(k, ctypes.c_double) for k in ("mean_batch_jobs", "p50_us", "p90_us", "p99_us",
                                               "max_us", "mean_us", "p50_queue_us", "p99_queue_us",
-                                              "p50_service_us", "p99_service_us")]
+                                              "p50_service_us", "p99_service_us",
+                                              "p50_batch_gpu_us", "p99_batch_gpu_us", "p99_batch_d2h_us",
+                                              "p99_batch_launch_us", "p50_batch_host_us", "p99_batch_host_us")]
 
 
 class PoissonSpec(ctypes.Structure):
@@ -126,7 +128,9 @@ class ServeReport(ctypes.Structure):
                [(k, ctypes.c_double) for k in ("wall_s", "throughput_per_s", "p50_us", "p90_us",
                                               "p99_us", "max_us", "mean_us", "mean_batch_jobs",
                                               "p50_queue_us", "p99_queue_us", "p50_service_us",
-                                              "p99_service_us")]
+                                              "p99_service_us",
+                                              "p50_batch_gpu_us", "p99_batch_gpu_us", "p99_batch_d2h_us",
+                                              "p99_batch_launch_us", "p50_batch_host_us", "p99_batch_host_us")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

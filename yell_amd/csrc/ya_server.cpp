@@ -89,6 +89,9 @@ class Arena {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // batch start / kernel end (timing)
+  clk::time_point host_dispatch;          // when the dispatcher took the batch
+  double launch_us = 0;                   // host time spent issuing the batch
   BatchItem *d_items = nullptr, *h_items = nullptr;
   BatchRes *d_res = nullptr, *h_res = nullptr;
   BatchQueue *d_q = nullptr;
@@ -120,6 +123,7 @@ struct ya_server {
 
   // statistics
   std::vector<double> lat_us, queue_us, service_us;
+  std::vector<double> b_gpu_us, b_d2h_us, b_launch_us, b_host_us;  // per batch
   uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_jobs = 0;
   uint64_t max_pending = 0;
 
@@ -151,6 +155,8 @@ int server_free(ya_server *s) {
   for (auto &sl : s->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.t0) (void)hipEventDestroy(sl.t0);
+    if (sl.t1) (void)hipEventDestroy(sl.t1);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     (void)hipFree(sl.d_items);
     (void)hipFree(sl.d_res);
@@ -192,6 +198,7 @@ int launch_slot(ya_server *s, int k) {
     if (e == hipSuccess)
       e = hipMemcpyAsync((void *)jobs[j].b, r.b, r.words * 8, hipMemcpyHostToDevice, sl.stream);
   }
+  if (e == hipSuccess) e = hipEventRecord(sl.t0, sl.stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(sl.d_items, sl.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, sl.stream);
   if (e == hipSuccess)
@@ -200,6 +207,7 @@ int launch_slot(ya_server *s, int k) {
     const unsigned grid = (unsigned)std::min<size_t>(s->cus, items.size());
     e = launch_batch(sl.d_items, segs, sl.d_res, sl.d_q, sl.d_scratch, grid, sl.stream);
   }
+  if (e == hipSuccess) e = hipEventRecord(sl.t1, sl.stream);
   for (size_t j = 0; j < jobs.size() && e == hipSuccess; ++j) {
     const Request &r = *sl.reqs[j];
     e = hipMemcpyAsync(r.c, jobs[j].c, r.words * 8, hipMemcpyDeviceToHost, sl.stream);
@@ -245,9 +253,12 @@ void dispatch_loop(ya_server *s) {
     sl.busy = true;
     s->batches++;
     s->batched_jobs += sl.reqs.size();
+    sl.host_dispatch = clk::now();
     lk.unlock();
     const int rc = launch_slot(s, k);
+    const double lus = std::chrono::duration<double, std::micro>(clk::now() - sl.host_dispatch).count();
     lk.lock();
+    sl.launch_us = lus;
     if (rc != YA_OK) {
       s->error = rc;
       for (auto &r : sl.reqs) {
@@ -284,6 +295,16 @@ void complete_loop(ya_server *s) {
     for (int code : ready) {
       const int k = code >= 0 ? code : -1 - code;
       Slot &sl = s->slots[k];
+      if (code >= 0) {
+        float g = 0, d2h = 0;
+        if (hipEventElapsedTime(&g, sl.t0, sl.t1) == hipSuccess &&
+            hipEventElapsedTime(&d2h, sl.t1, sl.done) == hipSuccess) {
+          s->b_gpu_us.push_back(1e3 * g);
+          s->b_d2h_us.push_back(1e3 * d2h);
+        }
+        s->b_launch_us.push_back(sl.launch_us);
+        s->b_host_us.push_back(std::chrono::duration<double, std::micro>(now - sl.host_dispatch).count());
+      }
       for (auto &r : sl.reqs) {
         r->t_done = now;
         r->state = code >= 0 ? 2 : 3;
@@ -361,7 +382,9 @@ int ya_server_create(ya_server **out, const ya_server_config *cfg_in) {
     sl.cap_items = sl.cap_res;
     const BatchQueue q0{};
     e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&sl.done);
+    if (e == hipSuccess) e = hipEventCreate(&sl.t0);
+    if (e == hipSuccess) e = hipEventCreate(&sl.t1);
     if (e == hipSuccess) e = hipMalloc(&sl.d_items, sl.cap_items * sizeof(BatchItem));
     if (e == hipSuccess) e = hipMalloc(&sl.d_res, sl.cap_res * sizeof(BatchRes));
     if (e == hipSuccess) e = hipMalloc(&sl.d_q, sizeof(BatchQueue));
@@ -467,6 +490,12 @@ int ya_server_get_stats(ya_server *s, ya_server_stats *out) {
   out->p99_queue_us = percentile(s->queue_us, 0.99);
   out->p50_service_us = percentile(s->service_us, 0.50);
   out->p99_service_us = percentile(s->service_us, 0.99);
+  out->p50_batch_gpu_us = percentile(s->b_gpu_us, 0.50);
+  out->p99_batch_gpu_us = percentile(s->b_gpu_us, 0.99);
+  out->p99_batch_d2h_us = percentile(s->b_d2h_us, 0.99);
+  out->p99_batch_launch_us = percentile(s->b_launch_us, 0.99);
+  out->p50_batch_host_us = percentile(s->b_host_us, 0.50);
+  out->p99_batch_host_us = percentile(s->b_host_us, 0.99);
   return YA_OK;
 }
 
@@ -476,6 +505,10 @@ int ya_server_reset_stats(ya_server *s) {
   s->lat_us.clear();
   s->queue_us.clear();
   s->service_us.clear();
+  s->b_gpu_us.clear();
+  s->b_d2h_us.clear();
+  s->b_launch_us.clear();
+  s->b_host_us.clear();
   s->submitted = s->completed = s->rejected = s->batches = s->batched_jobs = s->max_pending = 0;
   return YA_OK;
 }
@@ -639,6 +672,12 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   rep->p99_queue_us = st.p99_queue_us;
   rep->p50_service_us = st.p50_service_us;
   rep->p99_service_us = st.p99_service_us;
+  rep->p50_batch_gpu_us = st.p50_batch_gpu_us;
+  rep->p99_batch_gpu_us = st.p99_batch_gpu_us;
+  rep->p99_batch_d2h_us = st.p99_batch_d2h_us;
+  rep->p99_batch_launch_us = st.p99_batch_launch_us;
+  rep->p50_batch_host_us = st.p50_batch_host_us;
+  rep->p99_batch_host_us = st.p99_batch_host_us;
   (void)hipHostFree(results);
   cleanup();
   return YA_OK;
