@@ -206,6 +206,51 @@ def c4_bench(stream, ev, steps):
             "mac_ms": ms_mac, "mac_GBps": (2 * pairs + 2) * w * 8 / (ms_mac / 1e3) / 1e9}
 
 
+def crt_bench(stream, ev, reps=10):
+    """CRT lift (ya_poly_crt_lift) of one polynomial, device-resident in and out."""
+    out = {}
+    for n, L in ((1 << 14, 8), (1 << 16, 24)):
+        w = ctypes.c_size_t()
+        capi.call("ya_crt_limbs", L, ctypes.byref(w))
+        res = capi.sample_uniform(n, L, 77 + L)
+        din = capi.DeviceBuffer.from_host(res)
+        dout = capi.DeviceBuffer(n * w.value)
+        capi.call("ya_poly_crt_lift", dout.ptr, din.ptr, n, L, stream)
+        capi.call("ya_stream_synchronize", stream)
+        capi.call("ya_event_record", ev.a, stream)
+        for _ in range(reps):
+            capi.call("ya_poly_crt_lift", dout.ptr, din.ptr, n, L, stream)
+        capi.call("ya_event_record", ev.b, stream)
+        ms = ev.elapsed_ms() / reps
+        alg = (n * L + n * w.value) * 8
+        out["n%d_l%d" % (n, L)] = {"us_per_poly": ms * 1e3, "coeffs_per_s": n / (ms / 1e3),
+                                   "limbs": w.value, "algorithmic_bytes": alg,
+                                   "hbm_GBps_algorithmic": alg / (ms / 1e3) / 1e9}
+        if n == 1 << 14:
+            out["_c2_in"], out["_c2_out"] = res, dout.download().reshape(n, w.value)
+        din.close()
+        dout.close()
+    return out
+
+
+def crt_cpu_reference(crt):
+    """The reference's poly2mpz (GMP, one thread) on the same C2 polynomial,
+    and bit-exactness of the GPU lift against it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle  # checker only
+    res, got = crt.pop("_c2_in"), crt.pop("_c2_out")
+    try:
+        ref = _oracle.RefGmpLib()
+    except RuntimeError:
+        return {"available": False}
+    n, L = 1 << 14, 8
+    want = ref.poly2mpz(res, n, L, got.shape[1])
+    sec = ref.bench(res, n, L, 5)
+    return {"available": True, "kind": "reference", "ms_per_poly": sec * 1e3, "cores": 1,
+            "sample": "5 x poly<16384>(8).poly2mpz() of the reference (lib/gmp.cpp, GMP)",
+            "bit_exact": bool(np.array_equal(want, got))}
+
+
 def served_bench(rates, duration, shapes):
     """Poisson stream of host-operand requests (shape drawn from `shapes`):
     continuous batching vs one request per launch, same seeded stream."""
@@ -360,6 +405,8 @@ def main():
         out["c3_uneven_batch"] = c3_bench(stream, ev, max(2, args.steps // 4))
         log("C4 large-ring MAC")
         out["c4_large_ring_mac"] = c4_bench(stream, ev, 3)
+        log("CRT lift")
+        out["crt_lift"] = crt_bench(stream, ev)
         sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
         out["served"] = sv
         for r in sv["runs"]:
@@ -374,6 +421,12 @@ def main():
         out["cpu_baseline"] = cb
         out["bit_exact_vs_cpu"] = exact and same
         out["gpu_over_cpu"] = value / cb["value"]
+        if "crt_lift" in out:
+            log("CRT lift cpu reference")
+            out["crt_lift"]["cpu_reference"] = crt_cpu_reference(out["crt_lift"])
+    if "crt_lift" in out:
+        out["crt_lift"].pop("_c2_in", None)
+        out["crt_lift"].pop("_c2_out", None)
     capi.load().ya_batch_plan_destroy(plan)
     print(json.dumps(out), flush=True)
 

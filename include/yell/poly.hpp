@@ -11,10 +11,14 @@
 //     to_host() / from_host() move whole polynomials;
 //   * samplers are seeded (yell::set_seed); gaussian takes sigma directly
 //     (the reference wraps an MPFR-built FastGaussianNoise table);
-//   * poly2mpz (GMP CRT lift) is not provided;
+//   * poly2mpz (the CRT lift, include/yell/gmp.hpp:41-70) runs on the GPU;
+//     it is declared when <gmp.h> is on the include path (link -lgmp), and
+//     to_limbs() gives the same integers as little-endian 64-bit limbs
+//     without GMP;
 //   * failures throw std::runtime_error (the reference asserts).
 // Every arithmetic result is bit-identical to the reference's.
 #pragma once
+#include <array>
 #include <atomic>
 #include <cstring>
 #include <stdexcept>
@@ -24,6 +28,10 @@
 #include "yell/ops.hpp"
 #include "yell/params.hpp"
 #include "yell_amd.h"
+#if __has_include(<gmp.h>)
+#include <gmp.h>
+#define YELL_AMD_HAVE_GMP 1
+#endif
 
 namespace yell {
 
@@ -143,6 +151,38 @@ class poly {
     if (h.size() != degree * nmoduli_) throw std::runtime_error("from_host: size mismatch");
     if (!h.empty()) ya_check(ya_memcpy_htod(d_, h.data(), h.size() * sizeof(value_type), nullptr), "from_host");
   }
+
+  // ---- CRT lift (details/poly.hpp:162-165): coefficient i of the result is
+  // words [i * lift_limbs(), (i + 1) * lift_limbs()), little-endian.
+  size_t lift_limbs() const {
+    size_t w = 0;
+    ya_check(ya_crt_limbs(nmoduli_, &w), "ya_crt_limbs");
+    return w;
+  }
+  std::vector<uint64_t> to_limbs() const {
+    const size_t w = lift_limbs();
+    std::vector<uint64_t> h(degree * w);
+    uint64_t *d = nullptr;
+    ya_check(ya_malloc((void **)&d, h.size() * 8), "ya_malloc");
+    int rc = ya_poly_crt_lift(d, d_, degree, nmoduli_, nullptr);
+    if (rc == YA_OK) rc = ya_memcpy_dtoh(h.data(), d, h.size() * 8, nullptr);
+    ya_free(d);
+    ya_check(rc, "ya_poly_crt_lift");
+    return h;
+  }
+#ifdef YELL_AMD_HAVE_GMP
+  // The caller owns the returned integers (mpz_clear each), as in the reference.
+  std::array<mpz_t, degree_> poly2mpz() const {
+    const size_t w = lift_limbs();
+    const std::vector<uint64_t> h = to_limbs();
+    std::array<mpz_t, degree_> rop;
+    for (size_t i = 0; i < degree_; ++i) {
+      mpz_init2(rop[i], 64 * w);
+      mpz_import(rop[i], w, -1, sizeof(uint64_t), 0, 0, h.data() + i * w);
+    }
+    return rop;
+  }
+#endif
 
   bool operator==(const poly &o) const { return nmoduli_ == o.nmoduli_ && to_host() == o.to_host(); }
   bool operator!=(const poly &o) const { return !(*this == o); }

@@ -289,6 +289,28 @@ int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32_t rho, uin
  * coefficients (uniform positions), each +1 or p-1, shared by all residues. */
 int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed);
 
+/* ---- CRT lift: residues -> multi-limb integers ----------------------------
+ * Replaces poly<N>::poly2mpz (include/yell/details/poly.hpp:162-165), i.e.
+ * gmp::poly2mpz (include/yell/gmp.hpp:41-70) with the constants of
+ * gmp::gmp_precomputed (lib/gmp.cpp:6-51), for the primes P[0..nmoduli).
+ * Each coefficient i becomes S mod Q, Q = P[0]*...*P[nmoduli-1], where
+ * S = sum_j lift_j * r_j and lift_j = (Q/P[j]) * ((Q/P[j])^-1 mod P[j]) -- the
+ * value GMP returns there for any 64-bit residues r_j (canonical or not).
+ *
+ * ya_crt_limbs: 64-bit limbs of a lifted coefficient, ceil(bits(Q) / 64).
+ * ya_poly_crt_lift: residues (device, [nmoduli][degree]) -> out (device,
+ *   [degree][limbs], little-endian limbs per coefficient: the layout
+ *   mpz_import(rop, limbs, -1, 8, 0, 0, out + i * limbs) reads); any degree
+ *   in [1, 2^20]; stream-ordered.  out must not overlap residues.
+ * ya_crt_constants: the host copies of moduli_product_ (limbs limbs),
+ *   modulus_shoup (2 limbs), lifting_integers ([nmoduli][limbs]) and
+ *   shift_modulus_shoup; any output pointer may be NULL. */
+int ya_crt_limbs(size_t nmoduli, size_t *limbs);
+int ya_poly_crt_lift(uint64_t *out, const uint64_t *residues, size_t degree, size_t nmoduli,
+                     void *stream);
+int ya_crt_constants(size_t nmoduli, size_t limbs, uint64_t *moduli_product,
+                     uint64_t *modulus_shoup, uint64_t *lifting, size_t *shift);
+
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Device throughput of the VALU work this
  * engine is bound by, all operands in registers: kind 0 = v_mad_u64_u32

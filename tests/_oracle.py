@@ -10,6 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
 ORACLE_GMP = os.path.join(ROOT, "oracle", "lib", "liboracle_gmp.so")
 REF = os.path.join(ROOT, "oracle", "_ref", "libyellref.so")
+REF_GMP = os.path.join(ROOT, "oracle", "_ref", "libyellref_gmp.so")
 
 u64, sz, vp, i32 = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int
 
@@ -194,6 +195,48 @@ class RefLib:
         b = np.ascontiguousarray(b, dtype=np.uint64)
         assert self.lib.yref_polymul(n.bit_length() - 1, _p(out), _p(a), _p(b), L) == 0
         return out
+
+
+class RefGmpLib:
+    """The reference's CRT lift (lib/gmp.cpp, gmp.hpp poly2mpz) compiled in oracle/_ref."""
+
+    def __init__(self):
+        if not os.path.exists(REF_GMP):
+            raise RuntimeError("oracle/_ref/libyellref_gmp.so not built")
+        self.lib = _bind(ctypes.CDLL(REF_GMP), {
+            "yrefg_poly2mpz": (i32, [vp, i32, sz, vp, sz]),
+            "yrefg_constants": (i32, [sz, sz, vp, vp, vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+            "yrefg_bench_poly2mpz": (ctypes.c_double, [vp, i32, sz, i32]),
+        })
+
+    def poly2mpz(self, res, n, L, limbs):
+        out = np.empty((n, limbs), dtype=np.uint64)
+        res = np.ascontiguousarray(res, dtype=np.uint64)
+        rc = self.lib.yrefg_poly2mpz(_p(res), n.bit_length() - 1, L, _p(out), limbs)
+        assert rc == 0, rc
+        return out
+
+    def constants(self, L, limbs):
+        q = np.empty(limbs, dtype=np.uint64)
+        m = np.empty(2, dtype=np.uint64)
+        lift = np.empty((L, limbs), dtype=np.uint64)
+        shift, bits = sz(), sz()
+        assert self.lib.yrefg_constants(L, limbs, _p(q), _p(m), _p(lift), ctypes.byref(shift),
+                                        ctypes.byref(bits)) == 0
+        return q, m, lift, shift.value, bits.value
+
+    def bench(self, res, n, L, reps):
+        res = np.ascontiguousarray(res, dtype=np.uint64)
+        return self.lib.yrefg_bench_poly2mpz(_p(res), n.bit_length() - 1, L, reps)
+
+
+def crt_oracle():
+    """oracle/crt_oracle.py (Python-integer restatement of the CRT lift)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("crt_oracle", os.path.join(ROOT, "oracle", "crt_oracle.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
 
 
 def rand_residues(rng, n, L, primes, bound_mult=1):

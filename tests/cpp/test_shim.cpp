@@ -109,6 +109,33 @@ static void test_rlwe_example() {
   }
   CHECK(worst < (1ll << 20), "decryption noise %lld too large", worst);
   std::printf("rlwe: max |noise| = %lld\n", worst);
+
+  // example.cpp:67: the lifted plaintext; each integer reduces to its residues.
+  const size_t w = plain.lift_limbs();
+  const std::vector<uint64_t> limbs = plain.to_limbs();
+  CHECK(limbs.size() == Deg * w, "to_limbs size");
+  int bad = 0;
+  for (size_t i = 0; i < Deg; i += 97) {
+    for (size_t cm = 0; cm < NdM; ++cm) {
+      const T p = yell::params::P[cm];
+      unsigned __int128 r = 0;  // Horner from the top limb
+      for (size_t k = w; k-- > 0;) r = ((r << 64) | limbs[i * w + k]) % p;
+      if ((T)r != hp[cm * Deg + i]) ++bad;
+    }
+  }
+  CHECK(bad == 0, "%d lifted coefficients do not reduce to their residues", bad);
+#ifdef YELL_AMD_HAVE_GMP
+  auto coeffs = plain.poly2mpz();
+  for (size_t i = 0; i < Deg; ++i) {
+    if (i % 89 == 0) {
+      for (size_t cm = 0; cm < NdM; ++cm)
+        if (mpz_fdiv_ui(coeffs[i], yell::params::P[cm]) != hp[cm * Deg + i]) ++bad;
+    }
+    mpz_clear(coeffs[i]);
+  }
+  CHECK(bad == 0, "poly2mpz: %d coefficients do not reduce to their residues", bad);
+  std::printf("rlwe: poly2mpz checked (GMP)\n");
+#endif
 }
 
 int main() {

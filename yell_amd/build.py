@@ -80,6 +80,10 @@ def build_lib(changed_only=False, jobs=None):
     return LIB
 
 
+GMP_INC = "/opt/conda/include"
+SYS_GMP = "/usr/lib/x86_64-linux-gnu/libgmp.so.10"
+
+
 def build_cpp_tests():
     """tests/cpp/*.cpp -> tests/cpp/build/<name>, linked against libyell_amd.so."""
     src_dir = os.path.join(ROOT, "tests", "cpp")
@@ -90,6 +94,10 @@ def build_cpp_tests():
         cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
                "-o", exe, "-L", os.path.dirname(LIB), "-lyell_amd",
                "-Wl,-rpath,$ORIGIN/../../../yell_amd/lib"]
+        if os.path.exists(os.path.join(GMP_INC, "gmp.h")) and os.path.exists(SYS_GMP):
+            # poly2mpz of the C++ drop-in: GMP headers here, the system library by path
+            cmd[6:6] = ["-I", GMP_INC]
+            cmd.append(SYS_GMP)
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("C++ test build failed:\n" + r.stdout + r.stderr)

@@ -76,6 +76,9 @@ _SIGS = {
     "ya_sample_hwt": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
     "ya_pointer_is_device": (i32, [vp, ctypes.POINTER(i32)]),
     "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
+    "ya_crt_limbs": (i32, [sz, ctypes.POINTER(ctypes.c_size_t)]),
+    "ya_poly_crt_lift": (i32, [vp, vp, sz, sz, vp]),
+    "ya_crt_constants": (i32, [sz, sz, vp, vp, vp, ctypes.POINTER(ctypes.c_size_t)]),
     "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),
     "ya_batch_plan_run": (i32, [vp, vp]),
     "ya_batch_plan_run_graph": (i32, [vp, vp]),
