@@ -206,6 +206,30 @@ def c4_bench(stream, ev, steps):
             "mac_ms": ms_mac, "mac_GBps": (2 * pairs + 2) * w * 8 / (ms_mac / 1e3) / 1e9}
 
 
+def latency_bench(stream, ev, n, L, reps=20):
+    """Device time of ONE poly-multiplication (a single C2 request) as a split
+    launch (each residue over many workgroups) and as the one-workgroup-per-
+    residue fused kernel; median over reps."""
+    a = capi.DeviceBuffer.from_host(capi.sample_uniform(n, L, 31))
+    b = capi.DeviceBuffer.from_host(capi.sample_gaussian(n, L, 3.2, 32))
+    c = capi.DeviceBuffer(n * L)
+    out = {}
+    for name, mode in (("split_auto", 0), ("fused", -1)):
+        capi.call("ya_split_policy", mode)
+        ts = []
+        for _ in range(reps + 2):
+            capi.call("ya_stream_synchronize", stream)
+            capi.call("ya_event_record", ev.a, stream)
+            capi.call("ya_poly_multiply", c.ptr, a.ptr, b.ptr, n, L, stream)
+            capi.call("ya_event_record", ev.b, stream)
+            ts.append(ev.elapsed_ms() * 1e3)
+        out[name + "_us"] = float(np.median(ts[2:]))
+    capi.call("ya_split_policy", 0)
+    for d in (a, b, c):
+        d.close()
+    return out
+
+
 def crt_bench(stream, ev, reps=10):
     """CRT lift (ya_poly_crt_lift) of one polynomial, device-resident in and out."""
     out = {}
@@ -407,6 +431,8 @@ def main():
         out["c4_large_ring_mac"] = c4_bench(stream, ev, 3)
         log("CRT lift")
         out["crt_lift"] = crt_bench(stream, ev)
+        log("single poly-mult latency")
+        out["single_polymul_latency"] = latency_bench(stream, ev, n, L)
         sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
         out["served"] = sv
         for r in sv["runs"]:

@@ -178,19 +178,22 @@ typedef struct ya_polymul_job {
 } ya_polymul_job;
 typedef struct ya_batch_plan ya_batch_plan;
 int ya_batch_plan_create(ya_batch_plan **plan, const ya_polymul_job *jobs, size_t njobs);
-/* Launches the persistent kernel on `stream`. */
+/* Launches the plan's kernels on `stream`. */
 int ya_batch_plan_run(ya_batch_plan *plan, void *stream);
 /* Same work as a hipGraph: captured on first use, then replayed on `stream`. */
 int ya_batch_plan_run_graph(ya_batch_plan *plan, void *stream);
 /* Launch forms: a batch of one degree class <= 2^14 holding at least two
  * residues per CU runs as a grid of the fused kernel over the plan's residue
- * table (YA_BATCH_GRID); any other batch drains the persistent kernel's
- * queue (YA_BATCH_PERSISTENT).  Both give the same words. */
-enum { YA_BATCH_PERSISTENT = 1, YA_BATCH_GRID = 2 };
+ * table (YA_BATCH_GRID); a batch of one class 2^11 .. 2^14 too small to
+ * occupy a quarter of the CUs spreads each residue over several workgroups
+ * (YA_BATCH_SPLIT, see ya_split_policy; the plan holds a scratch copy of the
+ * b operands' partial transforms); any other batch drains the persistent
+ * kernel's queue (YA_BATCH_PERSISTENT).  All give the same words. */
+enum { YA_BATCH_PERSISTENT = 1, YA_BATCH_GRID = 2, YA_BATCH_SPLIT = 3 };
 int ya_batch_plan_info(const ya_batch_plan *plan, size_t *items, size_t *residues,
                        size_t *workgroups, int *form);
 /* Forces the launch form of plans created afterwards (0 = automatic; GRID
- * only where the batch allows it). */
+ * and SPLIT only where the batch allows them). */
 int ya_batch_force_form(int form);
 int ya_batch_plan_destroy(ya_batch_plan *plan);
 /* One-shot: create, run on `stream`, wait, destroy. */
@@ -210,6 +213,14 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
  * radix (3, 4, or 5 for kinds 0 and 1).  ya_tune_get returns le, or -1. */
 int ya_tune_get(int kind, int logn);
 int ya_tune_set(int kind, int logn, int le);
+/* Split launches (no counterpart in the reference).  A poly-multiplication
+ * launch of few residues of degree 2^11 .. 2^14 (ya_poly_multiply(_many), the
+ * served form's small batches) may spread each residue over 2^k workgroups
+ * (column kernels for the top k index bits, block kernels for the rest)
+ * instead of one, for latency.  mode 0: automatic (when one workgroup per
+ * residue would occupy under a quarter of the CUs); -1: never; k in 2..5:
+ * always, with that k, where 2^logn / 2^k >= 2^9.  Results are identical. */
+int ya_split_policy(int mode);
 
 /* ---- served form (request loop with continuous batching) --------------- *
  * No counterpart in the reference.  Clients submit poly-multiplication

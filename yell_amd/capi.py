@@ -77,6 +77,7 @@ _SIGS = {
     "ya_pointer_is_device": (i32, [vp, ctypes.POINTER(i32)]),
     "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
     "ya_crt_limbs": (i32, [sz, ctypes.POINTER(ctypes.c_size_t)]),
+    "ya_split_policy": (i32, [i32]),
     "ya_poly_crt_lift": (i32, [vp, vp, sz, sz, vp]),
     "ya_crt_constants": (i32, [sz, sz, vp, vp, vp, ctypes.POINTER(ctypes.c_size_t)]),
     "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),

@@ -200,7 +200,9 @@ struct ya_batch_plan {
   hipGraphExec_t exec = nullptr;
   double total_polymuls = 0;
   int form = YA_BATCH_PERSISTENT;  // launch form chosen at creation
-  int grid_logn = 0;               // degree class of the grid form
+  int grid_logn = 0;               // degree class of the grid and split forms
+  int split_k = 0;                 // split form: 2^k workgroups per residue
+  uint64_t *d_split = nullptr;     // split form: scratch [nres][2^grid_logn]
 };
 
 namespace {
@@ -215,6 +217,7 @@ int plan_free(ya_batch_plan *pl) {
   (void)hipFree(pl->d_res);
   (void)hipFree(pl->d_q);
   (void)hipFree(pl->d_scratch);
+  (void)hipFree(pl->d_split);
   delete pl;
   return YA_OK;
 }
@@ -355,8 +358,20 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
   const int lg = (int)items[0].logn;
   const bool grid_ok = one_class && lg <= 14;
   int form = (grid_ok && res.size() >= 2 * (size_t)cus) ? YA_BATCH_GRID : YA_BATCH_PERSISTENT;
+  int k = one_class ? split_factor(st, lg, res.size()) : 0;
+  if (k) form = YA_BATCH_SPLIT;
   const int ov = g_form_override.load();
-  if (ov == YA_BATCH_PERSISTENT || (ov == YA_BATCH_GRID && grid_ok)) form = ov;
+  if (ov == YA_BATCH_SPLIT && one_class && !k && lg >= kSplitMinLog && lg <= kSplitMaxLog)
+    k = split_choose_k(lg, res.size(), cus);
+  if (ov == YA_BATCH_PERSISTENT || (ov == YA_BATCH_GRID && grid_ok) || (ov == YA_BATCH_SPLIT && k)) form = ov;
+  if (form == YA_BATCH_SPLIT) {
+    e = hipMalloc(&pl->d_split, (res.size() << lg) * 8);
+    if (e != hipSuccess) {
+      plan_free(pl);
+      return fail(YA_ERR_ALLOC, "split scratch: %s", hipGetErrorString(e));
+    }
+    pl->split_k = k;
+  }
   pl->form = form;
   pl->grid_logn = lg;
   *out = pl;
@@ -364,6 +379,10 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
 }
 
 static hipError_t plan_launch(ya_batch_plan *pl, hipStream_t s) {
+  if (pl->form == YA_BATCH_SPLIT) {
+    const SplitArgs sa{pl->d_res, nullptr, nullptr, nullptr, nullptr, 0, pl->d_split};
+    return launch_polymul_split(sa, pl->grid_logn, pl->nres, pl->split_k, s);
+  }
   if (pl->form == YA_BATCH_GRID)
     return launch_polymul_table(pl->d_res, pl->nres, pl->grid_logn, tuned_shape(pl->grid_logn, 1), s);
   return launch_batch(pl->d_items, pl->segs, pl->d_res, pl->d_q, pl->d_scratch, pl->grid, s);
@@ -404,13 +423,16 @@ int ya_batch_plan_info(const ya_batch_plan *pl, size_t *items, size_t *residues,
   if (!pl) return fail(YA_ERR_INVALID, "null plan");
   if (items) *items = pl->nitems;
   if (residues) *residues = pl->nres;
-  if (workgroups) *workgroups = pl->form == YA_BATCH_GRID ? pl->nres : pl->grid;
+  if (workgroups)
+    *workgroups = pl->form == YA_BATCH_GRID    ? pl->nres
+                  : pl->form == YA_BATCH_SPLIT ? pl->nres << pl->split_k
+                                               : pl->grid;
   if (form) *form = pl->form;
   return YA_OK;
 }
 
 int ya_batch_force_form(int form) {
-  if (form != 0 && form != YA_BATCH_PERSISTENT && form != YA_BATCH_GRID)
+  if (form != 0 && form != YA_BATCH_PERSISTENT && form != YA_BATCH_GRID && form != YA_BATCH_SPLIT)
     return fail(YA_ERR_INVALID, "bad launch form %d", form);
   g_form_override.store(form);
   return YA_OK;

@@ -59,6 +59,29 @@ hipError_t launch_batch(const BatchItem *items, const BatchSegs &segs, const Bat
 
 // Grid launch of the fused poly-multiplication over a residue table whose
 // residues are all of degree 2^logn (<= 2^14), one workgroup per residue.
+// Poly-multiplication of few residues, each split over 2^k workgroups
+// (ya_kernels_split.hip), for degrees 2^kSplitMinLog .. 2^kSplitMaxLog.
+// Residues come from a table (tab != nullptr, one degree class) or are
+// contiguous (residue v at word v << logn of a, b and c, using
+// descs[v % nmod]).  The top layers of forward(b) go to scratch t (residue v
+// at word v << logn) or, when t is null, in place into b.
+struct SplitArgs {
+  const BatchRes *tab;
+  const uint64_t *a;
+  const uint64_t *b;
+  uint64_t *c;
+  const ModDesc *descs;
+  uint32_t nmod;
+  uint64_t *t;
+};
+constexpr int kSplitMinLog = 11, kSplitMaxLog = 14;
+constexpr int kSplitMinK = 2, kSplitMaxK = 5, kSplitMinS = 9;
+// Split factor for nres residues of 2^logn words on `cus` CUs: the smallest
+// k giving at least one block per CU (within the instantiated shapes); 0 if
+// the degree has no split shape.
+int split_choose_k(int logn, size_t nres, int cus);
+hipError_t launch_polymul_split(const SplitArgs &s, int logn, size_t nres, int k, hipStream_t st);
+
 hipError_t launch_polymul_table(const BatchRes *res, size_t nres, int logn, NttShape shape,
                                 hipStream_t st);
 

@@ -17,6 +17,7 @@
 #include "../../include/yell_amd.h"
 #include "params_table.inc"
 #include "ya_internal.hpp"
+#include "ya_batch.hpp"
 #include "ya_kernels.hpp"
 
 namespace ya {
@@ -120,6 +121,7 @@ struct Slot {
 
 struct DeviceState {
   int device = 0;
+  int cus = 256;
   hipStream_t tstream = nullptr;
   ModDesc *d_desc = nullptr;  // [(kMaxLogN + 1) * YA_NUM_MODULI]
   std::mutex mu;
@@ -130,6 +132,21 @@ static std::mutex g_dev_mu;
 static std::vector<std::unique_ptr<DeviceState>> g_devs;
 
 static inline size_t slot_index(int logn, size_t cm) { return (size_t)logn * YA_NUM_MODULI + cm; }
+
+int device_cus(DeviceState *st) { return st->cus; }
+
+// 0: automatic, -1: never split, k >= 2: split every eligible launch by 2^k.
+static std::atomic<int> g_split_mode{0};
+
+int split_factor(DeviceState *st, int logn, size_t nres) {
+  const int mode = g_split_mode.load(std::memory_order_relaxed);
+  if (mode < 0 || logn < kSplitMinLog || logn > kSplitMaxLog) return 0;
+  if (mode >= kSplitMinK)
+    return (mode <= kSplitMaxK && logn - mode >= kSplitMinS) ? mode : 0;
+  // automatic: when one workgroup per residue would leave most CUs idle
+  if (nres * 4 > (size_t)st->cus) return 0;
+  return split_choose_k(logn, nres, st->cus);
+}
 
 int current_device(DeviceState **out) {
   int dev = t_device;
@@ -153,6 +170,9 @@ int current_device(DeviceState **out) {
     e = hipMemset(st->d_desc, 0, nslots * sizeof(ModDesc));
     if (e != hipSuccess) return hip_fail(e, "hipMemset(desc)");
     st->slots.reset(new Slot[nslots]);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      st->cus = cus;
     g_devs[dev] = std::move(st);
   }
   *out = g_devs[dev].get();
@@ -646,7 +666,20 @@ static int polymul_impl(uint64_t *c, const uint64_t *a, const uint64_t *b, size_
   const ModDesc *d = desc_ptr(st, logn, 0);
   hipStream_t s = (hipStream_t)stream;
   if (logn <= kMaxBlockLog) {
-    hipError_t e = launch_polymul(c, a, b, d, logn, nmoduli * count, nmoduli, tuned_shape(logn, 1), s);
+    const size_t nres = nmoduli * count;
+    if (const int k = split_factor(st, logn, nres)) {
+      // few residues: spread each over 2^k workgroups; forward(b)'s top
+      // layers go to stream-ordered scratch (a and b are left untouched)
+      uint64_t *tmp = nullptr;
+      hipError_t e = hipMallocAsync((void **)&tmp, (nres << logn) * 8, s);
+      if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMallocAsync: %s", hipGetErrorString(e));
+      const SplitArgs sa{nullptr, a, b, c, d, (uint32_t)nmoduli, tmp};
+      e = launch_polymul_split(sa, logn, nres, k, s);
+      const hipError_t ef = hipFreeAsync(tmp, s);
+      if (e == hipSuccess) e = ef;
+      return e == hipSuccess ? YA_OK : hip_fail(e, "split polymul launch");
+    }
+    hipError_t e = launch_polymul(c, a, b, d, logn, nres, nmoduli, tuned_shape(logn, 1), s);
     return e == hipSuccess ? YA_OK : hip_fail(e, "polymul launch");
   }
   // Large rings: forward(a) into c, forward(b) into stream-ordered scratch,
@@ -672,6 +705,12 @@ static int polymul_impl(uint64_t *c, const uint64_t *a, const uint64_t *b, size_
   hipError_t ef = hipFreeAsync(tmp, s);
   if (e == hipSuccess) e = ef;
   return e == hipSuccess ? YA_OK : hip_fail(e, "polymul (large ring)");
+}
+
+int ya_split_policy(int mode) {
+  if (mode < -1 || mode == 1 || mode > kSplitMaxK) return fail(YA_ERR_INVALID, "split policy %d", mode);
+  g_split_mode.store(mode, std::memory_order_relaxed);
+  return YA_OK;
 }
 
 int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,

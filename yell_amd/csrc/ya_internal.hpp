@@ -24,6 +24,12 @@ const ModDesc *desc_ptr(DeviceState *st, int logn, size_t cm);
 // Base of the device descriptor array: entry (logn, cm) is at logn * 1000 + cm.
 const ModDesc *desc_table_base(DeviceState *st);
 
+// Compute units of the device (256 on MI355X).
+int device_cus(DeviceState *st);
+// Split factor for a poly-multiplication launch of nres residues of 2^logn
+// words under the split policy (ya_split_policy); 0 = one workgroup per residue.
+int split_factor(DeviceState *st, int logn, size_t nres);
+
 int log2_degree(size_t degree);
 int check_shape(size_t degree, size_t cm0, size_t nmoduli, int *logn);
 uint64_t prime(size_t cm);

@@ -204,8 +204,18 @@ int launch_slot(ya_server *s, int k) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(sl.d_res, sl.h_res, res.size() * sizeof(BatchRes), hipMemcpyHostToDevice, sl.stream);
   if (e == hipSuccess) {
-    const unsigned grid = (unsigned)std::min<size_t>(s->cus, items.size());
-    e = launch_batch(sl.d_items, segs, sl.d_res, sl.d_q, sl.d_scratch, grid, sl.stream);
+    bool one_class = true;
+    for (const auto &it : items) one_class &= it.logn == items[0].logn;
+    const int k = one_class ? split_factor(s->st, (int)items[0].logn, res.size()) : 0;
+    if (k) {
+      // few residues: spread each over 2^k workgroups; forward(b)'s top layers
+      // go in place into the arena copy of b, which the server owns
+      const SplitArgs sa{sl.d_res, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
+      e = launch_polymul_split(sa, (int)items[0].logn, res.size(), k, sl.stream);
+    } else {
+      const unsigned grid = (unsigned)std::min<size_t>(s->cus, items.size());
+      e = launch_batch(sl.d_items, segs, sl.d_res, sl.d_q, sl.d_scratch, grid, sl.stream);
+    }
   }
   if (e == hipSuccess) e = hipEventRecord(sl.t1, sl.stream);
   for (size_t j = 0; j < jobs.size() && e == hipSuccess; ++j) {
