@@ -206,6 +206,60 @@ def c4_bench(stream, ev, steps):
             "mac_ms": ms_mac, "mac_GBps": (2 * pairs + 2) * w * 8 / (ms_mac / 1e3) / 1e9}
 
 
+def u32_bench(stream, ev, reps=5):
+    """32-bit moduli build (ya32_poly_multiply_many): N = 2^14 with 16 primes
+    of 30 bits (a ~480-bit modulus, close to the 8 x 62-bit C2 modulus), 256
+    polys per launch (4096 residues, as the C2 step)."""
+    n, L, count = 1 << 14, 16, 256
+    w = n * L
+    rng = np.random.default_rng(5)
+    primes = [int(capi.load().ya32_prime(j)) for j in range(L)]
+    a = np.concatenate([rng.integers(0, primes[j], size=n, dtype=np.uint32) for j in range(L)] * count)
+    b = np.concatenate([rng.integers(0, primes[j], size=n, dtype=np.uint32) for j in range(L)] * count)
+    da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
+    dc = capi.DeviceBuffer(w * count, np.uint32)
+    args = (dc.ptr, da.ptr, db.ptr, n, L, count, stream)
+    capi.call("ya32_poly_multiply_many", *args)
+    capi.call("ya_stream_synchronize", stream)
+    capi.call("ya_event_record", ev.a, stream)
+    for _ in range(reps):
+        capi.call("ya32_poly_multiply_many", *args)
+    capi.call("ya_event_record", ev.b, stream)
+    ms = ev.elapsed_ms() / reps
+    c0 = dc.download(w)
+    mm = modmuls_per_polymul(n, L)
+    bits = sum(p.bit_length() for p in primes)
+    for d in (da, db, dc):
+        d.close()
+    return {"degree": n, "nmoduli": L, "modulus_bits": bits, "polys_per_launch": count,
+            "ms_per_launch": ms, "value": count / (ms / 1e3), "unit": "poly-mults/s",
+            "modmuls_T_per_s": mm * count / (ms / 1e3) / 1e12,
+            "algorithmic_GBps": 3 * w * 4 * count / (ms / 1e3) / 1e9,
+            "_a0": a[:w], "_b0": b[:w], "_c0": c0}
+
+
+def u32_cpu_reference(u32, seconds=3.0):
+    """The reference's 32-bit build (oracle/_ref/libyellref32.so) on the same
+    first poly: one thread, bounded sample, and bit-exactness of the GPU."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle  # checker only
+    a0, b0, c0 = u32.pop("_a0"), u32.pop("_b0"), u32.pop("_c0")
+    n, L = u32["degree"], u32["nmoduli"]
+    try:
+        ref = _oracle.RefLib(32)
+    except RuntimeError:
+        orc = _oracle.Oracle(32)
+        return {"kind": "port", "bit_exact": bool(np.array_equal(orc.polymul(a0, b0, n, L), c0))}
+    want = ref.polymul(a0, b0, n, L)
+    lg = n.bit_length() - 1
+    t1 = ref.lib.yref_bench_polymul(lg, capi.ptr(a0), capi.ptr(b0), L, 1, 1)
+    reps = max(1, int(seconds / max(t1, 1e-6)))
+    s1 = ref.lib.yref_bench_polymul(lg, capi.ptr(a0), capi.ptr(b0), L, 1, reps)
+    return {"kind": "reference", "cores": 1, "value": reps / s1, "unit": "poly-mults/s",
+            "sample": "%d poly-mults (N=%d, L=%d) of the reference's 32-bit build, one thread" % (reps, n, L),
+            "bit_exact": bool(np.array_equal(want, c0))}
+
+
 def latency_bench(stream, ev, n, L, reps=20):
     """Device time of ONE poly-multiplication (a single C2 request) as a split
     launch (each residue over many workgroups) and as the one-workgroup-per-
@@ -286,6 +340,11 @@ def served_bench(rates, duration, shapes):
         cfg = capi.ServerConfig(hbm_budget_bytes=2 << 30, max_pending=1024, max_moduli=16, **cfgkw)
         srv = ctypes.c_void_p()
         capi.call("ya_server_create", ctypes.byref(srv), ctypes.byref(cfg))
+        # untimed warm-up stream (first-use costs of the server's streams,
+        # arena and pinned pools stay out of the measured percentiles)
+        warm = capi.PoissonSpec(rate_per_s=rates[0], duration_s=0.3, nshapes=len(shapes), degrees=degrees,
+                                nmoduli=mods, pool=8, verify=0, seed=7)
+        capi.call("ya_server_run_poisson", srv, ctypes.byref(warm), ctypes.byref(capi.ServeReport()))
         for rate in rates:
             spec = capi.PoissonSpec(rate_per_s=rate, duration_s=duration, nshapes=len(shapes),
                                     degrees=degrees, nmoduli=mods, pool=8, verify=1, seed=99)
@@ -433,6 +492,8 @@ def main():
         out["crt_lift"] = crt_bench(stream, ev)
         log("single poly-mult latency")
         out["single_polymul_latency"] = latency_bench(stream, ev, n, L)
+        log("32-bit moduli build")
+        out["u32_moduli"] = u32_bench(stream, ev)
         sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
         out["served"] = sv
         for r in sv["runs"]:
@@ -450,9 +511,14 @@ def main():
         if "crt_lift" in out:
             log("CRT lift cpu reference")
             out["crt_lift"]["cpu_reference"] = crt_cpu_reference(out["crt_lift"])
+        if "u32_moduli" in out:
+            log("32-bit cpu reference")
+            out["u32_moduli"]["cpu_reference"] = u32_cpu_reference(out["u32_moduli"])
     if "crt_lift" in out:
         out["crt_lift"].pop("_c2_in", None)
         out["crt_lift"].pop("_c2_out", None)
+    for k in ("_a0", "_b0", "_c0"):
+        out.get("u32_moduli", {}).pop(k, None)
     capi.load().ya_batch_plan_destroy(plan)
     print(json.dumps(out), flush=True)
 

@@ -322,6 +322,44 @@ int ya_poly_crt_lift(uint64_t *out, const uint64_t *residues, size_t degree, siz
 int ya_crt_constants(size_t nmoduli, size_t limbs, uint64_t *moduli_product,
                      uint64_t *modulus_shoup, uint64_t *lifting, size_t *shift);
 
+/* ---- 32-bit moduli build -------------------------------------------------
+ * The reference's compile-time alternative (CMakeLists.txt:19
+ * YELL_USE_32BITS_MODULI -> include/yell/params_32.hpp): uint32_t words,
+ * uint64_t double words, 291 primes 2^30 - k*2^16 + 1 (kModulusBitsize 30),
+ * degrees 2^3 .. 2^15 (kMaxPolyDegree 2^15).  Same semantics, layout and value
+ * ranges as the 64-bit entry points of the same name without the 32 (the
+ * reference code is templated on params::value_type; every butterfly, Barrett
+ * and Shoup step is the same with 32-bit words): ntt<N>::forward /
+ * forward_lazy / backward (details/ntt.hpp:106-166), poly operators
+ * (ops.hpp:7-41, details/poly.hpp:119-186), and the fused poly-multiplication.
+ * Device buffers hold uint32_t words. */
+size_t ya32_num_moduli(void);
+uint32_t ya32_prime(size_t cm);
+uint32_t ya32_barrett_const(size_t cm);
+uint32_t ya32_root(size_t cm);
+int ya32_ntt_table_host(size_t degree, size_t cm, uint32_t *phi, uint32_t *phi_shoup,
+                        uint32_t *invphi, uint32_t *invphi_shoup, uint32_t *scalars);
+int ya32_ntt_init_tables(size_t degree, size_t count);
+int ya32_ntt_forward(uint32_t *x, size_t degree, size_t cm, void *stream);
+int ya32_ntt_forward_lazy(uint32_t *x, size_t degree, size_t cm, void *stream);
+int ya32_ntt_backward(uint32_t *x, size_t degree, size_t cm, void *stream);
+int ya32_poly_forward(uint32_t *poly, size_t degree, size_t nmoduli, void *stream);
+int ya32_poly_forward_lazy(uint32_t *poly, size_t degree, size_t nmoduli, void *stream);
+int ya32_poly_backward(uint32_t *poly, size_t degree, size_t nmoduli, void *stream);
+int ya32_poly_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t degree,
+                  size_t nmoduli, void *stream);
+int ya32_poly_add(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t degree,
+                  size_t nmoduli, void *stream);
+int ya32_poly_sub(uint32_t *out, const uint32_t *a, const uint32_t *b, size_t degree,
+                  size_t nmoduli, void *stream);
+int ya32_poly_negate(uint32_t *x, size_t degree, size_t nmoduli, void *stream);
+int ya32_poly_add_product_of(uint32_t *acc, const uint32_t *a, const uint32_t *b, size_t degree,
+                             size_t nmoduli, void *stream);
+int ya32_poly_multiply(uint32_t *c, const uint32_t *a, const uint32_t *b, size_t degree,
+                       size_t nmoduli, void *stream);
+int ya32_poly_multiply_many(uint32_t *c, const uint32_t *a, const uint32_t *b, size_t degree,
+                            size_t nmoduli, size_t count, void *stream);
+
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Device throughput of the VALU work this
  * engine is bound by, all operands in registers: kind 0 = v_mad_u64_u32

@@ -2,7 +2,9 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Compiled (by oracle/Makefile) together with the
 // reference's lib/ntt.cpp, lib/ops.cpp, lib/math.cpp and lib/params.cpp, read
-// in place under the reference checkout, into oracle/_ref/libyellref.so.  It
+// in place under the reference checkout, into oracle/_ref/libyellref.so, and
+// with -DYELL_USE_32BITS_MODULI (words T = uint32_t, degrees up to 2^15) into
+// oracle/_ref/libyellref32.so.  It
 // drives ntt<N>::forward / forward_lazy / backward and the ops:: functors
 // directly on caller buffers (the poly<N> container is bypassed because its
 // samplers are unseeded and its header pulls in GMP/MPFR).  Used to pin the
@@ -134,7 +136,7 @@ T yref_submod(T x, T y, size_t cm) { return yell::ops::submod{}(x, y, cm); }
 T yref_muladd(T rop, T x, T y, size_t cm) { return yell::ops::muladd{}(rop, x, y, cm); }
 T yref_shoupify(T x, size_t cm) { return yell::ops::shoupify(x, cm); }
 T yref_barrett(T hi, T lo, size_t cm) {
-  yell::params::gt_value_type v = ((yell::params::gt_value_type)hi << 64) | lo;
+  yell::params::gt_value_type v = ((yell::params::gt_value_type)hi << (8 * sizeof(T))) | lo;
   yell::ops::barret_reduction(&v, cm);
   return (T)v;
 }

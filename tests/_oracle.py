@@ -8,8 +8,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+ORACLE32 = os.path.join(ROOT, "oracle", "lib", "liboracle32.so")
 ORACLE_GMP = os.path.join(ROOT, "oracle", "lib", "liboracle_gmp.so")
 REF = os.path.join(ROOT, "oracle", "_ref", "libyellref.so")
+REF32 = os.path.join(ROOT, "oracle", "_ref", "libyellref32.so")
 REF_GMP = os.path.join(ROOT, "oracle", "_ref", "libyellref_gmp.so")
 
 u64, sz, vp, i32 = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int
@@ -27,40 +29,55 @@ def _bind(lib, sigs):
 
 
 class Oracle:
-    def __init__(self):
-        if not os.path.exists(ORACLE):
+    """The C restatement (oracle/lib/liboracle.so); word=32 selects the
+    32-bit moduli build (liboracle32.so, or32_* names, uint32 words)."""
+
+    def __init__(self, word=64):
+        path = ORACLE if word == 64 else ORACLE32
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built: make -C oracle")
-        self.lib = _bind(ctypes.CDLL(ORACLE), {
-            "or_prime": (u64, [sz]), "or_barrett_const": (u64, [sz]), "or_root": (u64, [sz]),
-            "or_barrett_hl": (u64, [u64, u64, sz]), "or_shoupify": (u64, [u64, sz]),
-            "or_mulmod": (u64, [u64, u64, sz]), "or_mulmod_shoup": (u64, [u64, u64, u64, sz]),
-            "or_addmod": (u64, [u64, u64, sz]), "or_submod": (u64, [u64, u64, sz]),
-            "or_muladd": (u64, [u64, u64, u64, sz]),
-            "or_negacyclic_forward_lazy": (i32, [vp, sz, vp, vp, u64]),
-            "or_negacyclic_backward_lazy": (i32, [vp, sz, vp, vp, u64]),
-            "or_tables_init": (i32, [sz, sz, vp, vp, vp, vp, vp]),
-            "or_forward": (i32, [vp, sz, sz]), "or_forward_lazy": (i32, [vp, sz, sz]),
-            "or_backward": (i32, [vp, sz, sz]),
-            "or_poly_forward": (i32, [vp, sz, sz]), "or_poly_forward_lazy": (i32, [vp, sz, sz]),
-            "or_poly_backward": (i32, [vp, sz, sz]),
-            "or_poly_mul": (None, [vp, vp, vp, sz, sz]), "or_poly_add": (None, [vp, vp, vp, sz, sz]),
-            "or_poly_sub": (None, [vp, vp, vp, sz, sz]), "or_poly_negate": (None, [vp, sz, sz]),
-            "or_poly_add_product_of": (None, [vp, vp, vp, sz, sz]),
-            "or_polymul": (i32, [vp, vp, vp, sz, sz]),
-            "or_schoolbook": (None, [vp, vp, vp, sz, sz]),
-        })
+        self.word = word
+        self.dt = np.uint64 if word == 64 else np.uint32
+        w = u64 if word == 64 else ctypes.c_uint32
+        pre = "or_" if word == 64 else "or32_"
+        sigs = {
+            "prime": (w, [sz]), "barrett_const": (w, [sz]), "root": (w, [sz]),
+            "barrett_hl": (w, [w, w, sz]), "shoupify": (w, [w, sz]),
+            "mulmod": (w, [w, w, sz]), "mulmod_shoup": (w, [w, w, w, sz]),
+            "addmod": (w, [w, w, sz]), "submod": (w, [w, w, sz]),
+            "muladd": (w, [w, w, w, sz]),
+            "negacyclic_forward_lazy": (i32, [vp, sz, vp, vp, w]),
+            "negacyclic_backward_lazy": (i32, [vp, sz, vp, vp, w]),
+            "tables_init": (i32, [sz, sz, vp, vp, vp, vp, vp]),
+            "forward": (i32, [vp, sz, sz]), "forward_lazy": (i32, [vp, sz, sz]),
+            "backward": (i32, [vp, sz, sz]),
+            "poly_forward": (i32, [vp, sz, sz]), "poly_forward_lazy": (i32, [vp, sz, sz]),
+            "poly_backward": (i32, [vp, sz, sz]),
+            "poly_mul": (None, [vp, vp, vp, sz, sz]), "poly_add": (None, [vp, vp, vp, sz, sz]),
+            "poly_sub": (None, [vp, vp, vp, sz, sz]), "poly_negate": (None, [vp, sz, sz]),
+            "poly_add_product_of": (None, [vp, vp, vp, sz, sz]),
+            "polymul": (i32, [vp, vp, vp, sz, sz]),
+            "schoolbook": (None, [vp, vp, vp, sz, sz]),
+        }
+        lib = _bind(ctypes.CDLL(path), {pre + k: v for k, v in sigs.items()})
+        self.raw = lib
+        # unprefixed access: self.lib.or_mulmod(...) style names map to this build
+        self.lib = _Prefixed(lib, pre)
+
+    def _arr(self, x):
+        return np.ascontiguousarray(x, dtype=self.dt)
 
     def prime(self, cm):
         return int(self.lib.or_prime(cm))
 
     def tables(self, n, cm):
-        arrs = [np.empty(n, dtype=np.uint64) for _ in range(4)]
-        sc = np.empty(4, dtype=np.uint64)
+        arrs = [np.empty(n, dtype=self.dt) for _ in range(4)]
+        sc = np.empty(4, dtype=self.dt)
         assert self.lib.or_tables_init(n, cm, *[_p(a) for a in arrs], _p(sc)) == 0
         return arrs + [sc]
 
     def _inplace(self, fn, x, *args):
-        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        y = self._arr(x).copy()
         assert getattr(self.lib, fn)(_p(y), *args) == 0
         return y
 
@@ -83,19 +100,18 @@ class Oracle:
         return self._inplace("or_poly_backward", x, n, L)
 
     def negacyclic_forward_lazy(self, x, n, w, ws, p):
-        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        y = self._arr(x).copy()
         assert self.lib.or_negacyclic_forward_lazy(_p(y), n, _p(w), _p(ws), p) == 0
         return y
 
     def negacyclic_backward_lazy(self, x, n, w, ws, p):
-        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        y = self._arr(x).copy()
         assert self.lib.or_negacyclic_backward_lazy(_p(y), n, _p(w), _p(ws), p) == 0
         return y
 
     def _binary(self, fn, a, b, n, L):
-        out = np.empty(n * L, dtype=np.uint64)
-        a = np.ascontiguousarray(a, dtype=np.uint64)
-        b = np.ascontiguousarray(b, dtype=np.uint64)
+        out = np.empty(n * L, dtype=self.dt)
+        a, b = self._arr(a), self._arr(b)
         getattr(self.lib, fn)(_p(out), _p(a), _p(b), n, L)
         return out
 
@@ -109,30 +125,39 @@ class Oracle:
         return self._binary("or_poly_sub", a, b, n, L)
 
     def poly_negate(self, a, n, L):
-        y = np.ascontiguousarray(a, dtype=np.uint64).copy()
+        y = self._arr(a).copy()
         self.lib.or_poly_negate(_p(y), n, L)
         return y
 
     def poly_add_product_of(self, acc, a, b, n, L):
-        y = np.ascontiguousarray(acc, dtype=np.uint64).copy()
-        a = np.ascontiguousarray(a, dtype=np.uint64)
-        b = np.ascontiguousarray(b, dtype=np.uint64)
+        y = self._arr(acc).copy()
+        a, b = self._arr(a), self._arr(b)
         self.lib.or_poly_add_product_of(_p(y), _p(a), _p(b), n, L)
         return y
 
     def polymul(self, a, b, n, L):
-        out = np.empty(n * L, dtype=np.uint64)
-        a = np.ascontiguousarray(a, dtype=np.uint64)
-        b = np.ascontiguousarray(b, dtype=np.uint64)
+        out = np.empty(n * L, dtype=self.dt)
+        a, b = self._arr(a), self._arr(b)
         assert self.lib.or_polymul(_p(out), _p(a), _p(b), n, L) == 0
         return out
 
     def schoolbook(self, a, b, n, cm):
-        out = np.empty(n, dtype=np.uint64)
-        a = np.ascontiguousarray(a, dtype=np.uint64)
-        b = np.ascontiguousarray(b, dtype=np.uint64)
+        out = np.empty(n, dtype=self.dt)
+        a, b = self._arr(a), self._arr(b)
         self.lib.or_schoolbook(_p(out), _p(a), _p(b), n, cm)
         return out
+
+
+class _Prefixed:
+    """Maps or_<name> attribute access to <prefix><name> of a library."""
+
+    def __init__(self, lib, prefix):
+        self._lib, self._pre = lib, prefix
+
+    def __getattr__(self, name):
+        if name.startswith("or_"):
+            name = self._pre + name[3:]
+        return getattr(self._lib, name)
 
 
 class GmpOracle:
@@ -151,26 +176,30 @@ class GmpOracle:
 
 
 class RefLib:
-    """The reference's own lib/{ntt,ops,math,params}.cpp compiled in oracle/_ref."""
+    """The reference's own lib/{ntt,ops,math,params}.cpp compiled in oracle/_ref;
+    word=32: its 32-bit moduli build (libyellref32.so)."""
 
-    def __init__(self):
-        if not os.path.exists(REF):
+    def __init__(self, word=64):
+        path = REF if word == 64 else REF32
+        if not os.path.exists(path):
             raise RuntimeError("oracle/_ref not built (reference checkout absent)")
-        self.lib = _bind(ctypes.CDLL(REF), {
+        self.dt = np.uint64 if word == 64 else np.uint32
+        w = u64 if word == 64 else ctypes.c_uint32
+        self.lib = _bind(ctypes.CDLL(path), {
             "yref_forward": (i32, [i32, vp, sz]), "yref_forward_lazy": (i32, [i32, vp, sz]),
             "yref_backward": (i32, [i32, vp, sz]),
             "yref_tables": (i32, [i32, sz, vp, vp, vp, vp, vp]),
             "yref_init_tables": (i32, [i32, sz]),
             "yref_polymul": (i32, [i32, vp, vp, vp, sz]),
             "yref_bench_polymul": (ctypes.c_double, [i32, vp, vp, sz, i32, i32]),
-            "yref_mulmod": (u64, [u64, u64, sz]), "yref_mulmod_shoup": (u64, [u64, u64, u64, sz]),
-            "yref_addmod": (u64, [u64, u64, sz]), "yref_submod": (u64, [u64, u64, sz]),
-            "yref_muladd": (u64, [u64, u64, u64, sz]), "yref_shoupify": (u64, [u64, sz]),
-            "yref_barrett": (u64, [u64, u64, sz]),
+            "yref_mulmod": (w, [w, w, sz]), "yref_mulmod_shoup": (w, [w, w, w, sz]),
+            "yref_addmod": (w, [w, w, sz]), "yref_submod": (w, [w, w, sz]),
+            "yref_muladd": (w, [w, w, w, sz]), "yref_shoupify": (w, [w, sz]),
+            "yref_barrett": (w, [w, w, sz]),
         })
 
     def _inplace(self, fn, x, n, cm):
-        y = np.ascontiguousarray(x, dtype=np.uint64).copy()
+        y = np.ascontiguousarray(x, dtype=self.dt).copy()
         assert getattr(self.lib, fn)(n.bit_length() - 1, _p(y), cm) == 0
         return y
 
@@ -184,15 +213,15 @@ class RefLib:
         return self._inplace("yref_backward", x, n, cm)
 
     def tables(self, n, cm):
-        arrs = [np.empty(n, dtype=np.uint64) for _ in range(4)]
-        sc = np.empty(4, dtype=np.uint64)
+        arrs = [np.empty(n, dtype=self.dt) for _ in range(4)]
+        sc = np.empty(4, dtype=self.dt)
         assert self.lib.yref_tables(n.bit_length() - 1, cm, *[_p(a) for a in arrs], _p(sc)) == 0
         return arrs + [sc]
 
     def polymul(self, a, b, n, L):
-        out = np.empty(n * L, dtype=np.uint64)
-        a = np.ascontiguousarray(a, dtype=np.uint64)
-        b = np.ascontiguousarray(b, dtype=np.uint64)
+        out = np.empty(n * L, dtype=self.dt)
+        a = np.ascontiguousarray(a, dtype=self.dt)
+        b = np.ascontiguousarray(b, dtype=self.dt)
         assert self.lib.yref_polymul(n.bit_length() - 1, _p(out), _p(a), _p(b), L) == 0
         return out
 

@@ -63,3 +63,37 @@ def test_prime_form_and_properties():
     for kk in range(1, k[-1]):
         if kk not in k:
             assert not _is_prime((1 << 62) - kk * (1 << 21) + 1)
+
+
+# ---- 32-bit moduli build (include/yell/params_32.hpp) -------------------------
+TABLE32 = os.path.join(os.path.dirname(HERE), "yell_amd", "csrc", "params32_table.inc")
+
+
+def _table32(name):
+    s = open(TABLE32).read()
+    m = re.search(r"#define " + name + r" \{(.*?)\}", s, re.S)
+    return [int(x.strip().rstrip("u"), 16) for x in m.group(1).replace("\\", "").split(",") if x.strip()]
+
+
+P32, PN32, ROOT32 = _table32("YA32_TABLE_P"), _table32("YA32_TABLE_PN"), _table32("YA32_TABLE_ROOT")
+FIX32 = json.load(open(os.path.join(HERE, "golden", "params32_ref.json")))
+
+
+@pytest.mark.parametrize("name", ["P", "Pn", "primitive_roots"])
+def test_32bit_tables_match_reference(name):
+    ours = {"P": P32, "Pn": PN32, "primitive_roots": ROOT32}[name]
+    assert len(ours) == FIX32["count"] == 291
+    assert [str(x) for x in ours[:64]] == FIX32[name + "_first64"]
+    assert _digest(ours) == FIX32[name + "_sha256"]
+
+
+def test_32bit_prime_form_and_properties():
+    for p, pn, r in zip(P32, PN32, ROOT32):
+        assert _is_prime(p)
+        assert (p - 1) % (1 << 16) == 0 and (1 << 29) < p < (1 << 30)
+        assert pn == ((1 << 64) // p) & 0xffffffff
+        assert pow(r, 1 << 15, p) == p - 1          # primitive 2^16-th root
+    assert all(a > b for a, b in zip(P32, P32[1:]))
+    # invkMaxPolyDegree (params_32.hpp:33) is 2^15 inverted mod each prime
+    inv = [pow(1 << 15, -1, p) for p in P32]
+    assert _digest(inv) == FIX32["invkMaxPolyDegree_sha256"]

@@ -78,6 +78,25 @@ _SIGS = {
     "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
     "ya_crt_limbs": (i32, [sz, ctypes.POINTER(ctypes.c_size_t)]),
     "ya_split_policy": (i32, [i32]),
+    "ya32_num_moduli": (sz, []),
+    "ya32_prime": (ctypes.c_uint32, [sz]),
+    "ya32_barrett_const": (ctypes.c_uint32, [sz]),
+    "ya32_root": (ctypes.c_uint32, [sz]),
+    "ya32_ntt_table_host": (i32, [sz, sz, vp, vp, vp, vp, vp]),
+    "ya32_ntt_init_tables": (i32, [sz, sz]),
+    "ya32_ntt_forward": (i32, [vp, sz, sz, vp]),
+    "ya32_ntt_forward_lazy": (i32, [vp, sz, sz, vp]),
+    "ya32_ntt_backward": (i32, [vp, sz, sz, vp]),
+    "ya32_poly_forward": (i32, [vp, sz, sz, vp]),
+    "ya32_poly_forward_lazy": (i32, [vp, sz, sz, vp]),
+    "ya32_poly_backward": (i32, [vp, sz, sz, vp]),
+    "ya32_poly_mul": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya32_poly_add": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya32_poly_sub": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya32_poly_negate": (i32, [vp, sz, sz, vp]),
+    "ya32_poly_add_product_of": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya32_poly_multiply": (i32, [vp, vp, vp, sz, sz, vp]),
+    "ya32_poly_multiply_many": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_poly_crt_lift": (i32, [vp, vp, sz, sz, vp]),
     "ya_crt_constants": (i32, [sz, sz, vp, vp, vp, ctypes.POINTER(ctypes.c_size_t)]),
     "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),
@@ -203,30 +222,34 @@ def ptr(a):
 
 
 class DeviceBuffer:
-    """A device allocation of uint64 words (freed on close / garbage collection)."""
+    """A device allocation of uint64 words (uint32 with dtype=np.uint32, the
+    32-bit moduli build), freed on close / garbage collection."""
 
-    def __init__(self, nwords):
+    def __init__(self, nwords, dtype=np.uint64):
         self.nwords = int(nwords)
+        self.dtype = np.dtype(dtype)
         p = ctypes.c_void_p()
-        call("ya_malloc", ctypes.byref(p), self.nwords * 8)
+        call("ya_malloc", ctypes.byref(p), self.nwords * self.dtype.itemsize)
         self.ptr = p.value or 0
 
     @classmethod
     def from_host(cls, arr, stream=None):
-        arr = np.ascontiguousarray(arr, dtype=np.uint64)
-        b = cls(arr.size)
+        arr = np.asarray(arr)
+        dt = np.uint32 if arr.dtype == np.uint32 else np.uint64
+        arr = np.ascontiguousarray(arr, dtype=dt)
+        b = cls(arr.size, dt)
         b.upload(arr, stream)
         return b
 
     def upload(self, arr, stream=None):
-        arr = np.ascontiguousarray(arr, dtype=np.uint64)
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
         assert arr.size <= self.nwords
-        call("ya_memcpy_htod", self.ptr, ptr(arr), arr.size * 8, stream)
+        call("ya_memcpy_htod", self.ptr, ptr(arr), arr.nbytes, stream)
 
     def download(self, nwords=None, stream=None):
         n = self.nwords if nwords is None else nwords
-        out = np.empty(n, dtype=np.uint64)
-        call("ya_memcpy_dtoh", ptr(out), self.ptr, n * 8, stream)
+        out = np.empty(n, dtype=self.dtype)
+        call("ya_memcpy_dtoh", ptr(out), self.ptr, out.nbytes, stream)
         return out
 
     def close(self):

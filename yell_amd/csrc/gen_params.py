@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Generate the 62-bit NTT-friendly prime table used by every kernel.
+"""Generate the NTT-friendly prime tables used by every kernel.
 
 Rule (reproduces the constants of the reference, include/yell/params_64.hpp:19-34):
   * P[i]  : the i-th largest prime of the form 2^62 - k*2^21 + 1 (k = 1, 2, ...),
@@ -7,9 +7,12 @@ Rule (reproduces the constants of the reference, include/yell/params_64.hpp:19-3
   * Pn[i] : low 64 bits of floor(2^128 / P[i])   (Barrett constant, params_64.hpp:29-31);
   * R[i]  : g^((P[i]-1)/2^21) mod P[i], g the smallest generator of Z/P[i]Z
             (a primitive 2^21-th root of unity, params_64.hpp:33-34).
+The 32-bit moduli build (include/yell/params_32.hpp:15-34) follows the same
+rule with 291 primes 2^30 - k*2^16 + 1, Pn = low 32 bits of floor(2^64 / P),
+primitive 2^16-th roots, and kMaxPolyDegree = 2^15.
 
-Writes params_table.inc (C initialiser lists) next to this script.
-Usage: python3 gen_params.py [count]
+Writes params_table.inc and params32_table.inc (C initialiser lists) next to
+this script.  Usage: python3 gen_params.py
 """
 import os
 import sys
@@ -64,31 +67,42 @@ def smallest_generator(p):
         g += 1
 
 
-def generate(count):
+def generate(count, bits=62, two_n=TWO_N_MAX, word=64):
     primes, k = [], 1
     while len(primes) < count:
-        p = (1 << 62) - k * TWO_N_MAX + 1
+        p = (1 << bits) - k * two_n + 1
         if is_prime(p):
             primes.append(p)
         k += 1
-    pn = [((1 << 128) // p) & ((1 << 64) - 1) for p in primes]
-    roots = [pow(smallest_generator(p), (p - 1) // TWO_N_MAX, p) for p in primes]
+    pn = [((1 << (2 * word)) // p) & ((1 << word) - 1) for p in primes]
+    roots = [pow(smallest_generator(p), (p - 1) // two_n, p) for p in primes]
     return primes, pn, roots
 
 
-def main():
-    count = int(sys.argv[1]) if len(sys.argv) > 1 else 1000
-    primes, pn, roots = generate(count)
-    here = os.path.dirname(os.path.abspath(__file__))
-    with open(os.path.join(here, "params_table.inc"), "w") as f:
-        f.write("// Generated by gen_params.py: %d primes 2^62 - k*2^21 + 1, descending.\n" % count)
-        f.write("// Columns: prime, low64(floor(2^128/prime)), primitive 2^21-th root.\n")
-        f.write("#define YA_NUM_MODULI %d\n" % count)
-        for name, vals in (("YA_TABLE_P", primes), ("YA_TABLE_PN", pn), ("YA_TABLE_ROOT", roots)):
+def write_table(path, prefix, count, primes, pn, roots, header, fmt):
+    with open(path, "w") as f:
+        f.write(header)
+        f.write("#define %s_NUM_MODULI %d\n" % (prefix, count))
+        for name, vals in ((prefix + "_TABLE_P", primes), (prefix + "_TABLE_PN", pn),
+                           (prefix + "_TABLE_ROOT", roots)):
             f.write("#define %s { \\\n" % name)
             for i in range(0, len(vals), 4):
-                f.write("  " + ", ".join("0x%016xULL" % v for v in vals[i:i + 4]) + ", \\\n")
+                f.write("  " + ", ".join(fmt % v for v in vals[i:i + 4]) + ", \\\n")
             f.write("}\n")
+
+
+def main():
+    here = os.path.dirname(os.path.abspath(__file__))
+    primes, pn, roots = generate(1000)
+    write_table(os.path.join(here, "params_table.inc"), "YA", 1000, primes, pn, roots,
+                "// Generated by gen_params.py: 1000 primes 2^62 - k*2^21 + 1, descending.\n"
+                "// Columns: prime, low64(floor(2^128/prime)), primitive 2^21-th root.\n",
+                "0x%016xULL")
+    primes, pn, roots = generate(291, bits=30, two_n=1 << 16, word=32)
+    write_table(os.path.join(here, "params32_table.inc"), "YA32", 291, primes, pn, roots,
+                "// Generated by gen_params.py: 291 primes 2^30 - k*2^16 + 1, descending.\n"
+                "// Columns: prime, low32(floor(2^64/prime)), primitive 2^16-th root.\n",
+                "0x%08xu")
 
 
 if __name__ == "__main__":

@@ -143,4 +143,91 @@ __device__ __forceinline__ uint64_t submod(uint64_t x, uint64_t y, uint64_t p) {
   return r >= p ? r - p : r;
 }
 
+// ---- 32-bit moduli build (YELL_USE_32BITS_MODULI, include/yell/params_32.hpp):
+// value_type uint32_t, gt_value_type uint64_t, primes < 2^30, so 4p < 2^32.
+// Every product is one native 32-bit multiply (v_mul_lo_u32 / v_mul_hi_u32)
+// or one v_mad_u64_u32.
+
+struct __attribute__((aligned(8))) Tw32 {
+  uint32_t w, ws;  // w' = floor(w * 2^32 / p)
+};
+
+struct PrimeK32 {
+  uint32_t p, p2;
+  __device__ __forceinline__ explicit PrimeK32(uint32_t q) : p(q), p2(q << 1) {}
+};
+
+// lib/ops.cpp:5-21 with shift = 32, delta = 32 - 30 = 2:
+// q = Pn * (x >> 32) + (x << 2) (mod 2^64); r = lo32(x) - hi32(q) * p; one -p.
+__device__ __forceinline__ uint32_t barrett_reduce32(uint64_t x, uint32_t p, uint32_t pn) {
+  const uint64_t q = (uint64_t)pn * (uint32_t)(x >> 32) + (x << 2);
+  const uint32_t r = (uint32_t)x - (uint32_t)(q >> 32) * p;
+  return r >= p ? r - p : r;
+}
+
+__device__ __forceinline__ uint32_t mulmod_barrett(uint32_t x, uint32_t y, uint32_t p, uint32_t pn) {
+  return barrett_reduce32((uint64_t)x * y, p, pn);
+}
+
+__device__ __forceinline__ uint32_t muladd_barrett(uint32_t acc, uint32_t x, uint32_t y, uint32_t p,
+                                                   uint32_t pn) {
+  return barrett_reduce32((uint64_t)x * y + acc, p, pn);
+}
+
+// include/yell/ops.hpp:122-142 with 32-bit words, x < 4p.
+__device__ __forceinline__ uint32_t mulmod_shoup(uint32_t x, uint32_t y, uint32_t ys, uint32_t p) {
+  const uint32_t r = x * y - __umulhi(x, ys) * p;
+  return r >= p ? r - p : r;
+}
+
+// lib/ntt.cpp:69-83 (bit_width = 32)
+__device__ __forceinline__ void ct_butterfly(uint32_t &x0, uint32_t &x1, const Tw32 &t, const PrimeK32 &k) {
+  const uint32_t u0 = x0 >= k.p2 ? x0 - k.p2 : x0;
+  const uint32_t v = x1 * t.w - __umulhi(x1, t.ws) * k.p;
+  x0 = u0 + v;
+  x1 = u0 + k.p2 - v;
+}
+
+// lib/ntt.cpp:47-63 (bit_width = 32)
+__device__ __forceinline__ void gs_butterfly(uint32_t &x0, uint32_t &x1, const Tw32 &t, const PrimeK32 &k) {
+  const uint32_t u0 = x0, u1 = x1;
+  uint32_t t0 = u0 + u1;
+  t0 = t0 >= k.p2 ? t0 - k.p2 : t0;
+  const uint32_t t1 = u0 + k.p2 - u1;
+  x0 = t0;
+  x1 = t1 * t.w - __umulhi(t1, t.ws) * k.p;
+}
+
+__device__ __forceinline__ uint32_t correct4p(uint32_t v, uint32_t p, uint32_t p2) {
+  v = v >= p2 ? v - p2 : v;
+  return v >= p ? v - p : v;
+}
+
+__device__ __forceinline__ uint32_t addmod(uint32_t x, uint32_t y, uint32_t p) {
+  const uint32_t r = x + y;
+  return r >= p ? r - p : r;
+}
+
+__device__ __forceinline__ uint32_t submod(uint32_t x, uint32_t y, uint32_t p) {
+  const uint32_t r = x + (p - y);
+  return r >= p ? r - p : r;
+}
+
+// Word-type traits: the per-prime constants of each build.
+template <class W>
+struct WordK;
+template <>
+struct WordK<uint64_t> {
+  using K = PrimeK;
+};
+template <>
+struct WordK<uint32_t> {
+  using K = PrimeK32;
+};
+
+template <class T>
+struct Ident {
+  using type = T;
+};
+
 }  // namespace ya

@@ -24,6 +24,14 @@ __device__ __forceinline__ void static_for(F &&f) {
   }
 }
 
+// XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (the hardware
+// deals consecutive workgroups to consecutive XCDs); give each XCD a
+// contiguous range [xcd*g/8, ...) of the g work units.
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g) {
+  const uint32_t xcd = b % 8u, pos = b / 8u;
+  return xcd * (g / 8u) + min(xcd, g % 8u) + pos;
+}
+
 // ---------------------------------------------------------------------------
 // Window start of the first forward pass of a 2^LB block.
 template <int LB, int LE>
@@ -35,10 +43,10 @@ struct FirstWindow {
 // global `src` (block base), or already in x (PRELOADED: loaded with
 // gmem_load<FirstWindow<LB,LE>::value, LE>).  On return x holds the words of
 // window [0, LE) for thread t (i = e | t << LE), in [0, 4p).
-template <int LB, int LE, bool PRELOADED = false>
-__device__ __forceinline__ void fwd_block(uint64_t (&x)[1 << LE], const uint64_t *__restrict__ src,
-                                          uint64_t *lds, const Tw *__restrict__ tw, uint32_t nlog,
-                                          uint32_t boff, uint64_t p, uint32_t t) {
+template <int LB, int LE, bool PRELOADED = false, class W, class TW>
+__device__ __forceinline__ void fwd_block(W (&x)[1 << LE], const typename Ident<W>::type *__restrict__ src,
+                                          W *lds, const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
+                                          typename Ident<W>::type p, uint32_t t) {
   constexpr int NP = (LB + LE - 1) / LE;
   static_for<0, NP>([&](auto PC) {
     constexpr int P = decltype(PC)::value;
@@ -69,10 +77,10 @@ __device__ __forceinline__ void fwd_block(uint64_t (&x)[1 << LE], const uint64_t
 //               lib/ntt.cpp:192), outputs in [0, 2p).
 enum { TOP_GS = 0, TOP_MERGED = 1, TOP_SKIP = 2 };
 
-template <int LB, int LE, int TOP>
-__device__ __forceinline__ void inv_block(uint64_t (&x)[1 << LE], uint64_t *__restrict__ dst,
-                                          uint64_t *lds, const Tw *__restrict__ tw, uint32_t nlog,
-                                          uint32_t boff, const ModDesc &d, uint32_t t) {
+template <int LB, int LE, int TOP, class W, class TW, class D>
+__device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::type *__restrict__ dst,
+                                          W *lds, const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
+                                          const D &d, uint32_t t) {
   constexpr int NP = (LB + LE - 1) / LE;
   static_for<0, NP>([&](auto QC) {
     constexpr int Q = decltype(QC)::value;

@@ -38,14 +38,6 @@ __device__ __forceinline__ void polymul_residue(uint64_t *__restrict__ c, const 
   inv_block<LB, LE, TOP_MERGED>(x, c, lds, d.inv, LB, 0u, d, t3);
 }
 
-// XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (the hardware
-// deals consecutive workgroups to consecutive XCDs); give each XCD a
-// contiguous range [xcd*g/8, ...) of the g work units.
-__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g) {
-  const uint32_t xcd = b % 8u, pos = b / 8u;
-  return xcd * (g / 8u) + min(xcd, g % 8u) + pos;
-}
-
 // Residue-major polynomials: residue res of the launch uses descs[res % nmod];
 // the XCD ranges cover the residues in prime-major order, so an XCD's L2
 // holds the twiddles of few primes.

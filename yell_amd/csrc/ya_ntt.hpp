@@ -38,6 +38,18 @@ struct ModDesc {
   uint64_t inv_n_phi_s;
 };
 
+// The same descriptor for the 32-bit moduli build (uint32 words, Tw32 pairs).
+struct ModDesc32 {
+  const Tw32 *fwd;
+  const Tw32 *inv;
+  uint32_t p;
+  uint32_t pn;           // Barrett constant low32(floor(2^64 / p))
+  uint32_t inv_n;
+  uint32_t inv_n_s;
+  uint32_t inv_n_phi;
+  uint32_t inv_n_phi_s;
+};
+
 __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
 
 // Twiddle load through a global-address-space pointer (the descriptor's
@@ -47,6 +59,11 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ Tw load_tw(const Tw *p) {
   const u64x2 v = *(const __attribute__((address_space(1))) u64x2 *)p;
   return Tw{v.x, v.y};
+}
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Tw32 load_tw(const Tw32 *p) {
+  const u32x2 v = *(const __attribute__((address_space(1))) u32x2 *)p;
+  return Tw32{v.x, v.y};
 }
 
 // Twiddle base for a pass whose window starts at WLO: the bits of gbase are
@@ -87,15 +104,17 @@ __device__ __forceinline__ uint32_t win_base(uint32_t t) {
   return tlo | (thi << (WLO + LE));
 }
 
-template <int WLO, int LE>
-__device__ __forceinline__ void lds_store(uint64_t *s, const uint64_t (&x)[1 << LE], uint32_t t) {
+// The window helpers below are generic in the word type W (uint64_t, or
+// uint32_t in the 32-bit moduli build), deduced from the register array.
+template <int WLO, int LE, class W>
+__device__ __forceinline__ void lds_store(W *s, const W (&x)[1 << LE], uint32_t t) {
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) s[lds_pad(base | ((uint32_t)e << WLO))] = x[e];
 }
 
-template <int WLO, int LE>
-__device__ __forceinline__ void lds_load(const uint64_t *s, uint64_t (&x)[1 << LE], uint32_t t) {
+template <int WLO, int LE, class W>
+__device__ __forceinline__ void lds_load(const W *s, W (&x)[1 << LE], uint32_t t) {
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = s[lds_pad(base | ((uint32_t)e << WLO))];
@@ -109,23 +128,31 @@ __device__ __forceinline__ void lds_load(const uint64_t *s, uint64_t (&x)[1 << L
 typedef __attribute__((address_space(1))) uint64_t gword;
 typedef __attribute__((address_space(1))) unsigned char gbyte;
 
-template <int WLO, int LE>
-__device__ __forceinline__ void gmem_load(const uint64_t *__restrict__ g, uint64_t (&x)[1 << LE],
+template <class W>
+struct WordShift {
+  static constexpr int value = sizeof(W) == 8 ? 3 : 2;
+};
+
+template <int WLO, int LE, class W>
+__device__ __forceinline__ void gmem_load(const typename Ident<W>::type *__restrict__ g, W (&x)[1 << LE],
                                           uint32_t t) {
+  typedef __attribute__((address_space(1))) W gw;
   const gbyte *gb = (const gbyte *)g;
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e)
-    x[e] = *(const gword *)(gb + ((base | ((uint32_t)e << WLO)) << 3));
+    x[e] = *(const gw *)(gb + ((base | ((uint32_t)e << WLO)) << WordShift<W>::value));
 }
 
-template <int WLO, int LE>
-__device__ __forceinline__ void gmem_store(uint64_t *__restrict__ g, const uint64_t (&x)[1 << LE],
+template <int WLO, int LE, class W>
+__device__ __forceinline__ void gmem_store(typename Ident<W>::type *__restrict__ g, const W (&x)[1 << LE],
                                            uint32_t t) {
+  typedef __attribute__((address_space(1))) W gw;
   gbyte *gb = (gbyte *)g;
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) *(gword *)(gb + ((base | ((uint32_t)e << WLO)) << 3)) = x[e];
+  for (int e = 0; e < (1 << LE); ++e)
+    *(gw *)(gb + ((base | ((uint32_t)e << WLO)) << WordShift<W>::value)) = x[e];
 }
 
 // Descriptor read through the constant address space: when the address is
@@ -149,17 +176,17 @@ __device__ __forceinline__ ModDesc load_desc(const ModDesc *p) {
 // index of word 0 of this thread with the window bits and the bits below the
 // window cleared (i.e. block offset | thi << (WLO+LE)); nlog = log2 N of the
 // whole residue.
-template <int WLO, int LE, int HI, int LO>
-__device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
-                                           uint32_t nlog, uint32_t gbase, uint64_t p) {
-  const PrimeK k(p);
+template <int WLO, int LE, int HI, int LO, class W, class TW>
+__device__ __forceinline__ void fwd_layers(W (&x)[1 << LE], const TW *__restrict__ tw, uint32_t nlog,
+                                           uint32_t gbase, typename Ident<W>::type p) {
+  const typename WordK<W>::K k(p);
 #pragma unroll
   for (int b = HI - 1; b >= LO; --b) {
     const int j = b - WLO;
     const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
 #pragma unroll
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
-      const Tw t = load_tw(tw + toff + g);
+      const TW t = load_tw(tw + toff + g);
 #pragma unroll
       for (int q = 0; q < (1 << j); ++q) {
         const int e0 = (g << (j + 1)) | q;
@@ -170,17 +197,17 @@ __device__ __forceinline__ void fwd_layers(uint64_t (&x)[1 << LE], const Tw *__r
 }
 
 // Inverse (GS) layers b = LO ... HI-1 of one pass (same indexing).
-template <int WLO, int LE, int HI, int LO>
-__device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__restrict__ tw,
-                                           uint32_t nlog, uint32_t gbase, uint64_t p) {
-  const PrimeK k(p);
+template <int WLO, int LE, int HI, int LO, class W, class TW>
+__device__ __forceinline__ void inv_layers(W (&x)[1 << LE], const TW *__restrict__ tw, uint32_t nlog,
+                                           uint32_t gbase, typename Ident<W>::type p) {
+  const typename WordK<W>::K k(p);
 #pragma unroll
   for (int b = LO; b < HI; ++b) {
     const int j = b - WLO;
     const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
 #pragma unroll
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
-      const Tw t = load_tw(tw + toff + g);
+      const TW t = load_tw(tw + toff + g);
 #pragma unroll
       for (int q = 0; q < (1 << j); ++q) {
         const int e0 = (g << (j + 1)) | q;
@@ -192,21 +219,21 @@ __device__ __forceinline__ void inv_layers(uint64_t (&x)[1 << LE], const Tw *__r
 
 // The merged last inverse layer on window bit J (include/yell/details/ntt.hpp:151-159):
 // x0 = (u + v) * N^-1, x1 = (u - v + 2p) * N^-1 * invphi[1], each by Shoup, canonical.
-template <int LE, int J>
-__device__ __forceinline__ void inv_last_layer(uint64_t (&x)[1 << LE], const ModDesc &d) {
-  const uint64_t p = d.p, p2 = p << 1;
+template <int LE, int J, class W, class D>
+__device__ __forceinline__ void inv_last_layer(W (&x)[1 << LE], const D &d) {
+  const W p = d.p, p2 = p << 1;
 #pragma unroll
   for (int e0 = 0; e0 < (1 << LE); ++e0) {
     if (e0 & (1 << J)) continue;
-    const uint64_t u = x[e0], v = x[e0 | (1 << J)];
+    const W u = x[e0], v = x[e0 | (1 << J)];
     x[e0] = mulmod_shoup(u + v, d.inv_n, d.inv_n_s, p);
     x[e0 | (1 << J)] = mulmod_shoup(u - v + p2, d.inv_n_phi, d.inv_n_phi_s, p);
   }
 }
 
-template <int LE>
-__device__ __forceinline__ void correct_all(uint64_t (&x)[1 << LE], uint64_t p) {
-  const uint64_t p2 = p << 1;
+template <int LE, class W>
+__device__ __forceinline__ void correct_all(W (&x)[1 << LE], typename Ident<W>::type p) {
+  const W p2 = p << 1;
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = correct4p(x[e], p, p2);
 }
