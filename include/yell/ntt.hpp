@@ -1,6 +1,7 @@
 // yell/ntt.hpp -- drop-in for the reference's yell::ntt<N>
 // (include/yell/ntt.hpp:6-66): the same static interface, with the tables
-// resident on the GPU and the transforms run by libyell_amd (yell_amd.h).
+// resident on the GPU and the transforms run by libyell_amd (yell_amd.h);
+// 64-bit moduli by default, 32-bit with -DYELL_USE_32BITS_MODULI.
 // op may be device memory (a poly's ptr_at) -- transformed in place on the
 // device -- or host memory, which is staged through a device buffer.
 // Failures throw std::runtime_error (the reference asserts).
@@ -28,6 +29,26 @@ inline bool is_device_ptr(const void *p) {
   return d != 0;
 }
 
+// Entry points of the build that params::value_type selects: uint64_t words
+// go to ya_*, uint32_t words (YELL_USE_32BITS_MODULI) to ya32_*.
+namespace detail {
+inline int api_forward(uint64_t *x, size_t n, size_t cm) { return ya_ntt_forward(x, n, cm, nullptr); }
+inline int api_forward(uint32_t *x, size_t n, size_t cm) { return ya32_ntt_forward(x, n, cm, nullptr); }
+inline int api_forward_lazy(uint64_t *x, size_t n, size_t cm) { return ya_ntt_forward_lazy(x, n, cm, nullptr); }
+inline int api_forward_lazy(uint32_t *x, size_t n, size_t cm) { return ya32_ntt_forward_lazy(x, n, cm, nullptr); }
+inline int api_backward(uint64_t *x, size_t n, size_t cm) { return ya_ntt_backward(x, n, cm, nullptr); }
+inline int api_backward(uint32_t *x, size_t n, size_t cm) { return ya32_ntt_backward(x, n, cm, nullptr); }
+inline int api_init_tables(const uint64_t *, size_t n, size_t k) { return ya_ntt_init_tables(n, k); }
+inline int api_init_tables(const uint32_t *, size_t n, size_t k) { return ya32_ntt_init_tables(n, k); }
+inline int api_table(size_t n, size_t cm, uint64_t *a, uint64_t *b, uint64_t *c, uint64_t *d, uint64_t *sc) {
+  return ya_ntt_table_download(n, cm, a, b, c, d, sc);
+}
+// the 32-bit build's device tables are uploaded from this same host construction
+inline int api_table(size_t n, size_t cm, uint32_t *a, uint32_t *b, uint32_t *c, uint32_t *d, uint32_t *sc) {
+  return ya32_ntt_table_host(n, cm, a, b, c, d, sc);
+}
+}  // namespace detail
+
 template <size_t degree_>
 class ntt {
   static constexpr size_t degree = degree_;
@@ -53,9 +74,9 @@ class ntt {
     t->invphiTbl.resize(degree);
     t->shoupInvphiTbl.resize(degree);
     T sc[4];
-    ya_check(ya_ntt_table_download(degree, cm, t->phiTbl.data(), t->shoupPhiTbl.data(),
-                                   t->invphiTbl.data(), t->shoupInvphiTbl.data(), sc),
-             "ya_ntt_table_download");
+    ya_check(detail::api_table(degree, cm, t->phiTbl.data(), t->shoupPhiTbl.data(), t->invphiTbl.data(),
+                               t->shoupInvphiTbl.data(), sc),
+             "ntt table");
     t->invDegree = sc[0];
     t->shoupInvDegree = sc[1];
     t->invphiInvDegree = sc[2];
@@ -65,7 +86,9 @@ class ntt {
     return raw;
   }
 
-  static void init_ntt_tables(size_t K) { ya_check(ya_ntt_init_tables(degree, K), "ya_ntt_init_tables"); }
+  static void init_ntt_tables(size_t K) {
+    ya_check(detail::api_init_tables((const T *)nullptr, degree, K), "init_ntt_tables");
+  }
 
   static void forward(T *op, size_t cm) { run(op, cm, 0); }
   static void forward_lazy(T *op, size_t cm) { run(op, cm, 1); }
@@ -74,9 +97,9 @@ class ntt {
  private:
   static void run(T *op, size_t cm, int dir) {
     auto call = [&](T *x) {
-      if (dir == 0) ya_check(ya_ntt_forward(x, degree, cm, nullptr), "ya_ntt_forward");
-      else if (dir == 1) ya_check(ya_ntt_forward_lazy(x, degree, cm, nullptr), "ya_ntt_forward_lazy");
-      else ya_check(ya_ntt_backward(x, degree, cm, nullptr), "ya_ntt_backward");
+      if (dir == 0) ya_check(detail::api_forward(x, degree, cm), "ntt forward");
+      else if (dir == 1) ya_check(detail::api_forward_lazy(x, degree, cm), "ntt forward_lazy");
+      else ya_check(detail::api_backward(x, degree, cm), "ntt backward");
     };
     if (is_device_ptr(op)) {
       call(op);

@@ -17,21 +17,25 @@ static inline void mod_correct(T &a, const T p) {
   a -= (a >= p) ? p : T(0);
 }
 
-// Barrett: q = Pn * (x >> 64) + (x << 2) (mod 2^128); r = lo64(x) - hi64(q) * p;
-// one conditional subtraction.
+// Barrett (lib/ops.cpp:5-21): with w = kModulusRepresentationBitsize and
+// d = w - kModulusBitsize, q = Pn * (x >> w) + (x << d) (mod 2^2w);
+// r = lo(x) - hi(q) * p (mod 2^w); one conditional subtraction.
 inline void barret_reduction(params::gt_value_type *x, size_t cm) {
   if (!x) return;
   using gT = params::gt_value_type;
   using T = params::value_type;
+  constexpr unsigned w = params::kModulusRepresentationBitsize;
+  constexpr unsigned d = w - params::kModulusBitsize;
   const gT v = *x;
-  const gT q = (gT)params::Pn[cm] * (v >> 64) + (v << 2);
-  T r = (T)v - (T)(q >> 64) * params::P[cm];
+  const gT q = (gT)params::Pn[cm] * (v >> w) + (v << d);
+  T r = (T)v - (T)(q >> w) * params::P[cm];
   mod_correct(r, params::P[cm]);
   *x = r;
 }
 
 inline params::value_type shoupify(params::value_type x, size_t cm) {
-  return (params::value_type)(((params::gt_value_type)x << 64) / params::P[cm]);
+  return (params::value_type)(((params::gt_value_type)x << params::kModulusRepresentationBitsize) /
+                              params::P[cm]);
 }
 
 struct addmod {
@@ -61,7 +65,7 @@ struct mulmod_shoup {
   T operator()(T x, T y, T yprime, size_t cm) const { compute(x, y, yprime, cm); return x; }
   void compute(T &x, T y, T yprime, size_t cm) const {
     const T p = params::P[cm];
-    const T q = (T)(((params::gt_value_type)x * yprime) >> 64) * p;
+    const T q = (T)(((params::gt_value_type)x * yprime) >> params::kModulusRepresentationBitsize) * p;
     x = x * y - q;
     mod_correct(x, p);
   }

@@ -11,7 +11,10 @@
 //     to_host() / from_host() move whole polynomials;
 //   * samplers are seeded (yell::set_seed); gaussian takes sigma directly
 //     (the reference wraps an MPFR-built FastGaussianNoise table);
-//   * poly2mpz (the CRT lift, include/yell/gmp.hpp:41-70) runs on the GPU;
+//   * with -DYELL_USE_32BITS_MODULI the 32-bit build (uint32 words, 30-bit
+//     primes, degrees up to 2^14) is used, as in the reference;
+//   * poly2mpz (the CRT lift, include/yell/gmp.hpp:41-70) runs on the GPU
+//     (64-bit build);
 //     it is declared when <gmp.h> is on the include path (link -lgmp), and
 //     to_limbs() gives the same integers as little-endian 64-bit limbs
 //     without GMP;
@@ -34,6 +37,42 @@
 #endif
 
 namespace yell {
+
+namespace detail {  // poly entry points of the build params::value_type selects
+#define YELL_AMD_API2(NAME, F64, F32)                                                                   \
+  inline int NAME(uint64_t *x, size_t n, size_t l) { return F64(x, n, l, nullptr); }                 \
+  inline int NAME(uint32_t *x, size_t n, size_t l) { return F32(x, n, l, nullptr); }
+YELL_AMD_API2(api_poly_forward, ya_poly_forward, ya32_poly_forward)
+YELL_AMD_API2(api_poly_forward_lazy, ya_poly_forward_lazy, ya32_poly_forward_lazy)
+YELL_AMD_API2(api_poly_backward, ya_poly_backward, ya32_poly_backward)
+YELL_AMD_API2(api_poly_negate, ya_poly_negate, ya32_poly_negate)
+#undef YELL_AMD_API2
+#define YELL_AMD_API3(NAME, F64, F32)                                                                   \
+  inline int NAME(uint64_t *o, const uint64_t *a, const uint64_t *b, size_t n, size_t l, void *s) {  \
+    return F64(o, a, b, n, l, s);                                                                     \
+  }                                                                                                   \
+  inline int NAME(uint32_t *o, const uint32_t *a, const uint32_t *b, size_t n, size_t l, void *s) {  \
+    return F32(o, a, b, n, l, s);                                                                     \
+  }
+YELL_AMD_API3(api_poly_mul, ya_poly_mul, ya32_poly_mul)
+YELL_AMD_API3(api_poly_add, ya_poly_add, ya32_poly_add)
+YELL_AMD_API3(api_poly_sub, ya_poly_sub, ya32_poly_sub)
+YELL_AMD_API3(api_poly_add_product_of, ya_poly_add_product_of, ya32_poly_add_product_of)
+YELL_AMD_API3(api_poly_multiply, ya_poly_multiply, ya32_poly_multiply)
+#undef YELL_AMD_API3
+inline int api_sample_uniform(uint64_t *h, size_t n, size_t l, uint64_t s) { return ya_sample_uniform(h, n, l, s); }
+inline int api_sample_uniform(uint32_t *h, size_t n, size_t l, uint64_t s) { return ya32_sample_uniform(h, n, l, s); }
+inline int api_sample_zo(uint64_t *h, size_t n, size_t l, uint32_t r, uint64_t s) { return ya_sample_zo(h, n, l, r, s); }
+inline int api_sample_zo(uint32_t *h, size_t n, size_t l, uint32_t r, uint64_t s) { return ya32_sample_zo(h, n, l, r, s); }
+inline int api_sample_hwt(uint64_t *h, size_t n, size_t l, uint32_t w, uint64_t s) { return ya_sample_hwt(h, n, l, w, s); }
+inline int api_sample_hwt(uint32_t *h, size_t n, size_t l, uint32_t w, uint64_t s) { return ya32_sample_hwt(h, n, l, w, s); }
+inline int api_sample_gaussian(uint64_t *h, size_t n, size_t l, double g, uint64_t s) {
+  return ya_sample_gaussian(h, n, l, g, s);
+}
+inline int api_sample_gaussian(uint32_t *h, size_t n, size_t l, double g, uint64_t s) {
+  return ya32_sample_gaussian(h, n, l, g, s);
+}
+}  // namespace detail
 
 struct uniform {};
 struct hwt_dist {
@@ -64,8 +103,9 @@ inline uint64_t next_seed() { return detail::seed_counter().fetch_add(0x9E3779B9
 
 template <size_t degree_>
 class poly {
-  static_assert(degree_ >= 8 && degree_ <= (1u << 17) && (degree_ & (degree_ - 1)) == 0,
-                "degree must be a power of two in [2^3, 2^17]");
+  static_assert(degree_ >= 8 && degree_ <= (1u << 17) && degree_ < params::kMaxPolyDegree &&
+                    (degree_ & (degree_ - 1)) == 0,
+                "degree must be a power of two in [2^3, 2^17] below params::kMaxPolyDegree");
 
  public:
   using value_type = params::value_type;
@@ -99,12 +139,18 @@ class poly {
   }
 
   // ---- samplers (details/poly.hpp:372-501), drawn on the host, uploaded.
-  void set(const uniform &) { sample([&](value_type *h, uint64_t s) { return ya_sample_uniform(h, degree, nmoduli_, s); }); }
-  void set(const ZO_dist &m) { sample([&](value_type *h, uint64_t s) { return ya_sample_zo(h, degree, nmoduli_, m.rho, s); }); }
-  void set(const hwt_dist &m) { sample([&](value_type *h, uint64_t s) { return ya_sample_hwt(h, degree, nmoduli_, m.hwt, s); }); }
+  void set(const uniform &) {
+    sample([&](value_type *h, uint64_t s) { return detail::api_sample_uniform(h, degree, nmoduli_, s); });
+  }
+  void set(const ZO_dist &m) {
+    sample([&](value_type *h, uint64_t s) { return detail::api_sample_zo(h, degree, nmoduli_, m.rho, s); });
+  }
+  void set(const hwt_dist &m) {
+    sample([&](value_type *h, uint64_t s) { return detail::api_sample_hwt(h, degree, nmoduli_, m.hwt, s); });
+  }
   void set(const gaussian &m) {
     std::vector<value_type> h(degree * nmoduli_);
-    ya_check(ya_sample_gaussian(h.data(), degree, nmoduli_, m.sigma, next_seed()), "ya_sample_gaussian");
+    ya_check(detail::api_sample_gaussian(h.data(), degree, nmoduli_, m.sigma, next_seed()), "gaussian sampler");
     if (m.amplifier != 1) {  // r *= amplifier on the signed sample (details/poly.hpp:419-422)
       for (size_t cm = 0; cm < nmoduli_; ++cm) {
         const value_type p = params::P[cm];
@@ -152,6 +198,7 @@ class poly {
     if (!h.empty()) ya_check(ya_memcpy_htod(d_, h.data(), h.size() * sizeof(value_type), nullptr), "from_host");
   }
 
+#ifndef YELL_USE_32BITS_MODULI
   // ---- CRT lift (details/poly.hpp:162-165): coefficient i of the result is
   // words [i * lift_limbs(), (i + 1) * lift_limbs()), little-endian.
   size_t lift_limbs() const {
@@ -183,23 +230,24 @@ class poly {
     return rop;
   }
 #endif
+#endif  // YELL_USE_32BITS_MODULI
 
   bool operator==(const poly &o) const { return nmoduli_ == o.nmoduli_ && to_host() == o.to_host(); }
   bool operator!=(const poly &o) const { return !(*this == o); }
 
   // ---- NTT (details/poly.hpp:341-357)
-  void forward() { if (nmoduli_) ya_check(ya_poly_forward(d_, degree, nmoduli_, nullptr), "forward"); }
-  void forward_lazy() { if (nmoduli_) ya_check(ya_poly_forward_lazy(d_, degree, nmoduli_, nullptr), "forward_lazy"); }
-  void backward() { if (nmoduli_) ya_check(ya_poly_backward(d_, degree, nmoduli_, nullptr), "backward"); }
+  void forward() { if (nmoduli_) ya_check(detail::api_poly_forward(d_, degree, nmoduli_), "forward"); }
+  void forward_lazy() { if (nmoduli_) ya_check(detail::api_poly_forward_lazy(d_, degree, nmoduli_), "forward_lazy"); }
+  void backward() { if (nmoduli_) ya_check(detail::api_poly_backward(d_, degree, nmoduli_), "backward"); }
 
   // ---- arithmetic (details/poly.hpp:289-328)
   poly &add_product_of(const poly &a, const poly &b) {
     same(a);
     same(b);
-    ya_check(ya_poly_add_product_of(d_, a.d_, b.d_, degree, nmoduli_, nullptr), "add_product_of");
+    ya_check(detail::api_poly_add_product_of(d_, a.d_, b.d_, degree, nmoduli_, nullptr), "add_product_of");
     return *this;
   }
-  void negate() { if (nmoduli_) ya_check(ya_poly_negate(d_, degree, nmoduli_, nullptr), "negate"); }
+  void negate() { if (nmoduli_) ya_check(detail::api_poly_negate(d_, degree, nmoduli_), "negate"); }
   // a * s computed as a * r + a * (s - r) with a fresh uniform r (details/poly.hpp:305-320).
   poly &mul_sensitive_poly(const poly &s) {
     same(s);
@@ -274,12 +322,12 @@ class poly {
     if (L) ya_check(FN(a.data(), a.data(), b.data(), N, L, nullptr), #FN);            \
     return a;                                                                         \
   }
-YELL_AMD_BINARY(operator*, ya_poly_mul)
-YELL_AMD_BINARY(operator+, ya_poly_add)
-YELL_AMD_BINARY(operator-, ya_poly_sub)
-YELL_AMD_SELF(operator*=, ya_poly_mul)
-YELL_AMD_SELF(operator+=, ya_poly_add)
-YELL_AMD_SELF(operator-=, ya_poly_sub)
+YELL_AMD_BINARY(operator*, detail::api_poly_mul)
+YELL_AMD_BINARY(operator+, detail::api_poly_add)
+YELL_AMD_BINARY(operator-, detail::api_poly_sub)
+YELL_AMD_SELF(operator*=, detail::api_poly_mul)
+YELL_AMD_SELF(operator+=, detail::api_poly_add)
+YELL_AMD_SELF(operator-=, detail::api_poly_sub)
 #undef YELL_AMD_BINARY
 #undef YELL_AMD_SELF
 
@@ -289,7 +337,7 @@ template <size_t N>
 poly<N> multiply(const poly<N> &a, const poly<N> &b) {
   const size_t L = std::min(a.moduli_count(), b.moduli_count());
   poly<N> r(L);
-  if (L) ya_check(ya_poly_multiply(r.data(), a.data(), b.data(), N, L, nullptr), "ya_poly_multiply");
+  if (L) ya_check(detail::api_poly_multiply(r.data(), a.data(), b.data(), N, L, nullptr), "poly multiply");
   return r;
 }
 

@@ -359,6 +359,12 @@ int ya32_poly_multiply(uint32_t *c, const uint32_t *a, const uint32_t *b, size_t
                        size_t nmoduli, void *stream);
 int ya32_poly_multiply_many(uint32_t *c, const uint32_t *a, const uint32_t *b, size_t degree,
                             size_t nmoduli, size_t count, void *stream);
+/* Host samplers of the 32-bit build (same rules as ya_sample_*, with the
+ * 30-bit primes: uniform words masked to 30 bits, minus p once). */
+int ya32_sample_uniform(uint32_t *out, size_t degree, size_t nmoduli, uint64_t seed);
+int ya32_sample_gaussian(uint32_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed);
+int ya32_sample_zo(uint32_t *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed);
+int ya32_sample_hwt(uint32_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Device throughput of the VALU work this

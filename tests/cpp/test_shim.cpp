@@ -1,6 +1,8 @@
 // test_shim.cpp -- exercises the C++ drop-in headers (include/yell/*.hpp)
 // on the GPU the way the reference's own users call yell::poly / ntt / ops
 // (benchmark_ntt.cc, example.cpp).  Exit status 0 = every check passed.
+// Built twice: 64-bit moduli (test_shim) and -DYELL_USE_32BITS_MODULI
+// (test_shim32), as the reference's compile-time switch.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -110,6 +112,7 @@ static void test_rlwe_example() {
   CHECK(worst < (1ll << 20), "decryption noise %lld too large", worst);
   std::printf("rlwe: max |noise| = %lld\n", worst);
 
+#ifndef YELL_USE_32BITS_MODULI
   // example.cpp:67: the lifted plaintext; each integer reduces to its residues.
   const size_t w = plain.lift_limbs();
   const std::vector<uint64_t> limbs = plain.to_limbs();
@@ -136,6 +139,7 @@ static void test_rlwe_example() {
   CHECK(bad == 0, "poly2mpz: %d coefficients do not reduce to their residues", bad);
   std::printf("rlwe: poly2mpz checked (GMP)\n");
 #endif
+#endif  // YELL_USE_32BITS_MODULI
 }
 
 int main() {

@@ -30,8 +30,8 @@ def digest(vals):
 
 
 def main():
-    tabs = arrays(open(REF).read())
-    fix = {"source": "include/yell/params_64.hpp:27,31,34", "count": len(tabs["P"])}
+    tabs = arrays(open(REF).read(), ("P", "Pn", "primitive_roots", "invkMaxPolyDegree"))
+    fix = {"source": "include/yell/params_64.hpp:27,31,34,38", "count": len(tabs["P"])}
     for k, v in tabs.items():
         fix[k + "_first64"] = [str(x) for x in v[:64]]
         fix[k + "_sha256"] = digest(v)

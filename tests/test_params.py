@@ -19,6 +19,7 @@ def _table(name):
 
 
 P, PN, ROOT = _table("YA_TABLE_P"), _table("YA_TABLE_PN"), _table("YA_TABLE_ROOT")
+INVDEG = _table("YA_TABLE_INVDEG")
 FIX = json.load(open(os.path.join(HERE, "golden", "params_ref.json")))
 
 
@@ -26,7 +27,8 @@ def _digest(v):
     return hashlib.sha256(",".join(str(x) for x in v).encode()).hexdigest()
 
 
-@pytest.mark.parametrize("ours,name", [(P, "P"), (PN, "Pn"), (ROOT, "primitive_roots")])
+@pytest.mark.parametrize("ours,name", [(P, "P"), (PN, "Pn"), (ROOT, "primitive_roots"),
+                                       (INVDEG, "invkMaxPolyDegree")])
 def test_matches_reference_tables(ours, name):
     assert len(ours) == FIX["count"] == 1000
     assert [str(x) for x in ours[:64]] == FIX[name + "_first64"]
@@ -76,12 +78,13 @@ def _table32(name):
 
 
 P32, PN32, ROOT32 = _table32("YA32_TABLE_P"), _table32("YA32_TABLE_PN"), _table32("YA32_TABLE_ROOT")
+INVDEG32 = _table32("YA32_TABLE_INVDEG")
 FIX32 = json.load(open(os.path.join(HERE, "golden", "params32_ref.json")))
 
 
-@pytest.mark.parametrize("name", ["P", "Pn", "primitive_roots"])
+@pytest.mark.parametrize("name", ["P", "Pn", "primitive_roots", "invkMaxPolyDegree"])
 def test_32bit_tables_match_reference(name):
-    ours = {"P": P32, "Pn": PN32, "primitive_roots": ROOT32}[name]
+    ours = {"P": P32, "Pn": PN32, "primitive_roots": ROOT32, "invkMaxPolyDegree": INVDEG32}[name]
     assert len(ours) == FIX32["count"] == 291
     assert [str(x) for x in ours[:64]] == FIX32[name + "_first64"]
     assert _digest(ours) == FIX32[name + "_sha256"]

@@ -85,22 +85,30 @@ SYS_GMP = "/usr/lib/x86_64-linux-gnu/libgmp.so.10"
 
 
 def build_cpp_tests():
-    """tests/cpp/*.cpp -> tests/cpp/build/<name>, linked against libyell_amd.so."""
+    """tests/cpp/*.cpp -> tests/cpp/build/<name>, linked against libyell_amd.so;
+    test_shim.cpp is also built with -DYELL_USE_32BITS_MODULI (test_shim32)."""
     src_dir = os.path.join(ROOT, "tests", "cpp")
     out_dir = os.path.join(src_dir, "build")
     os.makedirs(out_dir, exist_ok=True)
+    have_gmp = os.path.exists(os.path.join(GMP_INC, "gmp.h")) and os.path.exists(SYS_GMP)
     for src in sorted(glob.glob(os.path.join(src_dir, "*.cpp"))):
-        exe = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
-        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
-               "-o", exe, "-L", os.path.dirname(LIB), "-lyell_amd",
-               "-Wl,-rpath,$ORIGIN/../../../yell_amd/lib"]
-        if os.path.exists(os.path.join(GMP_INC, "gmp.h")) and os.path.exists(SYS_GMP):
-            # poly2mpz of the C++ drop-in: GMP headers here, the system library by path
-            cmd[6:6] = ["-I", GMP_INC]
-            cmd.append(SYS_GMP)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError("C++ test build failed:\n" + r.stdout + r.stderr)
+        name = os.path.splitext(os.path.basename(src))[0]
+        variants = [(name, [])]
+        if name == "test_shim":
+            variants.append((name + "32", ["-DYELL_USE_32BITS_MODULI"]))
+        for exe_name, defs in variants:
+            exe = os.path.join(out_dir, exe_name)
+            cmd = ["g++", "-std=c++17", "-O2", "-Wall"] + defs + ["-I", os.path.join(ROOT, "include"), src,
+                                                                "-o", exe, "-L", os.path.dirname(LIB), "-lyell_amd",
+                                                                "-Wl,-rpath,$ORIGIN/../../../yell_amd/lib"]
+            if have_gmp and not defs:
+                # poly2mpz of the C++ drop-in: GMP headers here, the system library by path
+                i = cmd.index(src)
+                cmd[i:i] = ["-I", GMP_INC]
+                cmd.append(SYS_GMP)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError("C++ test build failed:\n" + r.stdout + r.stderr)
 
 
 def build_oracle():

@@ -97,6 +97,10 @@ _SIGS = {
     "ya32_poly_add_product_of": (i32, [vp, vp, vp, sz, sz, vp]),
     "ya32_poly_multiply": (i32, [vp, vp, vp, sz, sz, vp]),
     "ya32_poly_multiply_many": (i32, [vp, vp, vp, sz, sz, sz, vp]),
+    "ya32_sample_uniform": (i32, [vp, sz, sz, u64]),
+    "ya32_sample_gaussian": (i32, [vp, sz, sz, ctypes.c_double, u64]),
+    "ya32_sample_zo": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
+    "ya32_sample_hwt": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
     "ya_poly_crt_lift": (i32, [vp, vp, sz, sz, vp]),
     "ya_crt_constants": (i32, [sz, sz, vp, vp, vp, ctypes.POINTER(ctypes.c_size_t)]),
     "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),

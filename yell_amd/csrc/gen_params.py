@@ -79,12 +79,14 @@ def generate(count, bits=62, two_n=TWO_N_MAX, word=64):
     return primes, pn, roots
 
 
-def write_table(path, prefix, count, primes, pn, roots, header, fmt):
+def write_table(path, prefix, count, primes, pn, roots, header, fmt, log_max_degree):
+    # invkMaxPolyDegree (params_64.hpp:38, params_32.hpp:33): 2^log_max_degree inverted mod P
+    invdeg = [pow(1 << log_max_degree, -1, p) for p in primes]
     with open(path, "w") as f:
         f.write(header)
         f.write("#define %s_NUM_MODULI %d\n" % (prefix, count))
         for name, vals in ((prefix + "_TABLE_P", primes), (prefix + "_TABLE_PN", pn),
-                           (prefix + "_TABLE_ROOT", roots)):
+                           (prefix + "_TABLE_ROOT", roots), (prefix + "_TABLE_INVDEG", invdeg)):
             f.write("#define %s { \\\n" % name)
             for i in range(0, len(vals), 4):
                 f.write("  " + ", ".join(fmt % v for v in vals[i:i + 4]) + ", \\\n")
@@ -96,13 +98,15 @@ def main():
     primes, pn, roots = generate(1000)
     write_table(os.path.join(here, "params_table.inc"), "YA", 1000, primes, pn, roots,
                 "// Generated by gen_params.py: 1000 primes 2^62 - k*2^21 + 1, descending.\n"
-                "// Columns: prime, low64(floor(2^128/prime)), primitive 2^21-th root.\n",
-                "0x%016xULL")
+                "// Columns: prime, low64(floor(2^128/prime)), primitive 2^21-th root,\n"
+                "// 2^20 inverted mod prime.\n",
+                "0x%016xULL", LOG_MAX_DEGREE)
     primes, pn, roots = generate(291, bits=30, two_n=1 << 16, word=32)
     write_table(os.path.join(here, "params32_table.inc"), "YA32", 291, primes, pn, roots,
                 "// Generated by gen_params.py: 291 primes 2^30 - k*2^16 + 1, descending.\n"
-                "// Columns: prime, low32(floor(2^64/prime)), primitive 2^16-th root.\n",
-                "0x%08xu")
+                "// Columns: prime, low32(floor(2^64/prime)), primitive 2^16-th root,\n"
+                "// 2^15 inverted mod prime.\n",
+                "0x%08xu", 15)
 
 
 if __name__ == "__main__":

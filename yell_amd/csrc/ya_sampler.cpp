@@ -47,31 +47,39 @@ struct Xoshiro {
 
 }  // namespace
 
-extern "C" int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed) {
-  int logn;
-  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
-  if (rc) return rc;
+namespace {
+
+// Shape check of either build: degree a power of two in [8, 2^maxlog],
+// nmoduli in [1, nmax].
+int sample_shape(size_t degree, size_t nmoduli, int maxlog, size_t nmax) {
+  if (degree < 8 || (degree & (degree - 1)) || degree > ((size_t)1 << maxlog))
+    return ya::fail(YA_ERR_INVALID, "degree %zu: a power of two in [8, 2^%d] is required", degree, maxlog);
+  if (nmoduli == 0 || nmoduli > nmax) return ya::fail(YA_ERR_INVALID, "nmoduli %zu out of [1, %zu]", nmoduli, nmax);
+  return YA_OK;
+}
+
+// include/yell/details/poly.hpp:372-396: per residue, a random word masked to
+// the prime's bit length (kModulusBitsize), minus p once if >= p.
+template <class W, class PF>
+int sample_uniform(W *out, size_t degree, size_t nmoduli, uint64_t seed, PF prime) {
   if (!out) return ya::fail(YA_ERR_INVALID, "null output");
   Xoshiro g(seed);
   for (size_t cm = 0; cm < nmoduli; ++cm) {
-    const uint64_t p = ya::prime(cm);
+    const uint64_t p = prime(cm);
     const int top = 63 - __builtin_clzll(p);  // floor(log2 p)
     const uint64_t mask = (top >= 63) ? ~0ULL : ((1ULL << (top + 1)) - 1);
-    uint64_t *dst = out + cm * degree;
+    W *dst = out + cm * degree;
     for (size_t i = 0; i < degree; ++i) {
       uint64_t v = g.next() & mask;
       if (v >= p) v -= p;
-      dst[i] = v;
+      dst[i] = (W)v;
     }
   }
   return YA_OK;
 }
 
-extern "C" int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, double sigma,
-                                  uint64_t seed) {
-  int logn;
-  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
-  if (rc) return rc;
+template <class W, class PF>
+int sample_gaussian(W *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed, PF prime) {
   if (!out) return ya::fail(YA_ERR_INVALID, "null output");
   if (!(sigma > 0.0) || sigma > 1e6) return ya::fail(YA_ERR_INVALID, "sigma %g out of range", sigma);
   const int tail = (int)std::ceil(13.0 * sigma);
@@ -99,20 +107,21 @@ extern "C" int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, 
     r[i] = neg ? -(int64_t)k : (int64_t)k;
   }
   for (size_t cm = 0; cm < nmoduli; ++cm) {
-    const uint64_t p = ya::prime(cm);
-    uint64_t *dst = out + cm * degree;
-    for (size_t i = 0; i < degree; ++i) dst[i] = r[i] < 0 ? p + r[i] : (uint64_t)r[i];
+    const int64_t p = (int64_t)prime(cm);
+    W *dst = out + cm * degree;
+    for (size_t i = 0; i < degree; ++i) {
+      // reduce |r| < p first (sigma large relative to a 30-bit prime), then map r < 0 to p + r
+      const int64_t v = r[i] % p;
+      dst[i] = (W)(v < 0 ? p + v : v);
+    }
   }
   return YA_OK;
 }
 
 // poly::set(ZO_dist) (include/yell/details/poly.hpp:438-451): one random byte
 // r per coefficient shared by all residues; r <= rho ? ((r & 2) ? p-1 : 1) : 0.
-extern "C" int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32_t rho,
-                            uint64_t seed) {
-  int logn;
-  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
-  if (rc) return rc;
+template <class W, class PF>
+int sample_zo(W *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, PF prime) {
   if (!out || rho > 255) return ya::fail(YA_ERR_INVALID, "null output or rho > 255");
   Xoshiro g(seed);
   std::vector<uint8_t> r(degree);
@@ -121,7 +130,7 @@ extern "C" int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32
     for (size_t k = 0; k < 8 && i + k < degree; ++k) r[i + k] = (uint8_t)(w >> (8 * k));
   }
   for (size_t cm = 0; cm < nmoduli; ++cm) {
-    const uint64_t p = ya::prime(cm);
+    const W p = (W)prime(cm);
     for (size_t i = 0; i < degree; ++i)
       out[cm * degree + i] = r[i] <= rho ? ((r[i] & 2) ? p - 1 : 1) : 0;
   }
@@ -131,11 +140,8 @@ extern "C" int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32
 // poly::set(hwt_dist) (include/yell/details/poly.hpp:458-501): exactly `hwt`
 // positions chosen uniformly (reservoir sampling), each +1 or -1 with equal
 // probability, shared by all residues (-1 stored as p-1).
-extern "C" int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint32_t hwt,
-                             uint64_t seed) {
-  int logn;
-  int rc = ya::check_shape(degree, 0, nmoduli, &logn);
-  if (rc) return rc;
+template <class W, class PF>
+int sample_hwt(W *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed, PF prime) {
   if (!out || hwt == 0 || hwt > degree) return ya::fail(YA_ERR_INVALID, "hwt must be in [1, degree]");
   Xoshiro g(seed);
   std::vector<size_t> hit(hwt);
@@ -150,10 +156,59 @@ extern "C" int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint3
   std::vector<uint8_t> sign(hwt);
   for (size_t k = 0; k < hwt; ++k) sign[k] = (uint8_t)(g.next() >> 63);
   for (size_t cm = 0; cm < nmoduli; ++cm) {
-    const uint64_t p = ya::prime(cm);
-    uint64_t *dst = out + cm * degree;
+    const W p = (W)prime(cm);
+    W *dst = out + cm * degree;
     std::fill(dst, dst + degree, 0);
     for (size_t k = 0; k < hwt; ++k) dst[hit[k]] = sign[k] ? 1 : p - 1;
   }
   return YA_OK;
 }
+
+uint64_t prime64(size_t cm) { return ya::prime(cm); }
+uint64_t prime32(size_t cm) { return ya32_prime(cm); }
+
+int shape64(size_t degree, size_t nmoduli) {
+  int logn;
+  return ya::check_shape(degree, 0, nmoduli, &logn);
+}
+int shape32(size_t degree, size_t nmoduli) { return sample_shape(degree, nmoduli, 15, ya32_num_moduli()); }
+
+}  // namespace
+
+extern "C" {
+
+int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed) {
+  int rc = shape64(degree, nmoduli);
+  return rc ? rc : sample_uniform(out, degree, nmoduli, seed, prime64);
+}
+int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed) {
+  int rc = shape64(degree, nmoduli);
+  return rc ? rc : sample_gaussian(out, degree, nmoduli, sigma, seed, prime64);
+}
+int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed) {
+  int rc = shape64(degree, nmoduli);
+  return rc ? rc : sample_zo(out, degree, nmoduli, rho, seed, prime64);
+}
+int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed) {
+  int rc = shape64(degree, nmoduli);
+  return rc ? rc : sample_hwt(out, degree, nmoduli, hwt, seed, prime64);
+}
+
+int ya32_sample_uniform(uint32_t *out, size_t degree, size_t nmoduli, uint64_t seed) {
+  int rc = shape32(degree, nmoduli);
+  return rc ? rc : sample_uniform(out, degree, nmoduli, seed, prime32);
+}
+int ya32_sample_gaussian(uint32_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed) {
+  int rc = shape32(degree, nmoduli);
+  return rc ? rc : sample_gaussian(out, degree, nmoduli, sigma, seed, prime32);
+}
+int ya32_sample_zo(uint32_t *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed) {
+  int rc = shape32(degree, nmoduli);
+  return rc ? rc : sample_zo(out, degree, nmoduli, rho, seed, prime32);
+}
+int ya32_sample_hwt(uint32_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed) {
+  int rc = shape32(degree, nmoduli);
+  return rc ? rc : sample_hwt(out, degree, nmoduli, hwt, seed, prime32);
+}
+
+}  // extern "C"
