@@ -210,7 +210,10 @@ int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
 int ya_autotune(int logn_lo, int logn_hi, double *report);
 /* Read / set one entry of the shape table: kind 1 = fused kernel, 2 =
  * batched kernel, 0 = standalone NTT kernels; le = log2 of the register-pass
- * radix (3, 4, or 5 for kinds 0 and 1).  ya_tune_get returns le, or -1. */
+ * radix (3, 4, or 5 for kinds 0 and 1).  kind 3: the fused kernel's
+ * dual-forward form (both operands' forward transforms in one pass sequence
+ * with shared twiddles; radix 8 / 16 at 2^10 and up), value 0 or 1.
+ * ya_tune_get returns le (or the kind 3 flag), or -1. */
 int ya_tune_get(int kind, int logn);
 int ya_tune_set(int kind, int logn, int le);
 /* Split launches (no counterpart in the reference).  A poly-multiplication

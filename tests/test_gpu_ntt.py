@@ -175,22 +175,29 @@ def test_extreme_values(logn):
     np.testing.assert_array_equal(polymul(z, top1, n, L), z)
 
 
-@pytest.mark.parametrize("le", [3, 4, 5])
-def test_every_pass_shape_of_fused_and_standalone_kernels(le):
-    """Radix-8/16/32 register passes (autotuner choices) give the oracle's words."""
+@pytest.mark.parametrize("le,dual", [(3, 0), (4, 0), (5, 0), (3, 1), (4, 1)])
+def test_every_pass_shape_of_fused_and_standalone_kernels(le, dual):
+    """Radix-8/16/32 register passes and the dual-forward form of the fused
+    kernel (autotuner choices) give the oracle's words.  Split launches are
+    switched off so the one-workgroup-per-residue kernels run."""
     lib = device()
-    saved = {(k, lg): lib.ya_tune_get(k, lg) for k in (0, 1) for lg in (12, 13, 14)}
+    saved = {(k, lg): lib.ya_tune_get(k, lg) for k in (0, 1, 3) for lg in (10, 11, 12, 13, 14)}
+    capi.call("ya_split_policy", -1)
     try:
-        for lg in (12, 13, 14):
+        for lg in (10, 11, 12, 13, 14):
             if le == 3 and lg == 14:
+                continue
+            if le == 5 and lg < 12:
                 continue
             for k in (0, 1):
                 capi.call("ya_tune_set", k, lg, le)
-            n, L = 1 << lg, 2
+            capi.call("ya_tune_set", 3, lg, dual)
+            n, L = 1 << lg, 3
             rng = np.random.default_rng(500 + lg + le)
             a = rand_bound(rng, n, L, 1, P)
             b = rand_bound(rng, n, L, 1, P)
-            say("shape", le, lg)
+            a[:5] = [0, 1, P[0] - 1, P[0] - 1, 2]
+            say("shape", le, lg, "dual", dual)
             np.testing.assert_array_equal(polymul(a, b, n, L), orc.polymul(a, b, n, L))
             x = rand_bound(rng, n, L, 4, P)
             np.testing.assert_array_equal(run_inplace("ya_poly_forward_lazy", x, n, L),
@@ -199,5 +206,6 @@ def test_every_pass_shape_of_fused_and_standalone_kernels(le):
             np.testing.assert_array_equal(run_inplace("ya_poly_backward", y, n, L),
                                           orc.poly_backward(y, n, L))
     finally:
+        capi.call("ya_split_policy", 0)
         for (k, lg), v in saved.items():
             capi.call("ya_tune_set", k, lg, v)

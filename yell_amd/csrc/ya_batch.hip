@@ -46,15 +46,24 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
   }
   uint64_t *my_lds = lds + sub * LdsWords<LB>::value;
   uint64_t x[1 << LE], y[1 << LE];
-  fwd_block<LB, LE>(x, r.a, my_lds, d.fwd, LB, 0u, d.p, tl);
-  if (active) gmem_store<0, LE>(r.c, x, tl);
-  const uint32_t t2 = phase_v(tl);
-  fwd_block<LB, LE>(x, r.b, my_lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
-  if (active) {
-    gmem_load<0, LE>(r.c, y, t2);
-  } else {  // no residue: any defined words will do, nothing is stored
+  if constexpr (LB >= 10) {
+    // both forward transforms at once (shared twiddles, no parking in c); a
+    // sub-group without a residue transforms the first one and stores nothing
+    constexpr int W0 = FirstWindow<LB, LE>::value;
+    gmem_load<W0, LE>(r.a, x, tl);
+    gmem_load<W0, LE>(r.b, y, tl);
+    fwd_block_dual<LB, LE>(x, y, my_lds, d.fwd, LB, 0u, d.p, tl);
+  } else {
+    fwd_block<LB, LE>(x, r.a, my_lds, d.fwd, LB, 0u, d.p, tl);
+    if (active) gmem_store<0, LE>(r.c, x, tl);
+    const uint32_t t2 = phase_v(tl);
+    fwd_block<LB, LE>(x, r.b, my_lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+    if (active) {
+      gmem_load<0, LE>(r.c, y, t2);
+    } else {  // no residue: any defined words will do, nothing is stored
 #pragma unroll
-    for (int e = 0; e < (1 << LE); ++e) y[e] = x[e];
+      for (int e = 0; e < (1 << LE); ++e) y[e] = x[e];
+    }
   }
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
@@ -549,13 +558,20 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
 }
 
 extern "C" int ya_tune_get(int kind, int logn) {
-  if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN) return -1;
+  if (kind < 0 || kind > 3 || logn < 0 || logn > kMaxLogN) return -1;
+  if (kind == 3) return tuned_shape(logn, 1).dual;
   return tuned_shape(logn, kind).le;
 }
 
 extern "C" int ya_tune_set(int kind, int logn, int le) {
+  if (kind == 3 && logn >= 0 && logn <= kMaxLogN && (le == 0 || le == 1)) {
+    set_tuned_dual(logn, le);
+    return YA_OK;
+  }
   if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN || le < 3 || le > 5 || (kind == 2 && le == 5))
     return fail(YA_ERR_INVALID, "bad tune entry kind=%d logn=%d le=%d", kind, logn, le);
-  set_tuned_shape(logn, kind, NttShape{le});
+  NttShape sh;
+  sh.le = le;
+  set_tuned_shape(logn, kind, sh);
   return YA_OK;
 }

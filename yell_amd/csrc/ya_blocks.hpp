@@ -68,6 +68,37 @@ __device__ __forceinline__ void fwd_block(W (&x)[1 << LE], const typename Ident<
   });
 }
 
+// Forward transforms of two blocks of one prime at once (x and y preloaded
+// with gmem_load<FirstWindow<LB,LE>::value, LE>): per pass the layers of both
+// share their twiddles; the exchanges go through LDS one block after the
+// other.  On return x and y hold window [0, LE), as fwd_block leaves them.
+template <int LB, int LE, class W, class TW>
+__device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE], W *lds,
+                                               const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
+                                               typename Ident<W>::type p, uint32_t t) {
+  constexpr int NP = (LB + LE - 1) / LE;
+  static_for<0, NP>([&](auto PC) {
+    constexpr int P = decltype(PC)::value;
+    constexpr int HI = LB - P * LE;
+    constexpr int LO = HI - LE > 0 ? HI - LE : 0;
+    constexpr int WLO = LO;
+    const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
+    fwd_layers2<WLO, LE, HI, LO>(x, y, tw, nlog, gbase, p);
+    if constexpr (P + 1 < NP) {
+      constexpr int HI2 = LB - (P + 1) * LE;
+      constexpr int WLO2 = HI2 - LE > 0 ? HI2 - LE : 0;
+      __syncthreads();
+      lds_store<WLO, LE>(lds, x, t);
+      __syncthreads();
+      lds_load<WLO2, LE>(lds, x, t);
+      __syncthreads();
+      lds_store<WLO, LE>(lds, y, t);
+      __syncthreads();
+      lds_load<WLO2, LE>(lds, y, t);
+    }
+  });
+}
+
 // Inverse transform of index bits [0, LB) of one block starting from words
 // in registers (window [0, LE), i = e | t << LE, each < 2p), written to
 // global `dst` (block base).  TOP selects what happens at bit LB-1:

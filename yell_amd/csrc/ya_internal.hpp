@@ -37,6 +37,8 @@ uint64_t prime(size_t cm);
 // Tuned launch shape for a degree class.  kind 0: standalone NTT kernels,
 // kind 1: fused poly-multiplication, kind 2: batched poly-multiplication.
 NttShape tuned_shape(int logn, int kind);
-void set_tuned_shape(int logn, int kind, NttShape s);
+void set_tuned_shape(int logn, int kind, NttShape s);  // sets le only
+// Fused poly-multiplication of degree 2^logn: dual-forward form on/off.
+void set_tuned_dual(int logn, int dual);
 
 }  // namespace ya

@@ -18,6 +18,7 @@ constexpr int kMaxLogN = 17;
 // Elements per thread (log2) chosen for a block of 2^lb words; 0 = default.
 struct NttShape {
   int le = 4;
+  int dual = 0;  // fused poly-multiplication: both forward transforms in one pass sequence
 };
 
 // forward / forward_lazy / backward over `nres` residues of 2^logn words,

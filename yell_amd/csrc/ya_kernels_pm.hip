@@ -7,12 +7,18 @@ namespace ya {
 // Fused c = backward(forward(a) (*) forward(b)) of one residue of 2^LB words
 // by one workgroup.  forward(a) is parked in c at the positions this thread
 // later reads back itself, so no other buffer is needed.
-template <int LB, int LE>
+template <int LB, int LE, bool DUAL = false>
 __device__ __forceinline__ void polymul_residue(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
                                                 const uint64_t *__restrict__ b, const ModDesc &d,
                                                 uint64_t *lds, uint32_t t) {
   uint64_t x[1 << LE], y[1 << LE];
-  if constexpr (LE <= 4) {
+  if constexpr (DUAL) {
+    // both forward transforms at once (shared twiddles, no parking in c)
+    constexpr int W0 = FirstWindow<LB, LE>::value;
+    gmem_load<W0, LE>(a, x, t);
+    gmem_load<W0, LE>(b, y, t);
+    fwd_block_dual<LB, LE>(x, y, lds, d.fwd, LB, 0u, d.p, t);
+  } else if constexpr (LE <= 4) {
     // both operands are requested at once (b waits in y during forward(a)),
     // so the second transform does not stall on HBM latency again
     constexpr int W0 = FirstWindow<LB, LE>::value;
@@ -41,7 +47,7 @@ __device__ __forceinline__ void polymul_residue(uint64_t *__restrict__ c, const 
 // Residue-major polynomials: residue res of the launch uses descs[res % nmod];
 // the XCD ranges cover the residues in prime-major order, so an XCD's L2
 // holds the twiddles of few primes.
-template <int LB, int LE>
+template <int LB, int LE, bool DUAL>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
           const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod) {
@@ -53,34 +59,53 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
   const uint32_t res = poly * nmod + cm;
   const ModDesc d = descs[cm];
   const size_t off = (size_t)res << LB;
-  polymul_residue<LB, LE>(c + off, a + off, b + off, d, lds, threadIdx.x);
+  polymul_residue<LB, LE, DUAL>(c + off, a + off, b + off, d, lds, threadIdx.x);
 }
 
 // One residue per workgroup from a residue table (a batch plan of one degree
 // class); the table is ordered by prime, and the XCD ranges follow it.
-template <int LB, int LE>
+template <int LB, int LE, bool DUAL>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_polymul_tab(const BatchRes *__restrict__ res, uint32_t nres) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
   const uint32_t v = xcd_order(blockIdx.x, nres);
   const BatchRes r = res[v];
   const ModDesc d = load_desc(r.d);
-  polymul_residue<LB, LE>(r.c, r.a, r.b, d, lds, threadIdx.x);
+  polymul_residue<LB, LE, DUAL>(r.c, r.a, r.b, d, lds, threadIdx.x);
 }
 
 namespace {
 
+// The dual-forward form is instantiated for the shapes where it can pay
+// (blocks of 2^10 and more, radix 8 / 16 passes).
+template <int LB, int LE>
+constexpr bool kDualShape = LB >= 10 && LE <= 4;
+
 template <int LB, int LE>
 hipError_t polymul_lb(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
-                      size_t nres, size_t nmod, hipStream_t st) {
-  hipLaunchKernelGGL((k_polymul<LB, LE>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c, a,
+                      size_t nres, size_t nmod, bool dual, hipStream_t st) {
+  if constexpr (kDualShape<LB, LE>) {
+    if (dual) {
+      hipLaunchKernelGGL((k_polymul<LB, LE, true>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c,
+                         a, b, descs, (uint32_t)nmod);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_polymul<LB, LE, false>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c, a,
                      b, descs, (uint32_t)nmod);
   return hipGetLastError();
 }
 
 template <int LB, int LE>
-hipError_t polymul_tab_lb(const BatchRes *res, size_t nres, hipStream_t st) {
-  hipLaunchKernelGGL((k_polymul_tab<LB, LE>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, res,
+hipError_t polymul_tab_lb(const BatchRes *res, size_t nres, bool dual, hipStream_t st) {
+  if constexpr (kDualShape<LB, LE>) {
+    if (dual) {
+      hipLaunchKernelGGL((k_polymul_tab<LB, LE, true>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st,
+                         res, (uint32_t)nres);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_polymul_tab<LB, LE, false>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, res,
                      (uint32_t)nres);
   return hipGetLastError();
 }
@@ -92,7 +117,7 @@ hipError_t launch_polymul_table(const BatchRes *res, size_t nres, int logn, NttS
   if (logn < kMinLogN || logn > kMaxBlockLog || nres == 0 || nres > 0x7fffffff)
     return hipErrorInvalidValue;
   const int le = clamp_le(logn, shape.le);
-#define YA_PMT_CALL(LB, LE) polymul_tab_lb<LB, LE>(res, nres, st)
+#define YA_PMT_CALL(LB, LE) polymul_tab_lb<LB, LE>(res, nres, shape.dual != 0, st)
   YA_LB_SWITCH(logn, le, YA_PMT_CALL)
 #undef YA_PMT_CALL
 }
@@ -103,7 +128,7 @@ hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, con
     return hipErrorInvalidValue;
   if (logn > kMaxBlockLog) return hipErrorNotSupported;  // composed by the caller
   const int le = clamp_le(logn, shape.le);
-#define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, nmod, st)
+#define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, nmod, shape.dual != 0, st)
   YA_LB_SWITCH(logn, le, YA_PM_CALL)
 #undef YA_PM_CALL
 }

@@ -196,6 +196,30 @@ __device__ __forceinline__ void fwd_layers(W (&x)[1 << LE], const TW *__restrict
   }
 }
 
+// The same forward layers applied to two residues of one prime at once (the
+// two operands of a poly-multiplication): each twiddle is loaded once and
+// used for both, and the two butterfly chains are independent.
+template <int WLO, int LE, int HI, int LO, class W, class TW>
+__device__ __forceinline__ void fwd_layers2(W (&x)[1 << LE], W (&y)[1 << LE], const TW *__restrict__ tw,
+                                            uint32_t nlog, uint32_t gbase, typename Ident<W>::type p) {
+  const typename WordK<W>::K k(p);
+#pragma unroll
+  for (int b = HI - 1; b >= LO; --b) {
+    const int j = b - WLO;
+    const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
+#pragma unroll
+    for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
+      const TW t = load_tw(tw + toff + g);
+#pragma unroll
+      for (int q = 0; q < (1 << j); ++q) {
+        const int e0 = (g << (j + 1)) | q;
+        ct_butterfly(x[e0], x[e0 | (1 << j)], t, k);
+        ct_butterfly(y[e0], y[e0 | (1 << j)], t, k);
+      }
+    }
+  }
+}
+
 // Inverse (GS) layers b = LO ... HI-1 of one pass (same indexing).
 template <int WLO, int LE, int HI, int LO, class W, class TW>
 __device__ __forceinline__ void inv_layers(W (&x)[1 << LE], const TW *__restrict__ tw, uint32_t nlog,
