@@ -185,7 +185,7 @@ int ya_batch_plan_run_graph(ya_batch_plan *plan, void *stream);
 /* Launch forms: a batch of one degree class <= 2^14 holding at least two
  * residues per CU runs as a grid of the fused kernel over the plan's residue
  * table (YA_BATCH_GRID); a batch of one class 2^11 .. 2^14 too small to
- * occupy a quarter of the CUs spreads each residue over several workgroups
+ * occupy half of the CUs spreads each residue over several workgroups
  * (YA_BATCH_SPLIT, see ya_split_policy; the plan holds a scratch copy of the
  * b operands' partial transforms); any other batch drains the persistent
  * kernel's queue (YA_BATCH_PERSISTENT).  All give the same words. */
@@ -221,7 +221,7 @@ int ya_tune_set(int kind, int logn, int le);
  * served form's small batches) may spread each residue over 2^k workgroups
  * (column kernels for the top k index bits, block kernels for the rest)
  * instead of one, for latency.  mode 0: automatic (when one workgroup per
- * residue would occupy under a quarter of the CUs); -1: never; k in 2..5:
+ * residue would occupy at most half of the CUs); -1: never; k in 2..5:
  * always, with that k, where 2^logn / 2^k >= 2^9.  Results are identical. */
 int ya_split_policy(int mode);
 

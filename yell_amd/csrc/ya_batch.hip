@@ -532,14 +532,20 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
       if (le <= 4) {  // the batched kernel's 1024-thread tiles take radix 8 or 16
         set_tuned_shape(lg, 2, NttShape{le});
         ya_batch_plan *pl = nullptr;
-        if ((rc = ya_batch_plan_create(&pl, jobs.data(), jobs.size()))) break;
+        // time the persistent kernel itself (this uniform batch would
+        // otherwise run in the grid form, which uses the fused shape)
+        const int saved_form = g_form_override.exchange(YA_BATCH_PERSISTENT);
+        rc = ya_batch_plan_create(&pl, jobs.data(), jobs.size());
+        g_form_override.store(saved_form);
+        if (rc) break;
         rc = time_ms(s, 3, &ms, [&]() { return ya_batch_plan_run(pl, s); });
         plan_free(pl);
         if (rep) rep[le - 3] = ms;
         if (ms < best[0]) best[0] = ms, arg[0] = le;
       }
       if (lg <= 14 && rc == YA_OK) {
-        const NttShape sh{le};
+        NttShape sh{le};
+        sh.dual = le <= 4 ? tuned_shape(lg, 1).dual : 0;  // the form that runs
         rc = time_ms(s, 3, &ms, [&]() {
           hipError_t ee = launch_polymul(c, a, b, desc_ptr(st, lg, 0), lg, njobs * L, L, sh, s);
           return ee == hipSuccess ? YA_OK : hip_fail(ee, "tune fused launch");

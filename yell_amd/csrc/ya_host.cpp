@@ -143,8 +143,9 @@ int split_factor(DeviceState *st, int logn, size_t nres) {
   if (mode < 0 || logn < kSplitMinLog || logn > kSplitMaxLog) return 0;
   if (mode >= kSplitMinK)
     return (mode <= kSplitMaxK && logn - mode >= kSplitMinS) ? mode : 0;
-  // automatic: when one workgroup per residue would leave most CUs idle
-  if (nres * 4 > (size_t)st->cus) return 0;
+  // automatic: when one workgroup per residue would leave at least half of
+  // the CUs idle (its latency is one CU transforming a whole residue)
+  if (nres * 2 > (size_t)st->cus) return 0;
   return split_choose_k(logn, nres, st->cus);
 }
 

@@ -99,17 +99,10 @@ k_split_blocks(SplitArgs s, uint32_t nlog) {
   constexpr int W0 = FirstWindow<S, LE>::value;
   gmem_load<W0, LE>(cj, x, t);
   gmem_load<W0, LE>(tj, y, t);
-  fwd_block<S, LE, true>(x, nullptr, lds, d.fwd, nlog, boff, d.p, t);
-  // park forward(a) at the words this thread reads back itself; every load of
-  // cj above completed before the first LDS exchange of fwd_block
-  gmem_store<0, LE>(cj, x, t);
+  // the remaining forward layers of both blocks at once (shared twiddles)
+  fwd_block_dual<S, LE>(x, y, lds, d.fwd, nlog, boff, d.p, t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) x[e] = y[e];
-  const uint32_t t2 = phase_v(t);
-  fwd_block<S, LE, true>(x, nullptr, lds, d.fwd + phase_zero_s(), nlog, boff, d.p, t2);
-  gmem_load<0, LE>(cj, y, t2);
-#pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+  for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(x[e], y[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);
   inv_block<S, LE, TOP_GS>(x, cj, lds, d.inv, nlog, boff, d, t3);
 }
