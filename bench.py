@@ -30,9 +30,9 @@ METRIC = "bit-exact poly-mults/s at N=2^14,L=8 batched vs plain; p50/p99 served 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes one poly-multiplication of the batched step moves, from the
 # FETCH_SIZE / WRITE_SIZE counter passes (calibrated against a 256 MiB copy)
-# in profiles/k_polymul_n16384_l8_trace_and_traffic.md: 1.714e9 B read +
-# 1.074e9 B written per launch of 512 poly-mults.
-TRAFFIC_BYTES_PER_POLYMUL = 5.444e6
+# in profiles/k_polymul_tab_dual_n16384_l8.md: 1.1535e9 B read + 5.369e8 B
+# written per launch of 512 poly-mults.
+TRAFFIC_BYTES_PER_POLYMUL = 3301576.0
 
 
 def log(*a):
@@ -470,7 +470,7 @@ def main():
                       "kernel": "k_polymul<14,4>, one workgroup per residue"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_POLYMUL * B,
-                     "traffic_source": "profiles/k_polymul_n16384_l8_trace_and_traffic.md",
+                     "traffic_source": "profiles/k_polymul_tab_dual_n16384_l8.md",
                      "kernel": "k_polymul_tab<14,4>" if form.value == 2 else "k_batch_polymul",
                      "bytes_per_polymul": bytes_per,
                      "polymuls_per_launch": B},
