@@ -16,8 +16,8 @@
 // window of bits [WLO, WLO + LE); the remaining index bits come from the
 // thread number t: the low WLO bits of i are the low WLO bits of t, the bits
 // above the window are t >> WLO.  Between passes the words are exchanged
-// through LDS with a padded layout (one spare word per 16) so that strided
-// windows do not all land in the same banks.
+// through LDS with a bank-spreading layout (lds_slot) so that strided windows
+// do not all land in the same banks.
 #pragma once
 
 #include "ya_arith.hpp"
@@ -51,6 +51,19 @@ struct ModDesc32 {
 };
 
 __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
+
+// LDS slot of word i in an exchange buffer.  64-bit words: an XOR swizzle of
+// the low five index bits with the five above bit 3 (radix-16 windows) or
+// bit 4 (radix-32), which spreads every exchange window of a pass over all
+// 64 four-byte banks (the padded layout left 2-way conflicts on the windows
+// at bits 6, 8 and 10, a quarter of the LDS cycles).  It only permutes index
+// bits 0..4 within a block, so it stays inside [0, 2^LB).  32-bit words keep
+// the one-word-in-16 padding.
+template <int LE, class W>
+__device__ __forceinline__ uint32_t lds_slot(uint32_t i) {
+  if constexpr (sizeof(W) == 8) return i ^ ((i >> (LE >= 5 ? 5 : 4)) & 31u);
+  return lds_pad(i);
+}
 
 // Twiddle load through a global-address-space pointer (the descriptor's
 // pointers are generic; without the cast the loads become FLAT and also
@@ -110,14 +123,14 @@ template <int WLO, int LE, class W>
 __device__ __forceinline__ void lds_store(W *s, const W (&x)[1 << LE], uint32_t t) {
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) s[lds_pad(base | ((uint32_t)e << WLO))] = x[e];
+  for (int e = 0; e < (1 << LE); ++e) s[lds_slot<LE, W>(base | ((uint32_t)e << WLO))] = x[e];
 }
 
 template <int WLO, int LE, class W>
 __device__ __forceinline__ void lds_load(const W *s, W (&x)[1 << LE], uint32_t t) {
   const uint32_t base = win_base<WLO, LE>(t);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) x[e] = s[lds_pad(base | ((uint32_t)e << WLO))];
+  for (int e = 0; e < (1 << LE); ++e) x[e] = s[lds_slot<LE, W>(base | ((uint32_t)e << WLO))];
 }
 
 // Global-memory word accesses of a window.  The pointers are cast to the
