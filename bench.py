@@ -329,6 +329,123 @@ def crt_cpu_reference(crt):
             "bit_exact": bool(np.array_equal(want, got))}
 
 
+def sampler_bench(stream, ev, reps=20):
+    """Device samplers drawing a C2 operand pair straight into HBM (a uniform,
+    b Gaussian sigma = 3.2; N = 2^14, L = 8) against the host samplers plus
+    the upload they replace; ZO and hwt at the RLWE flow's shape."""
+    n, L = 1 << 14, 8
+    da, db = capi.DeviceBuffer(n * L), capi.DeviceBuffer(n * L)
+    out = {"degree": n, "nmoduli": L}
+    runs = {"uniform": lambda: capi.sample_device("uniform", da, n, L, None, 1, stream=stream),
+            "gaussian": lambda: capi.sample_device("gaussian", db, n, L, 3.2, 2, stream=stream)}
+    for name, fn in runs.items():
+        fn()
+        capi.call("ya_stream_synchronize", stream)
+        ms = timed(ev, stream, fn, reps)
+        out[name + "_us"] = ms * 1e3
+        out[name + "_hbm_write_GBps"] = n * L * 8 / (ms / 1e3) / 1e9
+    out["pair_us"] = out["uniform_us"] + out["gaussian_us"]
+    ha, hb = capi.sample_host("uniform", n, L, None, 1), capi.sample_host("gaussian", n, L, 3.2, 2)
+    out["device_equals_host"] = bool(np.array_equal(da.download(), ha) and np.array_equal(db.download(), hb))
+    t0 = time.perf_counter()
+    hreps = 3
+    for _ in range(hreps):
+        ha, hb = capi.sample_host("uniform", n, L, None, 1), capi.sample_host("gaussian", n, L, 3.2, 2)
+        da.upload(ha, stream)
+        db.upload(hb, stream)
+    capi.call("ya_stream_synchronize", stream)
+    out["host_sample_plus_upload_us"] = (time.perf_counter() - t0) / hreps * 1e6
+    n2, L2 = 8192, 4
+    dz = capi.DeviceBuffer(n2 * L2)
+    for name, kind, param in (("zo_n8192_l4", "zo", 0x7F), ("hwt64_n8192_l4", "hwt", 64)):
+        fn = lambda: capi.sample_device(kind, dz, n2, L2, param, 3, stream=stream)
+        fn()
+        capi.call("ya_stream_synchronize", stream)
+        out[name + "_us"] = timed(ev, stream, fn, reps) * 1e3
+    for d in (da, db, dz):
+        d.close()
+    return out
+
+
+def rlwe_bench(stream, ev, batch=64, reps=5):
+    """The reference example's RLWE flow (example.cpp:12-58) at its own
+    parameters (N = 8192, L = 4, hwt 64, sigma 3.2): key generation once
+    (eager), then a step of `batch` encryptions + decryptions under that key,
+    captured as one graph and replayed; samples from the device samplers."""
+    from yell_amd.rlwe import Rlwe
+    n, L = 8192, 4
+    f = Rlwe(n, L, stream=stream)
+    sx, bx, ax = f.buffer(), f.buffer(), f.buffer()
+    f.keygen(sx, bx, ax, (1, 2, 3))  # first use: tables, Gaussian table
+    capi.call("ya_stream_synchronize", stream)
+    capi.call("ya_event_record", ev.a, stream)
+    f.keygen(sx, bx, ax, (1, 2, 3))
+    capi.call("ya_event_record", ev.b, stream)
+    keygen_us = ev.elapsed_ms() * 1e3
+    msgs = [f.buffer() for _ in range(batch)]
+    for k, m in enumerate(msgs):
+        capi.sample_device("uniform", m, n, L, None, 1000 + k, stream=stream)
+    work = [(f.buffer(), f.buffer(), f.buffer(), f.buffer()) for _ in range(batch)]
+
+    def step():
+        for k, (cb, ca, u, pl) in enumerate(work):
+            f.encrypt(cb, ca, u, msgs[k], bx, ax, (10 + 3 * k, 11 + 3 * k, 12 + 3 * k))
+            f.decrypt(pl, cb, ca, sx)
+
+    step()
+    capi.call("ya_stream_synchronize", stream)
+    eager_ms = timed(ev, stream, step, 1)
+    capi.call("ya_graph_begin", stream)
+    step()
+    g = ctypes.c_void_p()
+    capi.call("ya_graph_end", stream, ctypes.byref(g))
+    capi.call("ya_graph_launch", g.value, stream)
+    capi.call("ya_stream_synchronize", stream)
+    ms = timed(ev, stream, lambda: capi.call("ya_graph_launch", g.value, stream), reps)
+    capi.call("ya_graph_destroy", g.value)
+    first = {"plain0": work[0][3].download(), "ct0": (work[0][0].download(), work[0][1].download()),
+             "sx": sx.download(), "pk": (bx.download(), ax.download())}
+    for b in [sx, bx, ax] + msgs + [x for w in work for x in w]:
+        b.close()
+    return {"degree": n, "nmoduli": L, "hwt": 64, "sigma": 3.2, "batch": batch, "keygen_us": keygen_us,
+            "eager_step_ms": eager_ms, "graph_step_ms": ms, "value": batch / (ms / 1e3),
+            "unit": "encrypt+decrypt/s", "launches_per_encrypt_decrypt": 12, "_first": first}
+
+
+def rlwe_cpu_reference(rl, seconds=3.0):
+    """The same flow through the reference's own sources (oracle/_ref) on the
+    samples of the device run (host samplers, same seeds): bit-exactness of
+    message 0's key, ciphertext and decryption, then encrypt + decrypt rounds
+    timed on one thread and on all threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle  # checker only
+    first = rl.pop("_first")
+    n, L = rl["degree"], rl["nmoduli"]
+    h = lambda kind, param, s: capi.sample_host(kind, n, L, param, s, 64)
+    sx, e0, ax = h("hwt", 64, 1), h("gaussian", 3.2, 2), h("uniform", None, 3)
+    msg, u, e1, e2 = h("uniform", None, 1000), h("zo", 0x7F, 10), h("gaussian", 3.2, 11), h("gaussian", 3.2, 12)
+    try:
+        ref = _oracle.RefLib(64)
+    except RuntimeError:
+        orc = _oracle.Oracle(64)
+        got = _oracle.rlwe_oracle().flow(orc, n, L, sx, e0, ax, msg, u, e1, e2)
+        return {"kind": "port", "bit_exact": bool(np.array_equal(got[4], first["plain0"]))}
+    sxf, pk_bx, ct_bx, ct_ax, plain = ref.rlwe(n, L, sx, e0, ax, msg, u, e1, e2)
+    exact = all(np.array_equal(x, y) for x, y in ((sxf, first["sx"]), (pk_bx, first["pk"][0]),
+                                                   (ax, first["pk"][1]), (ct_bx, first["ct0"][0]),
+                                                   (ct_ax, first["ct0"][1]), (plain, first["plain0"])))
+    t1 = ref.bench_rlwe(n, L, sxf, pk_bx, ax, msg, u, e1, e2, 1, 2) / 2
+    reps = max(1, int(seconds / max(t1, 1e-6)))
+    s1 = ref.bench_rlwe(n, L, sxf, pk_bx, ax, msg, u, e1, e2, 1, reps)
+    th = min(16, os.cpu_count() or 1)
+    reps_t = max(1, int(seconds / max(t1, 1e-6)))
+    st = ref.bench_rlwe(n, L, sxf, pk_bx, ax, msg, u, e1, e2, th, reps_t)
+    return {"kind": "reference", "bit_exact": bool(exact), "unit": "encrypt+decrypt/s",
+            "value_1core": reps / s1, "value": th * reps_t / st, "cores": th,
+            "sample": "%d encrypt+decrypt rounds on 1 thread, then %d on each of %d threads, of example.cpp's "
+                      "flow (N=%d, L=%d) through the reference's sources" % (reps, reps_t, th, n, L)}
+
+
 def served_bench(rates, duration, shapes):
     """Poisson stream of host-operand requests (shape drawn from `shapes`):
     continuous batching vs one request per launch, same seeded stream."""
@@ -494,6 +611,10 @@ def main():
         out["single_polymul_latency"] = latency_bench(stream, ev, n, L)
         log("32-bit moduli build")
         out["u32_moduli"] = u32_bench(stream, ev)
+        log("device samplers")
+        out["device_samplers"] = sampler_bench(stream, ev)
+        log("RLWE flow")
+        out["rlwe_flow"] = rlwe_bench(stream, ev)
         sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
         out["served"] = sv
         for r in sv["runs"]:
@@ -514,11 +635,15 @@ def main():
         if "u32_moduli" in out:
             log("32-bit cpu reference")
             out["u32_moduli"]["cpu_reference"] = u32_cpu_reference(out["u32_moduli"])
+        if "rlwe_flow" in out:
+            log("RLWE cpu reference")
+            out["rlwe_flow"]["cpu_reference"] = rlwe_cpu_reference(out["rlwe_flow"])
     if "crt_lift" in out:
         out["crt_lift"].pop("_c2_in", None)
         out["crt_lift"].pop("_c2_out", None)
     for k in ("_a0", "_b0", "_c0"):
         out.get("u32_moduli", {}).pop(k, None)
+    out.get("rlwe_flow", {}).pop("_first", None)
     capi.load().ya_batch_plan_destroy(plan)
     print(json.dumps(out), flush=True)
 

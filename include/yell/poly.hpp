@@ -60,17 +60,30 @@ YELL_AMD_API3(api_poly_sub, ya_poly_sub, ya32_poly_sub)
 YELL_AMD_API3(api_poly_add_product_of, ya_poly_add_product_of, ya32_poly_add_product_of)
 YELL_AMD_API3(api_poly_multiply, ya_poly_multiply, ya32_poly_multiply)
 #undef YELL_AMD_API3
-inline int api_sample_uniform(uint64_t *h, size_t n, size_t l, uint64_t s) { return ya_sample_uniform(h, n, l, s); }
-inline int api_sample_uniform(uint32_t *h, size_t n, size_t l, uint64_t s) { return ya32_sample_uniform(h, n, l, s); }
-inline int api_sample_zo(uint64_t *h, size_t n, size_t l, uint32_t r, uint64_t s) { return ya_sample_zo(h, n, l, r, s); }
-inline int api_sample_zo(uint32_t *h, size_t n, size_t l, uint32_t r, uint64_t s) { return ya32_sample_zo(h, n, l, r, s); }
-inline int api_sample_hwt(uint64_t *h, size_t n, size_t l, uint32_t w, uint64_t s) { return ya_sample_hwt(h, n, l, w, s); }
-inline int api_sample_hwt(uint32_t *h, size_t n, size_t l, uint32_t w, uint64_t s) { return ya32_sample_hwt(h, n, l, w, s); }
-inline int api_sample_gaussian(uint64_t *h, size_t n, size_t l, double g, uint64_t s) {
-  return ya_sample_gaussian(h, n, l, g, s);
+// Device samplers (same seeded streams as the host samplers ya_sample_*).
+inline int api_sample_uniform(uint64_t *d, size_t n, size_t l, uint64_t s) {
+  return ya_sample_uniform_device(d, n, l, s, nullptr);
 }
-inline int api_sample_gaussian(uint32_t *h, size_t n, size_t l, double g, uint64_t s) {
-  return ya32_sample_gaussian(h, n, l, g, s);
+inline int api_sample_uniform(uint32_t *d, size_t n, size_t l, uint64_t s) {
+  return ya32_sample_uniform_device(d, n, l, s, nullptr);
+}
+inline int api_sample_zo(uint64_t *d, size_t n, size_t l, uint32_t r, uint64_t s) {
+  return ya_sample_zo_device(d, n, l, r, s, nullptr);
+}
+inline int api_sample_zo(uint32_t *d, size_t n, size_t l, uint32_t r, uint64_t s) {
+  return ya32_sample_zo_device(d, n, l, r, s, nullptr);
+}
+inline int api_sample_hwt(uint64_t *d, size_t n, size_t l, uint32_t w, uint64_t s) {
+  return ya_sample_hwt_device(d, n, l, w, s, nullptr);
+}
+inline int api_sample_hwt(uint32_t *d, size_t n, size_t l, uint32_t w, uint64_t s) {
+  return ya32_sample_hwt_device(d, n, l, w, s, nullptr);
+}
+inline int api_sample_gaussian(uint64_t *d, size_t n, size_t l, double g, uint64_t a, uint64_t s) {
+  return ya_sample_gaussian_device(d, n, l, g, a, s, nullptr);
+}
+inline int api_sample_gaussian(uint32_t *d, size_t n, size_t l, double g, uint64_t a, uint64_t s) {
+  return ya32_sample_gaussian_device(d, n, l, g, a, s, nullptr);
 }
 }  // namespace detail
 
@@ -138,31 +151,22 @@ class poly {
     std::swap(a.nmoduli_, b.nmoduli_);
   }
 
-  // ---- samplers (details/poly.hpp:372-501), drawn on the host, uploaded.
+  // ---- samplers (details/poly.hpp:200-330), drawn on the device into the
+  // residues (seeded; the host samplers ya_sample_* give the same words).
   void set(const uniform &) {
-    sample([&](value_type *h, uint64_t s) { return detail::api_sample_uniform(h, degree, nmoduli_, s); });
+    sample([&](uint64_t s) { return detail::api_sample_uniform(d_, degree, nmoduli_, s); });
   }
   void set(const ZO_dist &m) {
-    sample([&](value_type *h, uint64_t s) { return detail::api_sample_zo(h, degree, nmoduli_, m.rho, s); });
+    sample([&](uint64_t s) { return detail::api_sample_zo(d_, degree, nmoduli_, m.rho, s); });
   }
   void set(const hwt_dist &m) {
-    sample([&](value_type *h, uint64_t s) { return detail::api_sample_hwt(h, degree, nmoduli_, m.hwt, s); });
+    sample([&](uint64_t s) { return detail::api_sample_hwt(d_, degree, nmoduli_, m.hwt, s); });
   }
+  // r *= amplifier on the signed sample (details/poly.hpp:248-251), then r < 0 ? p + r : r.
   void set(const gaussian &m) {
-    std::vector<value_type> h(degree * nmoduli_);
-    ya_check(detail::api_sample_gaussian(h.data(), degree, nmoduli_, m.sigma, next_seed()), "gaussian sampler");
-    if (m.amplifier != 1) {  // r *= amplifier on the signed sample (details/poly.hpp:419-422)
-      for (size_t cm = 0; cm < nmoduli_; ++cm) {
-        const value_type p = params::P[cm];
-        for (size_t i = 0; i < degree; ++i) {
-          value_type &v = h[cm * degree + i];
-          const signed_type r = v > p / 2 ? -(signed_type)(p - v) : (signed_type)v;
-          const signed_type a = r * (signed_type)m.amplifier;
-          v = a < 0 ? p + a : (value_type)a;
-        }
-      }
-    }
-    from_host(h);
+    sample([&](uint64_t s) {
+      return detail::api_sample_gaussian(d_, degree, nmoduli_, m.sigma, (uint64_t)m.amplifier, s);
+    });
   }
 
   // ---- access
@@ -235,12 +239,12 @@ class poly {
   bool operator==(const poly &o) const { return nmoduli_ == o.nmoduli_ && to_host() == o.to_host(); }
   bool operator!=(const poly &o) const { return !(*this == o); }
 
-  // ---- NTT (details/poly.hpp:341-357)
+  // ---- NTT (details/poly.hpp:170-186)
   void forward() { if (nmoduli_) ya_check(detail::api_poly_forward(d_, degree, nmoduli_), "forward"); }
   void forward_lazy() { if (nmoduli_) ya_check(detail::api_poly_forward_lazy(d_, degree, nmoduli_), "forward_lazy"); }
   void backward() { if (nmoduli_) ya_check(detail::api_poly_backward(d_, degree, nmoduli_), "backward"); }
 
-  // ---- arithmetic (details/poly.hpp:289-328)
+  // ---- arithmetic (details/poly.hpp:118-158)
   poly &add_product_of(const poly &a, const poly &b) {
     same(a);
     same(b);
@@ -248,7 +252,7 @@ class poly {
     return *this;
   }
   void negate() { if (nmoduli_) ya_check(detail::api_poly_negate(d_, degree, nmoduli_), "negate"); }
-  // a * s computed as a * r + a * (s - r) with a fresh uniform r (details/poly.hpp:305-320).
+  // a * s computed as a * r + a * (s - r) with a fresh uniform r (details/poly.hpp:134-150).
   poly &mul_sensitive_poly(const poly &s) {
     same(s);
     poly r0(nmoduli_, uniform{});
@@ -287,9 +291,7 @@ class poly {
  private:
   template <class F>
   void sample(F f) {
-    std::vector<value_type> h(degree * nmoduli_);
-    if (nmoduli_) ya_check(f(h.data(), next_seed()), "sampler");
-    from_host(h);
+    if (nmoduli_) ya_check(f(next_seed()), "sampler");
   }
   void copy_from(const poly &o) {
     if (nmoduli_) ya_check(ya_memcpy_dtod(d_, o.d_, bytes(), nullptr), "copy");

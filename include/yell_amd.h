@@ -96,6 +96,18 @@ int ya_event_destroy(void *event);
 int ya_event_record(void *event, void *stream);
 int ya_event_synchronize(void *event);
 int ya_event_elapsed_ms(float *ms, void *start, void *end);
+/* Stream capture (no counterpart in the reference, which has no device
+ * streams): every entry point called on `stream` between ya_graph_begin and
+ * ya_graph_end is recorded, not run, and *graph replays the sequence with one
+ * ya_graph_launch.  Call each captured entry point once beforehand (tables,
+ * Gaussian tables and device state are built on first use, which capture
+ * forbids).  Entry points that allocate or clear memory -- ya_memset_zero,
+ * ya_sample_hwt_device, ya_poly_multiply / _many when they take the split or
+ * large-ring path -- are not to be captured. */
+int ya_graph_begin(void *stream);
+int ya_graph_end(void *stream, void **graph);
+int ya_graph_launch(void *graph, void *stream);
+int ya_graph_destroy(void *graph);
 
 /* ---- one residue (include/yell/details/ntt.hpp:106-166) ----------------- */
 /* ntt<N>::forward(op, cm) / forward_lazy(op, cm) / backward(op, cm), in place
@@ -288,20 +300,51 @@ typedef struct ya_serve_report {
 } ya_serve_report;
 int ya_server_run_poisson(ya_server *server, const ya_poisson_spec *spec, ya_serve_report *report);
 
-/* ---- host samplers (input source; details/poly.hpp:201-260) ------------- */
+/* ---- host samplers (input source; details/poly.hpp:200-330) ------------- */
 /* Seeded replacements of poly::set(uniform) and poly::set(gaussian): fill a
- * HOST residue-major buffer.  uniform: per residue, 64-bit word & (2^62-1),
- * minus p once if >= p.  gaussian: one centred discrete Gaussian sample per
- * coefficient (parameter sigma: density proportional to exp(-x^2/(2 sigma^2))),
- * shared by all residues and mapped r < 0 ? p + r : r. */
+ * HOST residue-major buffer.  The random bytes are the stream of `seed`:
+ * block b (32 bytes, four little-endian 64-bit words) is Philox4x64-10 with
+ * counter (b, 0, 0, 0) and key (seed, 0).  uniform (details/poly.hpp:200-228):
+ * residue cm takes stream words cm*degree .. cm*degree+degree-1, each masked to
+ * the prime's bit length (2^62-1) and reduced by p once if >= p.  gaussian
+ * (details/poly.hpp:236-260): one centred discrete Gaussian sample per
+ * coefficient (density proportional to exp(-x^2/(2 sigma^2)), sigma in
+ * (0, 1e6]), drawn from the table of ya_gaussian_cdt with stream word 2i
+ * (magnitude) and the top bit of word 2i+1 (sign), shared by all residues and
+ * mapped r < 0 ? p + r : r. */
 int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed);
 int ya_sample_gaussian(uint64_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed);
-/* poly::set(ZO_dist) (details/poly.hpp:438-451): a byte r per coefficient,
- * r <= rho ? ((r & 2) ? p-1 : 1) : 0, shared by all residues. */
+/* poly::set(ZO_dist) (details/poly.hpp:267-279): stream byte i is
+ * coefficient i's r, r <= rho ? ((r & 2) ? p-1 : 1) : 0, shared by all
+ * residues. */
 int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed);
-/* poly::set(hwt_dist) (details/poly.hpp:458-501): exactly hwt nonzero
- * coefficients (uniform positions), each +1 or p-1, shared by all residues. */
+/* poly::set(hwt_dist) (details/poly.hpp:287-330): exactly hwt nonzero
+ * coefficients, shared by all residues.  Reservoir: slots 0..hwt-1 start at
+ * positions 0..hwt-1; step k = hwt..degree-1 draws pos uniform below k
+ * (first of the four words of Philox block (k, 1, 0, 0) with
+ * w <= floor((2^64-1)/k)*k, else the fourth; pos = w mod k) and replaces
+ * slot pos if pos < hwt.  The j-th smallest hit position is +1 if stream
+ * word j has bit 1 set, else p-1. */
 int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed);
+/* The Gaussian samplers' table: cdf[k] = floor(P(|X| <= k) * 2^64) for
+ * k < tail = ceil(13 sigma), cdf[tail] = 2^64-1; *count = tail + 1.  out may
+ * be NULL to query the count. */
+int ya_gaussian_cdt(double sigma, uint64_t *out, size_t capacity, size_t *count);
+
+/* ---- device samplers (details/poly.hpp:200-330 on the device) ------------
+ * The same streams and mappings as the host samplers above, written straight
+ * into a DEVICE residue-major buffer (16-byte aligned) on `stream`: the call
+ * with the same seed returns the same words as ya_sample_*.  No counterpart in
+ * the reference (its samplers are host-only); they replace host sampling plus
+ * the upload.  gaussian: the signed sample r is multiplied by `amplifier`
+ * (gaussian::amplifier, details/poly.hpp:248-251; wrapping 64-bit product a)
+ * and mapped a < 0 ? p + a : a; amplifier 1 is ya_sample_gaussian.  hwt uses
+ * stream-ordered scratch (hipMallocAsync) of hwt + 2*ceil(degree/32) words. */
+int ya_sample_uniform_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint64_t seed, void *stream);
+int ya_sample_gaussian_device(uint64_t *d_out, size_t degree, size_t nmoduli, double sigma, uint64_t amplifier,
+                              uint64_t seed, void *stream);
+int ya_sample_zo_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, void *stream);
+int ya_sample_hwt_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed, void *stream);
 
 /* ---- CRT lift: residues -> multi-limb integers ----------------------------
  * Replaces poly<N>::poly2mpz (include/yell/details/poly.hpp:162-165), i.e.
@@ -363,11 +406,21 @@ int ya32_poly_multiply(uint32_t *c, const uint32_t *a, const uint32_t *b, size_t
 int ya32_poly_multiply_many(uint32_t *c, const uint32_t *a, const uint32_t *b, size_t degree,
                             size_t nmoduli, size_t count, void *stream);
 /* Host samplers of the 32-bit build (same rules as ya_sample_*, with the
- * 30-bit primes: uniform words masked to 30 bits, minus p once). */
+ * 30-bit primes; uniform reads the stream as 32-bit words, as the 32-bit
+ * reference fills its uint32 residues with random bytes: residue cm takes
+ * 32-bit words cm*degree .. cm*degree+degree-1, masked to 30 bits, minus p
+ * once). */
 int ya32_sample_uniform(uint32_t *out, size_t degree, size_t nmoduli, uint64_t seed);
 int ya32_sample_gaussian(uint32_t *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed);
 int ya32_sample_zo(uint32_t *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed);
 int ya32_sample_hwt(uint32_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed);
+/* Device samplers of the 32-bit build (as ya_sample_*_device; uniform takes
+ * 32-bit stream words, residue cm from word cm*degree on). */
+int ya32_sample_uniform_device(uint32_t *d_out, size_t degree, size_t nmoduli, uint64_t seed, void *stream);
+int ya32_sample_gaussian_device(uint32_t *d_out, size_t degree, size_t nmoduli, double sigma, uint64_t amplifier,
+                                uint64_t seed, void *stream);
+int ya32_sample_zo_device(uint32_t *d_out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, void *stream);
+int ya32_sample_hwt_device(uint32_t *d_out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed, void *stream);
 
 /* ---- measurement ---------------------------------------------------------
  * No counterpart in the reference.  Device throughput of the VALU work this

@@ -39,8 +39,83 @@ void polymul_n(T *c, const T *a, const T *b, size_t L, T *wa, T *wb) {
   for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::backward(c + cm * N, cm);
 }
 
+// The RLWE flow of the reference's example.cpp:12-58 on caller arrays, each
+// [L][N], with the reference's transforms and functors in the order the
+// example's poly<N> calls run them (add_product_of: details/poly.hpp:119-132,
+// negate: :152-158, forward / forward_lazy / backward: :171-186, +=:
+// ops.hpp:26-41).  The samples come from the caller: sx (hwt), pk_bx (the
+// Gaussian e0), pk_ax (uniform), u (ZO), ct_bx (Gaussian e1), ct_ax (Gaussian
+// e2); sx, pk_bx, u, ct_bx, ct_ax are transformed in place.
+template <size_t N>
+struct Rlwe {
+  static void fwd(T *x, size_t L) { for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::forward(x + cm * N, cm); }
+  static void fwd_lazy(T *x, size_t L) {
+    for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::forward_lazy(x + cm * N, cm);
+  }
+  static void bwd(T *x, size_t L) { for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::backward(x + cm * N, cm); }
+  static void add_product_of(T *acc, const T *a, const T *b, size_t L) {
+    yell::ops::muladd muladd;
+    for (size_t cm = 0; cm < L; ++cm)
+      for (size_t i = 0; i < N; ++i) muladd.compute(acc[cm * N + i], a[cm * N + i], b[cm * N + i], cm);
+  }
+  static void negate(T *x, size_t L) {
+    for (size_t cm = 0; cm < L; ++cm) {
+      const T p = yell::params::P[cm];
+      for (size_t i = 0; i < N; ++i) x[cm * N + i] = p - x[cm * N + i];
+    }
+  }
+  static void add(T *x, const T *y, size_t L) {
+    yell::ops::addmod addmod;
+    for (size_t cm = 0; cm < L; ++cm)
+      for (size_t i = 0; i < N; ++i) addmod.compute(x[cm * N + i], y[cm * N + i], cm);
+  }
+  static void keygen(size_t L, T *sx, T *pk_bx, const T *pk_ax) {
+    fwd(sx, L);                         // example.cpp:15
+    fwd(pk_bx, L);                      // :25
+    add_product_of(pk_bx, pk_ax, sx, L);  // :26
+    negate(pk_bx, L);                   // :27
+  }
+  static void encrypt(size_t L, const T *pk_bx, const T *pk_ax, const T *msg, T *u, T *ct_bx, T *ct_ax) {
+    fwd(u, L);                          // :40
+    add(ct_bx, msg, L);                 // :42
+    fwd_lazy(ct_bx, L);                 // :43
+    add_product_of(ct_bx, u, pk_bx, L);   // :44
+    fwd_lazy(ct_ax, L);                 // :47
+    add_product_of(ct_ax, u, pk_ax, L);   // :48
+  }
+  static void decrypt(size_t L, const T *sx, const T *ct_bx, const T *ct_ax, T *plain) {
+    std::memcpy(plain, ct_bx, sizeof(T) * N * L);  // :55
+    add_product_of(plain, sx, ct_ax, L);  // :56
+    bwd(plain, L);                      // :57
+  }
+  // `reps` encrypt + decrypt rounds on each of `threads` std::threads; each
+  // round first copies the samples into the thread's own buffers (about 1% of
+  // a round); tables built beforehand.  Returns wall-clock seconds.
+  static double bench(size_t L, const T *sx, const T *pk_bx, const T *pk_ax, const T *msg, const T *u,
+                      const T *e1, const T *e2, int threads, int reps) {
+    for (size_t cm = 0; cm < L; ++cm) yell::ntt<N>::init_table(cm);
+    std::vector<std::thread> pool;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int th = 0; th < threads; ++th) {
+      pool.emplace_back([=]() {
+        std::vector<T> wu(N * L), wb(N * L), wa(N * L), pl(N * L);
+        for (int r = 0; r < reps; ++r) {
+          std::memcpy(wu.data(), u, sizeof(T) * N * L);
+          std::memcpy(wb.data(), e1, sizeof(T) * N * L);
+          std::memcpy(wa.data(), e2, sizeof(T) * N * L);
+          encrypt(L, pk_bx, pk_ax, msg, wu.data(), wb.data(), wa.data());
+          decrypt(L, sx, wb.data(), wa.data(), pl.data());
+        }
+      });
+    }
+    for (auto &t : pool) t.join();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
 template <size_t N>
 struct Ops {
+  static constexpr size_t kN = N;
   static void fwd(T *x, size_t cm) { yell::ntt<N>::forward(x, cm); }
   static void fwd_lazy(T *x, size_t cm) { yell::ntt<N>::forward_lazy(x, cm); }
   static void bwd(T *x, size_t cm) { yell::ntt<N>::backward(x, cm); }
@@ -124,6 +199,24 @@ double yref_bench_polymul(int logn, const T *a, const T *b, size_t L, int thread
     case 14: s = Ops<(1u << 14)>::bench(a, b, L, threads, reps); break;
     case 15: s = Ops<(1u << 15)>::bench(a, b, L, threads, reps); break;
     case 16: s = Ops<(1u << 16)>::bench(a, b, L, threads, reps); break;
+    default: break;
+  }
+  return s;
+}
+
+int yref_rlwe(int logn, size_t L, T *sx, T *pk_bx, const T *pk_ax, const T *msg, T *u, T *ct_bx, T *ct_ax,
+              T *plain) {
+  YREF_SWITCH(logn, (Rlwe<O::kN>::keygen(L, sx, pk_bx, pk_ax), Rlwe<O::kN>::encrypt(L, pk_bx, pk_ax, msg, u, ct_bx, ct_ax),
+                     Rlwe<O::kN>::decrypt(L, sx, ct_bx, ct_ax, plain)));
+  return 0;
+}
+double yref_bench_rlwe(int logn, size_t L, const T *sx, const T *pk_bx, const T *pk_ax, const T *msg, const T *u,
+                       const T *e1, const T *e2, int threads, int reps) {
+  double s = -1.0;
+  switch (logn) {
+    case 12: s = Rlwe<(1u << 12)>::bench(L, sx, pk_bx, pk_ax, msg, u, e1, e2, threads, reps); break;
+    case 13: s = Rlwe<(1u << 13)>::bench(L, sx, pk_bx, pk_ax, msg, u, e1, e2, threads, reps); break;
+    case 14: s = Rlwe<(1u << 14)>::bench(L, sx, pk_bx, pk_ax, msg, u, e1, e2, threads, reps); break;
     default: break;
   }
   return s;

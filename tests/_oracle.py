@@ -196,6 +196,8 @@ class RefLib:
             "yref_addmod": (w, [w, w, sz]), "yref_submod": (w, [w, w, sz]),
             "yref_muladd": (w, [w, w, w, sz]), "yref_shoupify": (w, [w, sz]),
             "yref_barrett": (w, [w, w, sz]),
+            "yref_rlwe": (i32, [i32, sz] + [vp] * 8),
+            "yref_bench_rlwe": (ctypes.c_double, [i32, sz] + [vp] * 7 + [i32, i32]),
         })
 
     def _inplace(self, fn, x, n, cm):
@@ -224,6 +226,23 @@ class RefLib:
         b = np.ascontiguousarray(b, dtype=self.dt)
         assert self.lib.yref_polymul(n.bit_length() - 1, _p(out), _p(a), _p(b), L) == 0
         return out
+
+
+    def rlwe(self, n, L, sx, e0, pk_ax, msg, u, e1, e2):
+        """example.cpp:12-58 through the reference's sources:
+        (sx in NTT form, pk_bx, ct_bx, ct_ax, plain)."""
+        c = lambda x: np.ascontiguousarray(x, dtype=self.dt).copy()
+        sx, bx, ax, m, u, cb, ca = c(sx), c(e0), c(pk_ax), c(msg), c(u), c(e1), c(e2)
+        plain = np.empty(n * L, dtype=self.dt)
+        assert self.lib.yref_rlwe(n.bit_length() - 1, L, _p(sx), _p(bx), _p(ax), _p(m), _p(u), _p(cb), _p(ca),
+                                  _p(plain)) == 0
+        return sx, bx, cb, ca, plain
+
+    def bench_rlwe(self, n, L, sxf, pk_bx, pk_ax, msg, u, e1, e2, threads, reps):
+        """Seconds for `reps` encrypt + decrypt rounds on each of `threads` threads."""
+        c = lambda x: np.ascontiguousarray(x, dtype=self.dt)
+        args = [c(x) for x in (sxf, pk_bx, pk_ax, msg, u, e1, e2)]
+        return self.lib.yref_bench_rlwe(n.bit_length() - 1, L, *[_p(a) for a in args], threads, reps)
 
 
 class RefGmpLib:
@@ -263,6 +282,25 @@ def crt_oracle():
     """oracle/crt_oracle.py (Python-integer restatement of the CRT lift)."""
     import importlib.util
     spec = importlib.util.spec_from_file_location("crt_oracle", os.path.join(ROOT, "oracle", "crt_oracle.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def sampler_oracle():
+    """oracle/sampler_oracle.py (the samplers over numpy's Philox)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("sampler_oracle",
+                                                  os.path.join(ROOT, "oracle", "sampler_oracle.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def rlwe_oracle():
+    """oracle/rlwe_oracle.py (the example's RLWE flow over the C oracle)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("rlwe_oracle", os.path.join(ROOT, "oracle", "rlwe_oracle.py"))
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
     return mod

@@ -142,11 +142,43 @@ static void test_rlwe_example() {
 #endif  // YELL_USE_32BITS_MODULI
 }
 
+// poly::set draws on the device; the words equal the host sampler's for the
+// seed that set() consumed (next_seed() returns the seed, then advances it).
+static void test_device_samplers() {
+  constexpr size_t N = 2048, L = 3;
+  std::vector<T> h(N * L);
+  const uint64_t s0 = 777;
+  yell::set_seed(s0);
+  const uint64_t first = yell::next_seed();
+  const uint64_t step = yell::next_seed() - first;  // the seed increment
+  yell::set_seed(s0);
+  yell::poly<N> u(L, yell::uniform{}), z(L, yell::ZO_dist{}), w(L, yell::hwt_dist(64));
+  yell::poly<N> g(L, yell::gaussian(3.2));
+#ifdef YELL_USE_32BITS_MODULI
+  auto host = [&](int k, uint64_t seed) {
+    return k == 0 ? ya32_sample_uniform(h.data(), N, L, seed) : k == 1 ? ya32_sample_zo(h.data(), N, L, 0x7F, seed)
+           : k == 2 ? ya32_sample_hwt(h.data(), N, L, 64, seed) : ya32_sample_gaussian(h.data(), N, L, 3.2, seed);
+  };
+#else
+  auto host = [&](int k, uint64_t seed) {
+    return k == 0 ? ya_sample_uniform(h.data(), N, L, seed) : k == 1 ? ya_sample_zo(h.data(), N, L, 0x7F, seed)
+           : k == 2 ? ya_sample_hwt(h.data(), N, L, 64, seed) : ya_sample_gaussian(h.data(), N, L, 3.2, seed);
+  };
+#endif
+  const yell::poly<N> *polys[4] = {&u, &z, &w, &g};
+  for (int k = 0; k < 4; ++k) {
+    CHECK(host(k, s0 + k * step) == YA_OK, "host sampler %d", k);
+    CHECK(polys[k]->to_host() == h, "device sampler %d differs from the host sampler", k);
+  }
+}
+
 int main() {
   if (ya_init(0) != YA_OK) {
     std::fprintf(stderr, "no device: %s\n", ya_last_error());
     return 2;
   }
+  yell::set_seed(42);
+  test_device_samplers();
   yell::set_seed(42);
   test_polymul();
   test_ntt_host_pointer_and_tables();

@@ -1,0 +1,60 @@
+"""Device samplers (ya_sample_*_device, ya32_sample_*_device): the same words
+as the host samplers for the same seed, bit-exact, at small and full sizes
+(C2: N = 2^14, L = 8; C4: N = 2^16, L = 24), hwt at weight 1 and at the
+degree, the Gaussian amplifier against the oracle."""
+import numpy as np
+import pytest
+
+from _oracle import Oracle, sampler_oracle
+from gpu_util import device, say
+from yell_amd import capi
+
+pytestmark = pytest.mark.gpu
+S = sampler_oracle()
+O32 = Oracle(32)
+
+CASES = [("uniform", 8, 1, None), ("uniform", 1 << 14, 8, None), ("uniform", 1 << 16, 24, None),
+         ("gaussian", 8, 1, 3.2), ("gaussian", 1 << 14, 8, 3.2), ("gaussian", 4096, 3, 1000.0),
+         ("zo", 8, 1, 127), ("zo", 1 << 14, 8, 100), ("zo", 4096, 2, 255),
+         ("hwt", 8, 1, 1), ("hwt", 8, 2, 8), ("hwt", 1 << 14, 8, 64), ("hwt", 1 << 14, 2, 1 << 14),
+         ("hwt", 1 << 16, 3, 4096)]
+
+
+def run_device(kind, n, L, param, seed, word, amplifier=1):
+    device()
+    buf = capi.DeviceBuffer(n * L, np.uint64 if word == 64 else np.uint32)
+    capi.sample_device(kind, buf, n, L, param, seed, amplifier=amplifier, word=word)
+    out = buf.download()
+    buf.close()
+    return out
+
+
+@pytest.mark.parametrize("word", [64, 32])
+@pytest.mark.parametrize("kind,n,L,param", CASES)
+def test_device_equals_host(word, kind, n, L, param):
+    if word == 32 and n > (1 << 15):
+        pytest.skip("the 32-bit build stops at 2^15")
+    for seed in (1, 2 ** 63 + 5):
+        say("sampler", word, kind, n, L, param, seed)
+        got = run_device(kind, n, L, param, seed, word)
+        np.testing.assert_array_equal(got, capi.sample_host(kind, n, L, param, seed, word))
+
+
+@pytest.mark.parametrize("word", [64, 32])
+def test_gaussian_amplifier(word):
+    n, L, amp = 4096, 3, 7
+    say("sampler amplifier", word)
+    ps = [int(p) for p in capi.primes(L)] if word == 64 else [O32.prime(j) for j in range(L)]
+    got = run_device("gaussian", n, L, 3.2, 21, word, amplifier=amp)
+    np.testing.assert_array_equal(got, S.gaussian(ps, n, capi.gaussian_cdt(3.2), 21, amp, word))
+
+
+def test_device_sampler_errors():
+    lib = device()
+    buf = capi.DeviceBuffer(64)
+    assert lib.ya_sample_uniform_device(buf.ptr + 8, 8, 1, 1, None) == 1      # not 16-byte aligned
+    assert lib.ya_sample_hwt_device(buf.ptr, 8, 1, 0, 1, None) == 1
+    assert lib.ya_sample_zo_device(buf.ptr, 8, 1, 256, 1, None) == 1
+    assert lib.ya_sample_uniform_device(buf.ptr, 12, 1, 1, None) == 1
+    assert lib.ya32_sample_uniform_device(buf.ptr, 1 << 16, 1, 1, None) == 1
+    buf.close()

@@ -74,6 +74,15 @@ _SIGS = {
     "ya_sample_gaussian": (i32, [vp, sz, sz, ctypes.c_double, u64]),
     "ya_sample_zo": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
     "ya_sample_hwt": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
+    "ya_graph_begin": (i32, [vp]),
+    "ya_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
+    "ya_graph_launch": (i32, [vp, vp]),
+    "ya_graph_destroy": (i32, [vp]),
+    "ya_gaussian_cdt": (i32, [ctypes.c_double, vp, sz, ctypes.POINTER(ctypes.c_size_t)]),
+    "ya_sample_uniform_device": (i32, [vp, sz, sz, u64, vp]),
+    "ya_sample_gaussian_device": (i32, [vp, sz, sz, ctypes.c_double, u64, u64, vp]),
+    "ya_sample_zo_device": (i32, [vp, sz, sz, ctypes.c_uint32, u64, vp]),
+    "ya_sample_hwt_device": (i32, [vp, sz, sz, ctypes.c_uint32, u64, vp]),
     "ya_pointer_is_device": (i32, [vp, ctypes.POINTER(i32)]),
     "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
     "ya_crt_limbs": (i32, [sz, ctypes.POINTER(ctypes.c_size_t)]),
@@ -101,6 +110,10 @@ _SIGS = {
     "ya32_sample_gaussian": (i32, [vp, sz, sz, ctypes.c_double, u64]),
     "ya32_sample_zo": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
     "ya32_sample_hwt": (i32, [vp, sz, sz, ctypes.c_uint32, u64]),
+    "ya32_sample_uniform_device": (i32, [vp, sz, sz, u64, vp]),
+    "ya32_sample_gaussian_device": (i32, [vp, sz, sz, ctypes.c_double, u64, u64, vp]),
+    "ya32_sample_zo_device": (i32, [vp, sz, sz, ctypes.c_uint32, u64, vp]),
+    "ya32_sample_hwt_device": (i32, [vp, sz, sz, ctypes.c_uint32, u64, vp]),
     "ya_poly_crt_lift": (i32, [vp, vp, sz, sz, vp]),
     "ya_crt_constants": (i32, [sz, sz, vp, vp, vp, ctypes.POINTER(ctypes.c_size_t)]),
     "ya_batch_plan_create": (i32, [ctypes.POINTER(vp), vp, sz]),
@@ -283,6 +296,39 @@ def sample_gaussian(degree, nmoduli, sigma, seed):
     out = np.empty(degree * nmoduli, dtype=np.uint64)
     call("ya_sample_gaussian", ptr(out), degree, nmoduli, sigma, seed)
     return out
+
+
+def gaussian_cdt(sigma):
+    """The Gaussian samplers' cumulative table (uint64, ceil(13 sigma) + 1 entries)."""
+    n = ctypes.c_size_t()
+    call("ya_gaussian_cdt", sigma, None, 0, ctypes.byref(n))
+    out = np.empty(n.value, dtype=np.uint64)
+    call("ya_gaussian_cdt", sigma, ptr(out), out.size, ctypes.byref(n))
+    return out
+
+
+def sample_host(kind, degree, nmoduli, param, seed, word=64):
+    """Host sampler `kind` ('uniform', 'gaussian', 'zo', 'hwt') of the 64- or
+    32-bit build; `param` is sigma, rho or hwt (ignored for uniform)."""
+    dt = np.uint64 if word == 64 else np.uint32
+    pre = "ya_" if word == 64 else "ya32_"
+    out = np.empty(degree * nmoduli, dtype=dt)
+    args = () if kind == "uniform" else (param,)
+    call(pre + "sample_" + kind, ptr(out), degree, nmoduli, *args, seed)
+    return out
+
+
+def sample_device(kind, buf, degree, nmoduli, param, seed, amplifier=1, stream=None, word=64):
+    """Device sampler `kind` into DeviceBuffer `buf` (the same words as
+    sample_host with the same seed; gaussian takes `amplifier`)."""
+    pre = "ya_" if word == 64 else "ya32_"
+    if kind == "uniform":
+        args = ()
+    elif kind == "gaussian":
+        args = (param, amplifier)
+    else:
+        args = (param,)
+    call(pre + "sample_" + kind + "_device", buf.ptr, degree, nmoduli, *args, seed, stream)
 
 
 def table_host(degree, cm):

@@ -486,6 +486,35 @@ int ya_event_elapsed_ms(float *ms, void *start, void *end) {
   return YA_OK;
 }
 
+// ---- stream capture into a replayable graph --------------------------------
+int ya_graph_begin(void *stream) {
+  if (!stream) return fail(YA_ERR_INVALID, "capture needs a created stream, not the null stream");
+  YA_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  return YA_OK;
+}
+int ya_graph_end(void *stream, void **graph) {
+  if (!stream || !graph) return fail(YA_ERR_INVALID, "null stream or graph");
+  *graph = nullptr;
+  hipGraph_t g = nullptr;
+  YA_HIP(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  const hipError_t ed = hipGraphDestroy(g);
+  if (e == hipSuccess) e = ed;
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  *graph = x;
+  return YA_OK;
+}
+int ya_graph_launch(void *graph, void *stream) {
+  if (!graph) return fail(YA_ERR_INVALID, "null graph");
+  YA_HIP(hipGraphLaunch((hipGraphExec_t)graph, (hipStream_t)stream), "hipGraphLaunch");
+  return YA_OK;
+}
+int ya_graph_destroy(void *graph) {
+  if (graph) YA_HIP(hipGraphExecDestroy((hipGraphExec_t)graph), "hipGraphExecDestroy");
+  return YA_OK;
+}
+
 // ---- NTT ------------------------------------------------------------------
 static int ntt_common(uint64_t *x, size_t degree, size_t cm0, size_t nres, size_t count, int dir,
                       void *stream) {

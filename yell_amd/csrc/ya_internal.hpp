@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "ya_kernels.hpp"
 
 namespace ya {
@@ -40,5 +42,11 @@ NttShape tuned_shape(int logn, int kind);
 void set_tuned_shape(int logn, int kind, NttShape s);  // sets le only
 // Fused poly-multiplication of degree 2^logn: dual-forward form on/off.
 void set_tuned_dual(int logn, int dual);
+
+// Largest Gaussian parameter of the samplers: ceil(13 sigma) stays below every
+// prime of both builds, so a signed sample maps to one residue without reduction.
+constexpr double kMaxGaussianSigma = 1e6;
+// Cumulative table of the Gaussian samplers (host and device use the same one).
+int gaussian_cdt(double sigma, std::vector<uint64_t> &cdf);
 
 }  // namespace ya

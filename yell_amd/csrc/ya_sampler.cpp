@@ -2,50 +2,53 @@
 //
 // The reference's samplers draw from an unseeded Salsa20 keystream
 // (lib/prng/fastrandombytes.cpp:21-34), so their outputs cannot be
-// reproduced; these keep the reference's mapping of random words to residues
-// and are seeded:
-//   uniform   include/yell/details/poly.hpp:372-396
-//   gaussian  include/yell/details/poly.hpp:408-431 (one signed sample per
+// reproduced.  These keep the reference's mapping of random bytes to
+// residues and read the seeded Philox stream of ya_philox.hpp; the device
+// samplers (ya_sampler_dev.hip) read the same stream and return the same
+// words:
+//   uniform   include/yell/details/poly.hpp:200-228
+//   gaussian  include/yell/details/poly.hpp:236-260 (one signed sample per
 //             coefficient shared by all residues, r < 0 ? p + r : r)
-// The random words come from xoshiro256** (Blackman & Vigna) seeded through
-// splitmix64; the Gaussian is a cumulative-distribution-table sampler over
-// |x| <= ceil(13 sigma) (tail mass below 2^-120).
+//   ZO        include/yell/details/poly.hpp:267-279
+//   hwt       include/yell/details/poly.hpp:287-330
+// The Gaussian is a cumulative-distribution-table sampler over
+// |x| <= ceil(13 sigma) (tail mass below 2^-120); the reference uses its own
+// table sampler (include/yell/prng/FastGaussianNoise.hpp:484-602) for the same
+// distribution.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <algorithm>
 #include <vector>
 
 #include "../../include/yell_amd.h"
 #include "ya_internal.hpp"
+#include "ya_philox.hpp"
 
-namespace {
+namespace ya {
 
-struct Xoshiro {
-  uint64_t s[4];
-  explicit Xoshiro(uint64_t seed) {
-    for (auto &w : s) {  // splitmix64
-      seed += 0x9e3779b97f4a7c15ULL;
-      uint64_t z = seed;
-      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-      z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-      w = z ^ (z >> 31);
-    }
+// cdf[k] = P(|X| <= k) * 2^64 for k < tail, cdf[tail] = 2^64 - 1, with
+// P(X = x) proportional to exp(-x^2 / (2 sigma^2)) over |x| <= tail.
+int gaussian_cdt(double sigma, std::vector<uint64_t> &cdf) {
+  if (!(sigma > 0.0) || sigma > kMaxGaussianSigma)
+    return fail(YA_ERR_INVALID, "sigma %g out of (0, %g]", sigma, kMaxGaussianSigma);
+  const int tail = (int)std::ceil(13.0 * sigma);
+  std::vector<long double> w(tail + 1);
+  long double tot = 0;
+  for (int k = 0; k <= tail; ++k) {
+    w[k] = std::exp(-(long double)k * k / (2.0L * sigma * sigma)) * (k ? 2.0L : 1.0L);
+    tot += w[k];
   }
-  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-  uint64_t next() {
-    const uint64_t r = rotl(s[1] * 5, 7) * 9;
-    const uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl(s[3], 45);
-    return r;
+  cdf.assign(tail + 1, 0);
+  long double acc = 0;
+  for (int k = 0; k <= tail; ++k) {
+    acc += w[k] / tot;
+    const long double v = acc * 18446744073709551616.0L;
+    cdf[k] = (k == tail || v >= 18446744073709551615.0L) ? ~0ULL : (uint64_t)v;
   }
-};
+  return YA_OK;
+}
 
-}  // namespace
+}  // namespace ya
 
 namespace {
 
@@ -58,19 +61,26 @@ int sample_shape(size_t degree, size_t nmoduli, int maxlog, size_t nmax) {
   return YA_OK;
 }
 
-// include/yell/details/poly.hpp:372-396: per residue, a random word masked to
-// the prime's bit length (kModulusBitsize), minus p once if >= p.
+// Word g (of type W) of the byte stream of `seed`.
+template <class W>
+W stream_value(uint64_t seed, size_t g) {
+  if constexpr (sizeof(W) == 8) return ya::stream_word(seed, g);
+  return (W)(ya::stream_word(seed, g >> 1) >> (32 * (g & 1)));
+}
+
+// poly::set(uniform) (include/yell/details/poly.hpp:200-228): residue cm
+// takes the next degree words of the stream; each is masked to the prime's
+// bit length (floor(log2 p) + 1 bits) and reduced by p once if >= p.
 template <class W, class PF>
 int sample_uniform(W *out, size_t degree, size_t nmoduli, uint64_t seed, PF prime) {
   if (!out) return ya::fail(YA_ERR_INVALID, "null output");
-  Xoshiro g(seed);
   for (size_t cm = 0; cm < nmoduli; ++cm) {
     const uint64_t p = prime(cm);
-    const int top = 63 - __builtin_clzll(p);  // floor(log2 p)
+    const int top = 63 - __builtin_clzll(p);
     const uint64_t mask = (top >= 63) ? ~0ULL : ((1ULL << (top + 1)) - 1);
     W *dst = out + cm * degree;
     for (size_t i = 0; i < degree; ++i) {
-      uint64_t v = g.next() & mask;
+      uint64_t v = (uint64_t)stream_value<W>(seed, cm * degree + i) & mask;
       if (v >= p) v -= p;
       dst[i] = (W)v;
     }
@@ -78,88 +88,67 @@ int sample_uniform(W *out, size_t degree, size_t nmoduli, uint64_t seed, PF prim
   return YA_OK;
 }
 
+// poly::set(gaussian) (include/yell/details/poly.hpp:236-260): coefficient i
+// takes stream words 2i (magnitude, from the table) and 2i+1 (sign, top
+// bit); the signed sample r is shared by all residues, r < 0 ? p + r : r.
+// |r| <= ceil(13 sigma) is below every prime of both builds.
 template <class W, class PF>
 int sample_gaussian(W *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed, PF prime) {
   if (!out) return ya::fail(YA_ERR_INVALID, "null output");
-  if (!(sigma > 0.0) || sigma > 1e6) return ya::fail(YA_ERR_INVALID, "sigma %g out of range", sigma);
-  const int tail = (int)std::ceil(13.0 * sigma);
-  std::vector<long double> w(tail + 1);
-  long double tot = 0;
-  for (int k = 0; k <= tail; ++k) {
-    w[k] = std::exp(-(long double)k * k / (2.0L * sigma * sigma)) * (k ? 2.0L : 1.0L);
-    tot += w[k];
-  }
-  // cdf[k] = P(|X| <= k) scaled to 2^64; the last entry saturates.
-  std::vector<uint64_t> cdf(tail + 1);
-  long double acc = 0;
-  for (int k = 0; k <= tail; ++k) {
-    acc += w[k] / tot;
-    const long double v = acc * 18446744073709551616.0L;
-    cdf[k] = (k == tail || v >= 18446744073709551615.0L) ? ~0ULL : (uint64_t)v;
-  }
-  Xoshiro g(seed);
+  std::vector<uint64_t> cdf;
+  int rc = ya::gaussian_cdt(sigma, cdf);
+  if (rc) return rc;
+  const size_t tail = cdf.size() - 1;
   std::vector<int64_t> r(degree);
   for (size_t i = 0; i < degree; ++i) {
-    const uint64_t u = g.next();
-    int k = 0;
+    const uint64_t u = ya::stream_word(seed, 2 * i), s = ya::stream_word(seed, 2 * i + 1);
+    size_t k = 0;
     while (k < tail && u >= cdf[k]) ++k;
-    const bool neg = k != 0 && (g.next() >> 63);
-    r[i] = neg ? -(int64_t)k : (int64_t)k;
+    r[i] = (k != 0 && (s >> 63)) ? -(int64_t)k : (int64_t)k;
   }
   for (size_t cm = 0; cm < nmoduli; ++cm) {
     const int64_t p = (int64_t)prime(cm);
     W *dst = out + cm * degree;
-    for (size_t i = 0; i < degree; ++i) {
-      // reduce |r| < p first (sigma large relative to a 30-bit prime), then map r < 0 to p + r
-      const int64_t v = r[i] % p;
-      dst[i] = (W)(v < 0 ? p + v : v);
-    }
+    for (size_t i = 0; i < degree; ++i) dst[i] = (W)(r[i] < 0 ? p + r[i] : r[i]);
   }
   return YA_OK;
 }
 
-// poly::set(ZO_dist) (include/yell/details/poly.hpp:438-451): one random byte
-// r per coefficient shared by all residues; r <= rho ? ((r & 2) ? p-1 : 1) : 0.
+// poly::set(ZO_dist) (include/yell/details/poly.hpp:267-279): byte i of the
+// stream is coefficient i's r, shared by all residues;
+// r <= rho ? ((r & 2) ? p-1 : 1) : 0.
 template <class W, class PF>
 int sample_zo(W *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, PF prime) {
   if (!out || rho > 255) return ya::fail(YA_ERR_INVALID, "null output or rho > 255");
-  Xoshiro g(seed);
   std::vector<uint8_t> r(degree);
-  for (size_t i = 0; i < degree; i += 8) {
-    const uint64_t w = g.next();
-    for (size_t k = 0; k < 8 && i + k < degree; ++k) r[i + k] = (uint8_t)(w >> (8 * k));
-  }
+  for (size_t i = 0; i < degree; ++i) r[i] = (uint8_t)(ya::stream_word(seed, i >> 3) >> (8 * (i & 7)));
   for (size_t cm = 0; cm < nmoduli; ++cm) {
     const W p = (W)prime(cm);
-    for (size_t i = 0; i < degree; ++i)
-      out[cm * degree + i] = r[i] <= rho ? ((r[i] & 2) ? p - 1 : 1) : 0;
+    for (size_t i = 0; i < degree; ++i) out[cm * degree + i] = r[i] <= rho ? ((r[i] & 2) ? p - 1 : 1) : 0;
   }
   return YA_OK;
 }
 
-// poly::set(hwt_dist) (include/yell/details/poly.hpp:458-501): exactly `hwt`
-// positions chosen uniformly (reservoir sampling), each +1 or -1 with equal
-// probability, shared by all residues (-1 stored as p-1).
+// poly::set(hwt_dist) (include/yell/details/poly.hpp:287-330): slots
+// hit[0..hwt) start at positions 0..hwt-1; for k = hwt .. degree-1 a position
+// pos uniform below k (ya::reservoir_pos) replaces slot pos if pos < hwt.
+// The hit positions are sorted and the j-th smallest gets +1 if stream word j
+// has bit 1 set, else p-1 (-1); all residues share the pattern.
 template <class W, class PF>
 int sample_hwt(W *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed, PF prime) {
   if (!out || hwt == 0 || hwt > degree) return ya::fail(YA_ERR_INVALID, "hwt must be in [1, degree]");
-  Xoshiro g(seed);
   std::vector<size_t> hit(hwt);
   for (size_t k = 0; k < hwt; ++k) hit[k] = k;
-  for (size_t k = hwt; k < degree; ++k) {  // reservoir sampling, unbiased draw in [0, k]
-    const uint64_t lim = UINT64_MAX - UINT64_MAX % (k + 1);
-    uint64_t v;
-    do v = g.next(); while (v >= lim);
-    const size_t pos = (size_t)(v % (k + 1));
+  for (size_t k = hwt; k < degree; ++k) {
+    const uint64_t pos = ya::reservoir_pos(seed, k);
     if (pos < hwt) hit[pos] = k;
   }
-  std::vector<uint8_t> sign(hwt);
-  for (size_t k = 0; k < hwt; ++k) sign[k] = (uint8_t)(g.next() >> 63);
+  std::sort(hit.begin(), hit.end());
   for (size_t cm = 0; cm < nmoduli; ++cm) {
     const W p = (W)prime(cm);
     W *dst = out + cm * degree;
     std::fill(dst, dst + degree, 0);
-    for (size_t k = 0; k < hwt; ++k) dst[hit[k]] = sign[k] ? 1 : p - 1;
+    for (size_t j = 0; j < hwt; ++j) dst[hit[j]] = (ya::stream_word(seed, j) & 2) ? 1 : p - 1;
   }
   return YA_OK;
 }
@@ -176,6 +165,18 @@ int shape32(size_t degree, size_t nmoduli) { return sample_shape(degree, nmoduli
 }  // namespace
 
 extern "C" {
+
+int ya_gaussian_cdt(double sigma, uint64_t *out, size_t capacity, size_t *count) {
+  std::vector<uint64_t> cdf;
+  int rc = ya::gaussian_cdt(sigma, cdf);
+  if (rc) return rc;
+  if (count) *count = cdf.size();
+  if (out) {
+    if (capacity < cdf.size()) return ya::fail(YA_ERR_INVALID, "capacity %zu < %zu table entries", capacity, cdf.size());
+    std::copy(cdf.begin(), cdf.end(), out);
+  }
+  return YA_OK;
+}
 
 int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed) {
   int rc = shape64(degree, nmoduli);
