@@ -367,11 +367,13 @@ def sampler_bench(stream, ev, reps=20):
     return out
 
 
-def rlwe_bench(stream, ev, batch=64, reps=5):
+def rlwe_bench(stream, ev, batch=256, reps=5):
     """The reference example's RLWE flow (example.cpp:12-58) at its own
     parameters (N = 8192, L = 4, hwt 64, sigma 3.2): key generation once
-    (eager), then a step of `batch` encryptions + decryptions under that key,
-    captured as one graph and replayed; samples from the device samplers."""
+    (eager), then a step of `batch` encryptions + decryptions under that key
+    (one launch per flow step over all messages: the *_many transforms,
+    samplers and pointwise forms), captured as one graph and replayed;
+    samples from the device samplers."""
     from yell_amd.rlwe import Rlwe
     n, L = 8192, 4
     f = Rlwe(n, L, stream=stream)
@@ -382,15 +384,14 @@ def rlwe_bench(stream, ev, batch=64, reps=5):
     f.keygen(sx, bx, ax, (1, 2, 3))
     capi.call("ya_event_record", ev.b, stream)
     keygen_us = ev.elapsed_ms() * 1e3
-    msgs = [f.buffer() for _ in range(batch)]
-    for k, m in enumerate(msgs):
-        capi.sample_device("uniform", m, n, L, None, 1000 + k, stream=stream)
-    work = [(f.buffer(), f.buffer(), f.buffer(), f.buffer()) for _ in range(batch)]
+    msgs = f.buffer(batch)
+    capi.call("ya_sample_uniform_device_many", msgs.ptr, n, L, batch, 1000, stream)
+    cb, ca, u, pl = (f.buffer(batch) for _ in range(4))
+    seeds = (10, 10000, 20000)
 
     def step():
-        for k, (cb, ca, u, pl) in enumerate(work):
-            f.encrypt(cb, ca, u, msgs[k], bx, ax, (10 + 3 * k, 11 + 3 * k, 12 + 3 * k))
-            f.decrypt(pl, cb, ca, sx)
+        f.encrypt_many(cb, ca, u, msgs, bx, ax, batch, seeds)
+        f.decrypt_many(pl, cb, ca, sx, batch)
 
     step()
     capi.call("ya_stream_synchronize", stream)
@@ -403,13 +404,14 @@ def rlwe_bench(stream, ev, batch=64, reps=5):
     capi.call("ya_stream_synchronize", stream)
     ms = timed(ev, stream, lambda: capi.call("ya_graph_launch", g.value, stream), reps)
     capi.call("ya_graph_destroy", g.value)
-    first = {"plain0": work[0][3].download(), "ct0": (work[0][0].download(), work[0][1].download()),
-             "sx": sx.download(), "pk": (bx.download(), ax.download())}
-    for b in [sx, bx, ax] + msgs + [x for w in work for x in w]:
+    w = n * L
+    first = {"plain0": pl.download(w), "ct0": (cb.download(w), ca.download(w)),
+             "sx": sx.download(), "pk": (bx.download(), ax.download()), "seeds": seeds}
+    for b in (sx, bx, ax, msgs, cb, ca, u, pl):
         b.close()
     return {"degree": n, "nmoduli": L, "hwt": 64, "sigma": 3.2, "batch": batch, "keygen_us": keygen_us,
             "eager_step_ms": eager_ms, "graph_step_ms": ms, "value": batch / (ms / 1e3),
-            "unit": "encrypt+decrypt/s", "launches_per_encrypt_decrypt": 12, "_first": first}
+            "unit": "encrypt+decrypt/s", "launches_per_step": 12, "_first": first}
 
 
 def rlwe_cpu_reference(rl, seconds=3.0):
@@ -423,7 +425,8 @@ def rlwe_cpu_reference(rl, seconds=3.0):
     n, L = rl["degree"], rl["nmoduli"]
     h = lambda kind, param, s: capi.sample_host(kind, n, L, param, s, 64)
     sx, e0, ax = h("hwt", 64, 1), h("gaussian", 3.2, 2), h("uniform", None, 3)
-    msg, u, e1, e2 = h("uniform", None, 1000), h("zo", 0x7F, 10), h("gaussian", 3.2, 11), h("gaussian", 3.2, 12)
+    su, se1, se2 = first["seeds"]
+    msg, u, e1, e2 = h("uniform", None, 1000), h("zo", 0x7F, su), h("gaussian", 3.2, se1), h("gaussian", 3.2, se2)
     try:
         ref = _oracle.RefLib(64)
     except RuntimeError:

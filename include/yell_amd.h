@@ -153,6 +153,21 @@ int ya_poly_forward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t 
 int ya_poly_forward_lazy_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count,
                               void *stream);
 int ya_poly_backward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream);
+/* The pointwise operations over `count` polynomials stored back to back (as
+ * the *_many transforms): polynomial k of out is op(a_k, b_k), with b_k = b
+ * itself when b_shared != 0 (one operand for all, e.g. a key), else the k-th
+ * polynomial of b.  Same words as `count` calls of the single forms; out may
+ * alias a; a shared b must not overlap out.  add_product_of_many:
+ * acc_k.add_product_of(a_k, b_k). */
+int ya_poly_mul_many(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                     size_t count, int b_shared, void *stream);
+int ya_poly_add_many(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                     size_t count, int b_shared, void *stream);
+int ya_poly_sub_many(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                     size_t count, int b_shared, void *stream);
+int ya_poly_add_product_of_many(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t degree,
+                                size_t nmoduli, size_t count, int b_shared, void *stream);
+int ya_poly_negate_many(uint64_t *x, size_t degree, size_t nmoduli, size_t count, void *stream);
 /* acc.add_product_of(a[0], b[0]); acc.add_product_of(a[1], b[1]); ... in one
  * pass (details/poly.hpp:119-132 per pair, same words): a and b are HOST
  * arrays of npairs DEVICE pointers to polys of the acc's shape. */
@@ -345,6 +360,14 @@ int ya_sample_gaussian_device(uint64_t *d_out, size_t degree, size_t nmoduli, do
                               uint64_t seed, void *stream);
 int ya_sample_zo_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, void *stream);
 int ya_sample_hwt_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed, void *stream);
+/* `count` polynomials back to back in one launch: polynomial k (words
+ * [k*L*N, (k+1)*L*N)) holds what the single call with seed + k writes. */
+int ya_sample_uniform_device_many(uint64_t *d_out, size_t degree, size_t nmoduli, size_t count, uint64_t seed,
+                                  void *stream);
+int ya_sample_gaussian_device_many(uint64_t *d_out, size_t degree, size_t nmoduli, size_t count, double sigma,
+                                   uint64_t amplifier, uint64_t seed, void *stream);
+int ya_sample_zo_device_many(uint64_t *d_out, size_t degree, size_t nmoduli, size_t count, uint32_t rho,
+                             uint64_t seed, void *stream);
 
 /* ---- CRT lift: residues -> multi-limb integers ----------------------------
  * Replaces poly<N>::poly2mpz (include/yell/details/poly.hpp:162-165), i.e.

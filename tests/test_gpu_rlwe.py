@@ -46,6 +46,36 @@ def test_device_flow_matches_oracle(word, n, L):
         b.close()
 
 
+def test_many_messages_match_single_flow():
+    """encrypt_many / decrypt_many of 6 messages = encrypt / decrypt of each
+    with seeds + k, bit-exact."""
+    device()
+    n, L, count = 8192, 4, 6
+    f = Rlwe(n, L)
+    sx, bx, ax = f.buffer(), f.buffer(), f.buffer()
+    f.keygen(sx, bx, ax, (1, 2, 3))
+    msgs = capi.DeviceBuffer(n * L * count)
+    capi.call("ya_sample_uniform_device_many", msgs.ptr, n, L, count, 500, None)
+    cb, ca, u, pl = (f.buffer(count) for _ in range(4))
+    say("rlwe many", count)
+    f.encrypt_many(cb, ca, u, msgs, bx, ax, count, (10, 20, 30))
+    f.decrypt_many(pl, cb, ca, sx, count)
+    got = [x.download().reshape(count, n * L) for x in (cb, ca, pl)]
+    w = n * L
+    m_all = msgs.download()
+    for k in range(count):
+        m1, c1, a1, u1, p1 = (f.buffer() for _ in range(5))
+        m1.upload(m_all[k * w:(k + 1) * w])
+        f.encrypt(c1, a1, u1, m1, bx, ax, (10 + k, 20 + k, 30 + k))
+        f.decrypt(p1, c1, a1, sx)
+        for name, g, one in zip(("ct_bx", "ct_ax", "plain"), got, (c1, a1, p1)):
+            np.testing.assert_array_equal(g[k], one.download(), err_msg="%s of message %d" % (name, k))
+        for b in (m1, c1, a1, u1, p1):
+            b.close()
+    for b in (sx, bx, ax, msgs, cb, ca, u, pl):
+        b.close()
+
+
 def test_graph_replay_matches_eager():
     """Capture encrypt + decrypt of 4 messages on a created stream, replay it
     three times with eager work in between: each replay equals the eager run."""

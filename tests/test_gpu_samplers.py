@@ -49,6 +49,21 @@ def test_gaussian_amplifier(word):
     np.testing.assert_array_equal(got, S.gaussian(ps, n, capi.gaussian_cdt(3.2), 21, amp, word))
 
 
+@pytest.mark.parametrize("kind,param", [("uniform", None), ("gaussian", 3.2), ("zo", 0x7F)])
+def test_many_equals_single(kind, param):
+    """Polynomial k of a *_many launch = the single call with seed + k."""
+    n, L, count, seed = 2048, 3, 5, 40
+    say("sampler many", kind)
+    device()
+    buf = capi.DeviceBuffer(n * L * count)
+    args = () if kind == "uniform" else ((param, 1) if kind == "gaussian" else (param,))
+    capi.call("ya_sample_%s_device_many" % kind, buf.ptr, n, L, count, *args, seed, None)
+    got = buf.download().reshape(count, n * L)
+    buf.close()
+    for k in range(count):
+        np.testing.assert_array_equal(got[k], capi.sample_host(kind, n, L, param, seed + k, 64))
+
+
 def test_device_sampler_errors():
     lib = device()
     buf = capi.DeviceBuffer(64)

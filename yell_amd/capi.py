@@ -83,6 +83,14 @@ _SIGS = {
     "ya_sample_gaussian_device": (i32, [vp, sz, sz, ctypes.c_double, u64, u64, vp]),
     "ya_sample_zo_device": (i32, [vp, sz, sz, ctypes.c_uint32, u64, vp]),
     "ya_sample_hwt_device": (i32, [vp, sz, sz, ctypes.c_uint32, u64, vp]),
+    "ya_sample_uniform_device_many": (i32, [vp, sz, sz, sz, u64, vp]),
+    "ya_sample_gaussian_device_many": (i32, [vp, sz, sz, sz, ctypes.c_double, u64, u64, vp]),
+    "ya_sample_zo_device_many": (i32, [vp, sz, sz, sz, ctypes.c_uint32, u64, vp]),
+    "ya_poly_mul_many": (i32, [vp, vp, vp, sz, sz, sz, i32, vp]),
+    "ya_poly_add_many": (i32, [vp, vp, vp, sz, sz, sz, i32, vp]),
+    "ya_poly_sub_many": (i32, [vp, vp, vp, sz, sz, sz, i32, vp]),
+    "ya_poly_add_product_of_many": (i32, [vp, vp, vp, sz, sz, sz, i32, vp]),
+    "ya_poly_negate_many": (i32, [vp, sz, sz, sz, vp]),
     "ya_pointer_is_device": (i32, [vp, ctypes.POINTER(i32)]),
     "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
     "ya_crt_limbs": (i32, [sz, ctypes.POINTER(ctypes.c_size_t)]),

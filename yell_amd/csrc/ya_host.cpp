@@ -660,6 +660,53 @@ static int pw_common(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *
   return YA_OK;
 }
 
+// The pointwise ops over `count` polynomials back to back; b_shared: one b
+// for all.  A shared b must not overlap out (each polynomial's result would
+// race with the others' reads of b).
+static int pw_many(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree,
+                   size_t nmoduli, size_t count, int b_shared, void *stream) {
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!out || !a || (!b && op != PW_NEGATE)) return fail(YA_ERR_INVALID, "null buffer");
+  if (count == 0 || count * nmoduli > (1u << 30)) return fail(YA_ERR_INVALID, "bad count %zu", count);
+  const size_t pw = degree * nmoduli;
+  if (b && b_shared && b < out + pw * count && out < b + pw)
+    return fail(YA_ERR_INVALID, "a shared b must not overlap out");
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  const size_t per = 65535 / nmoduli;
+  for (size_t k0 = 0; k0 < count; k0 += per) {
+    const size_t kc = std::min(per, count - k0);
+    const uint64_t *bk = b ? (b_shared ? b : b + k0 * pw) : nullptr;
+    hipError_t e = launch_pointwise_many(op, out + k0 * pw, a + k0 * pw, bk, desc_ptr(st, logn, 0), degree,
+                                         nmoduli, kc, b_shared ? 0 : pw, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "pointwise launch");
+  }
+  return YA_OK;
+}
+
+int ya_poly_mul_many(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                     size_t count, int b_shared, void *stream) {
+  return pw_many(PW_MUL, out, a, b, degree, nmoduli, count, b_shared, stream);
+}
+int ya_poly_add_many(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                     size_t count, int b_shared, void *stream) {
+  return pw_many(PW_ADD, out, a, b, degree, nmoduli, count, b_shared, stream);
+}
+int ya_poly_sub_many(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
+                     size_t count, int b_shared, void *stream) {
+  return pw_many(PW_SUB, out, a, b, degree, nmoduli, count, b_shared, stream);
+}
+int ya_poly_add_product_of_many(uint64_t *acc, const uint64_t *a, const uint64_t *b, size_t degree,
+                                size_t nmoduli, size_t count, int b_shared, void *stream) {
+  return pw_many(PW_MULADD, acc, a, b, degree, nmoduli, count, b_shared, stream);
+}
+int ya_poly_negate_many(uint64_t *x, size_t degree, size_t nmoduli, size_t count, void *stream) {
+  return pw_many(PW_NEGATE, x, x, nullptr, degree, nmoduli, count, 0, stream);
+}
+
 int ya_poly_mul(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
                 void *stream) {
   return pw_common(PW_MUL, out, a, b, degree, nmoduli, stream);

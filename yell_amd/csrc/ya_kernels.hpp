@@ -59,5 +59,10 @@ enum PwOp { PW_MUL = 0, PW_ADD = 1, PW_SUB = 2, PW_NEGATE = 3, PW_MULADD = 4, PW
 //   CORRECT: out = a reduced from [0,4p) to [0,p).
 hipError_t launch_pointwise(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
                             const ModDesc *descs, size_t n, size_t nres, hipStream_t st);
+// `count` polynomials of nmod residues back to back in out and a; b's
+// polynomial k starts at k * b_stride (0: one b for all); nmod * count <= 65535.
+hipError_t launch_pointwise_many(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
+                                 const ModDesc *descs, size_t n, size_t nmod, size_t count, size_t b_stride,
+                                 hipStream_t st);
 
 }  // namespace ya

@@ -4,27 +4,31 @@
 namespace ya {
 
 // ---------------------------------------------------------------------------
-// Pointwise ops: two words per thread, residue r = blockIdx.y.
+// Pointwise ops: two words per thread.  Block row y is residue r = y % nmod
+// of polynomial y / nmod (polynomials back to back in out and a); b's
+// polynomial starts at (y / nmod) * b_stride (0: one b shared by all).
 constexpr int kPwThreads = 256;
 
 template <int OP>
 __global__ void __launch_bounds__(kPwThreads)
 k_pointwise(uint64_t *__restrict__ out, const uint64_t *__restrict__ a,
-            const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t n) {
-  const uint32_t r = blockIdx.y;
+            const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t n,
+            uint32_t nmod, uint64_t b_stride) {
+  const uint32_t y = blockIdx.y, r = y % nmod;
   const uint32_t i = (blockIdx.x * kPwThreads + threadIdx.x) * 2u;
   if (i >= n) return;
   const uint64_t p = descs[r].p, pn = descs[r].pn;
-  const size_t k = (size_t)r * n + i;
+  const size_t k = (size_t)y * n + i;
+  const size_t kb = (size_t)(y / nmod) * b_stride + (size_t)r * n + i;
   uint64_t o[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const uint64_t x = a[k + j];
-    if constexpr (OP == PW_MUL) o[j] = mulmod_barrett(x, b[k + j], p, pn);
-    else if constexpr (OP == PW_ADD) o[j] = addmod(x, b[k + j], p);
-    else if constexpr (OP == PW_SUB) o[j] = submod(x, b[k + j], p);
+    if constexpr (OP == PW_MUL) o[j] = mulmod_barrett(x, b[kb + j], p, pn);
+    else if constexpr (OP == PW_ADD) o[j] = addmod(x, b[kb + j], p);
+    else if constexpr (OP == PW_SUB) o[j] = submod(x, b[kb + j], p);
     else if constexpr (OP == PW_NEGATE) o[j] = p - x;
-    else if constexpr (OP == PW_MULADD) o[j] = muladd_barrett(out[k + j], x, b[k + j], p, pn);
+    else if constexpr (OP == PW_MULADD) o[j] = muladd_barrett(out[k + j], x, b[kb + j], p, pn);
     else o[j] = correct4p(x, p, p << 1);
   }
   out[k] = o[0];
@@ -61,22 +65,32 @@ hipError_t launch_mac(uint64_t *acc, const MacPairs &pairs, int npairs, const Mo
   return hipGetLastError();
 }
 
-hipError_t launch_pointwise(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
-                            const ModDesc *descs, size_t n, size_t nres, hipStream_t st) {
-  if (n < 2 || (n & 1) || nres == 0 || nres > 65535) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)((n / 2 + kPwThreads - 1) / kPwThreads), (unsigned)nres);
+hipError_t launch_pointwise_many(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
+                                 const ModDesc *descs, size_t n, size_t nmod, size_t count, size_t b_stride,
+                                 hipStream_t st) {
+  if (n < 2 || (n & 1) || nmod == 0 || count == 0 || nmod * count > 65535) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((n / 2 + kPwThreads - 1) / kPwThreads), (unsigned)(nmod * count));
   const dim3 block(kPwThreads);
-  const uint32_t n32 = (uint32_t)n;
+  const uint32_t n32 = (uint32_t)n, m32 = (uint32_t)nmod;
+  const uint64_t bs = b_stride;
+#define YA_PW_CASE(OP) \
+  case OP: hipLaunchKernelGGL(k_pointwise<OP>, grid, block, 0, st, out, a, b, descs, n32, m32, bs); break;
   switch (op) {
-    case PW_MUL: hipLaunchKernelGGL(k_pointwise<PW_MUL>, grid, block, 0, st, out, a, b, descs, n32); break;
-    case PW_ADD: hipLaunchKernelGGL(k_pointwise<PW_ADD>, grid, block, 0, st, out, a, b, descs, n32); break;
-    case PW_SUB: hipLaunchKernelGGL(k_pointwise<PW_SUB>, grid, block, 0, st, out, a, b, descs, n32); break;
-    case PW_NEGATE: hipLaunchKernelGGL(k_pointwise<PW_NEGATE>, grid, block, 0, st, out, a, b, descs, n32); break;
-    case PW_MULADD: hipLaunchKernelGGL(k_pointwise<PW_MULADD>, grid, block, 0, st, out, a, b, descs, n32); break;
-    case PW_CORRECT: hipLaunchKernelGGL(k_pointwise<PW_CORRECT>, grid, block, 0, st, out, a, b, descs, n32); break;
+    YA_PW_CASE(PW_MUL)
+    YA_PW_CASE(PW_ADD)
+    YA_PW_CASE(PW_SUB)
+    YA_PW_CASE(PW_NEGATE)
+    YA_PW_CASE(PW_MULADD)
+    YA_PW_CASE(PW_CORRECT)
     default: return hipErrorInvalidValue;
   }
+#undef YA_PW_CASE
   return hipGetLastError();
+}
+
+hipError_t launch_pointwise(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
+                            const ModDesc *descs, size_t n, size_t nres, hipStream_t st) {
+  return launch_pointwise_many(op, out, a, b, descs, n, nres, 1, 0, st);
 }
 
 }  // namespace ya

@@ -11,6 +11,7 @@
 // write of L x N words: HBM-write bound at the configs' sizes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -54,10 +55,14 @@ __device__ __forceinline__ void store2(W *dst, uint64_t a, uint64_t b) {
 
 // Uniform: one stream block (32 bytes = 32 / sizeof(W) words) per thread
 // iteration; a block never straddles two residues (degree >= 8).
+// Polynomial blockIdx.y of a launch over several: seed + blockIdx.y, at
+// out + blockIdx.y * words (words = nmoduli << logn).
 template <class W>
 __global__ void __launch_bounds__(kThreads) k_sample_uniform(W *__restrict__ out, uint32_t logn, uint64_t nblocks,
                                                              uint64_t seed) {
   constexpr uint32_t E = 32 / sizeof(W);
+  out += (uint64_t)blockIdx.y * nblocks * E;
+  seed += blockIdx.y;
   for (uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x; b < nblocks; b += (uint64_t)gridDim.x * kThreads) {
     const Philox4 r = stream_block(seed, b);
     const uint64_t g0 = b * E;
@@ -102,6 +107,8 @@ __global__ void __launch_bounds__(kThreads) k_sample_gaussian(W *__restrict__ ou
                                                               const uint64_t *__restrict__ cdf, uint32_t tail,
                                                               uint64_t amplifier, uint64_t seed) {
   const uint64_t npairs = 1ULL << (logn - 1);
+  out += ((uint64_t)blockIdx.y * nmod) << logn;
+  seed += blockIdx.y;
   for (uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x; b < npairs; b += (uint64_t)gridDim.x * kThreads) {
     const Philox4 r = stream_block(seed, b);
     const uint32_t k0 = cdt_search(cdf, tail, r.v[0]), k1 = cdt_search(cdf, tail, r.v[2]);
@@ -120,6 +127,8 @@ template <class W>
 __global__ void __launch_bounds__(kThreads) k_sample_zo(W *__restrict__ out, uint32_t logn, uint32_t nmod, uint32_t rho,
                                                         uint64_t seed) {
   const uint64_t nwords = 1ULL << (logn - 3);
+  out += ((uint64_t)blockIdx.y * nmod) << logn;
+  seed += blockIdx.y;
   for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nwords; j += (uint64_t)gridDim.x * kThreads) {
     const uint64_t w = stream_word(seed, j);
     for (uint32_t cm = 0; cm < nmod; ++cm) {
@@ -275,40 +284,62 @@ int launched(const char *what) {
   return e == hipSuccess ? YA_OK : hip_fail(e, what);
 }
 
+// Polynomials are launched in chunks of at most 65535 (grid.y).
+constexpr size_t kMaxPolysPerLaunch = 65535;
+
+int check_count(size_t count, size_t nmoduli) {
+  if (count == 0 || count * nmoduli > (1u << 30)) return fail(YA_ERR_INVALID, "bad count %zu", count);
+  return YA_OK;
+}
+
 template <class W>
-int dev_uniform(W *out, size_t degree, size_t nmoduli, uint64_t seed, void *stream) {
+int dev_uniform(W *out, size_t degree, size_t nmoduli, size_t count, uint64_t seed, void *stream) {
   int logn;
   int rc = dev_shape(out, degree, nmoduli, &logn);
+  if (!rc) rc = check_count(count, nmoduli);
   if (rc) return rc;
   const uint64_t nblocks = (uint64_t)degree * nmoduli * sizeof(W) / 32;
-  hipLaunchKernelGGL(k_sample_uniform<W>, dim3(grid_for(nblocks)), dim3(kThreads), 0, (hipStream_t)stream, out,
-                     (uint32_t)logn, nblocks, seed);
+  for (size_t k0 = 0; k0 < count; k0 += kMaxPolysPerLaunch) {
+    const size_t kc = std::min(kMaxPolysPerLaunch, count - k0);
+    hipLaunchKernelGGL(k_sample_uniform<W>, dim3(grid_for(nblocks), (unsigned)kc), dim3(kThreads), 0,
+                       (hipStream_t)stream, out + k0 * degree * nmoduli, (uint32_t)logn, nblocks, seed + k0);
+  }
   return launched("uniform sampler launch");
 }
 
 template <class W>
-int dev_gaussian(W *out, size_t degree, size_t nmoduli, double sigma, uint64_t amplifier, uint64_t seed,
-                 void *stream) {
+int dev_gaussian(W *out, size_t degree, size_t nmoduli, size_t count, double sigma, uint64_t amplifier,
+                 uint64_t seed, void *stream) {
   int logn;
   int rc = dev_shape(out, degree, nmoduli, &logn);
+  if (!rc) rc = check_count(count, nmoduli);
   if (rc) return rc;
   const uint64_t *cdf;
   uint32_t tail;
   rc = device_cdt(sigma, &cdf, &tail);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_sample_gaussian<W>, dim3(grid_for(degree / 2)), dim3(kThreads), 0, (hipStream_t)stream, out,
-                     (uint32_t)logn, (uint32_t)nmoduli, cdf, tail, amplifier, seed);
+  for (size_t k0 = 0; k0 < count; k0 += kMaxPolysPerLaunch) {
+    const size_t kc = std::min(kMaxPolysPerLaunch, count - k0);
+    hipLaunchKernelGGL(k_sample_gaussian<W>, dim3(grid_for(degree / 2), (unsigned)kc), dim3(kThreads), 0,
+                       (hipStream_t)stream, out + k0 * degree * nmoduli, (uint32_t)logn, (uint32_t)nmoduli, cdf,
+                       tail, amplifier, seed + k0);
+  }
   return launched("gaussian sampler launch");
 }
 
 template <class W>
-int dev_zo(W *out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, void *stream) {
+int dev_zo(W *out, size_t degree, size_t nmoduli, size_t count, uint32_t rho, uint64_t seed, void *stream) {
   if (rho > 255) return fail(YA_ERR_INVALID, "rho %u > 255", rho);
   int logn;
   int rc = dev_shape(out, degree, nmoduli, &logn);
+  if (!rc) rc = check_count(count, nmoduli);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_sample_zo<W>, dim3(grid_for(degree / 8)), dim3(kThreads), 0, (hipStream_t)stream, out,
-                     (uint32_t)logn, (uint32_t)nmoduli, rho, seed);
+  for (size_t k0 = 0; k0 < count; k0 += kMaxPolysPerLaunch) {
+    const size_t kc = std::min(kMaxPolysPerLaunch, count - k0);
+    hipLaunchKernelGGL(k_sample_zo<W>, dim3(grid_for(degree / 8), (unsigned)kc), dim3(kThreads), 0,
+                       (hipStream_t)stream, out + k0 * degree * nmoduli, (uint32_t)logn, (uint32_t)nmoduli, rho,
+                       seed + k0);
+  }
   return launched("zo sampler launch");
 }
 
@@ -347,28 +378,40 @@ using namespace ya;
 extern "C" {
 
 int ya_sample_uniform_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint64_t seed, void *stream) {
-  return dev_uniform(d_out, degree, nmoduli, seed, stream);
+  return dev_uniform(d_out, degree, nmoduli, 1, seed, stream);
 }
 int ya_sample_gaussian_device(uint64_t *d_out, size_t degree, size_t nmoduli, double sigma, uint64_t amplifier,
                               uint64_t seed, void *stream) {
-  return dev_gaussian(d_out, degree, nmoduli, sigma, amplifier, seed, stream);
+  return dev_gaussian(d_out, degree, nmoduli, 1, sigma, amplifier, seed, stream);
 }
 int ya_sample_zo_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, void *stream) {
-  return dev_zo(d_out, degree, nmoduli, rho, seed, stream);
+  return dev_zo(d_out, degree, nmoduli, 1, rho, seed, stream);
 }
 int ya_sample_hwt_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed, void *stream) {
   return dev_hwt(d_out, degree, nmoduli, hwt, seed, stream);
 }
+int ya_sample_uniform_device_many(uint64_t *d_out, size_t degree, size_t nmoduli, size_t count, uint64_t seed,
+                                  void *stream) {
+  return dev_uniform(d_out, degree, nmoduli, count, seed, stream);
+}
+int ya_sample_gaussian_device_many(uint64_t *d_out, size_t degree, size_t nmoduli, size_t count, double sigma,
+                                   uint64_t amplifier, uint64_t seed, void *stream) {
+  return dev_gaussian(d_out, degree, nmoduli, count, sigma, amplifier, seed, stream);
+}
+int ya_sample_zo_device_many(uint64_t *d_out, size_t degree, size_t nmoduli, size_t count, uint32_t rho,
+                             uint64_t seed, void *stream) {
+  return dev_zo(d_out, degree, nmoduli, count, rho, seed, stream);
+}
 
 int ya32_sample_uniform_device(uint32_t *d_out, size_t degree, size_t nmoduli, uint64_t seed, void *stream) {
-  return dev_uniform(d_out, degree, nmoduli, seed, stream);
+  return dev_uniform(d_out, degree, nmoduli, 1, seed, stream);
 }
 int ya32_sample_gaussian_device(uint32_t *d_out, size_t degree, size_t nmoduli, double sigma, uint64_t amplifier,
                                 uint64_t seed, void *stream) {
-  return dev_gaussian(d_out, degree, nmoduli, sigma, amplifier, seed, stream);
+  return dev_gaussian(d_out, degree, nmoduli, 1, sigma, amplifier, seed, stream);
 }
 int ya32_sample_zo_device(uint32_t *d_out, size_t degree, size_t nmoduli, uint32_t rho, uint64_t seed, void *stream) {
-  return dev_zo(d_out, degree, nmoduli, rho, seed, stream);
+  return dev_zo(d_out, degree, nmoduli, 1, rho, seed, stream);
 }
 int ya32_sample_hwt_device(uint32_t *d_out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed, void *stream) {
   return dev_hwt(d_out, degree, nmoduli, hwt, seed, stream);

@@ -9,6 +9,11 @@ add_product_of, and the negated public key holds p where the product was 0.
 Buffers are DeviceBuffer objects of degree * nmoduli words; every call is
 stream-ordered and allocates nothing, so an encryption can be captured into a
 graph (ya_graph_begin / ya_graph_end) once its calls have run once.
+
+encrypt_many / decrypt_many (64-bit build) run `count` messages under one key
+with one launch per step (the *_many transforms, samplers and pointwise
+forms, the key a shared operand); message k gets the words encrypt / decrypt
+give it with seeds + k.
 """
 import numpy as np
 
@@ -24,8 +29,8 @@ class Rlwe:
         self.hwt, self.sigma, self.rho = hwt, sigma, rho
         self.nbytes = degree * nmoduli * np.dtype(self.dt).itemsize
 
-    def buffer(self):
-        return capi.DeviceBuffer(self.n * self.L, self.dt)
+    def buffer(self, count=1):
+        return capi.DeviceBuffer(self.n * self.L * count, self.dt)
 
     def _op(self, fn, *bufs):
         capi.call(self.pre + fn, *[b.ptr for b in bufs], self.n, self.L, self.stream)
@@ -60,3 +65,31 @@ class Rlwe:
         capi.call("ya_memcpy_dtod", plain.ptr, ct_bx.ptr, self.nbytes, self.stream)  # :55
         self._op("poly_add_product_of", plain, sx, ct_ax)     # :56
         self._op("poly_backward", plain)                      # :57
+
+    # ---- many messages under one key, one launch per step (64-bit build) ----
+    def _many(self, fn, *bufs, count, shared=None):
+        extra = () if shared is None else (shared,)
+        capi.call("ya_poly_" + fn, *[b.ptr for b in bufs], self.n, self.L, count, *extra, self.stream)
+
+    def encrypt_many(self, ct_bx, ct_ax, u, msgs, pk_bx, pk_ax, count, seeds):
+        """encrypt() of messages 0..count-1 (polynomials back to back in msgs)
+        with seeds (seeds[0] + k, seeds[1] + k, seeds[2] + k)."""
+        assert self.word == 64
+        n, L, st = self.n, self.L, self.stream
+        capi.call("ya_sample_zo_device_many", u.ptr, n, L, count, self.rho, seeds[0], st)
+        self._many("forward_many", u, count=count)
+        capi.call("ya_sample_gaussian_device_many", ct_bx.ptr, n, L, count, self.sigma, 1, seeds[1], st)
+        self._many("add_many", ct_bx, ct_bx, msgs, count=count, shared=0)
+        self._many("forward_lazy_many", ct_bx, count=count)
+        self._many("add_product_of_many", ct_bx, u, pk_bx, count=count, shared=1)
+        capi.call("ya_sample_gaussian_device_many", ct_ax.ptr, n, L, count, self.sigma, 1, seeds[2], st)
+        self._many("forward_lazy_many", ct_ax, count=count)
+        self._many("add_product_of_many", ct_ax, u, pk_ax, count=count, shared=1)
+
+    def decrypt_many(self, plain, ct_bx, ct_ax, sx, count):
+        """decrypt() of ciphertexts 0..count-1; sx * ct_ax is taken as
+        ct_ax * sx (the Barrett muladd of x * y + acc is symmetric in x, y)."""
+        assert self.word == 64
+        capi.call("ya_memcpy_dtod", plain.ptr, ct_bx.ptr, self.nbytes * count, self.stream)
+        self._many("add_product_of_many", plain, ct_ax, sx, count=count, shared=1)
+        self._many("backward_many", plain, count=count)
