@@ -396,6 +396,11 @@ def rlwe_bench(stream, ev, batch=256, reps=5):
     step()
     capi.call("ya_stream_synchronize", stream)
     eager_ms = timed(ev, stream, step, 1)
+    capi.call("ya_pair_forward", 0)  # one residue per workgroup in the transforms, for comparison
+    step()
+    eager_unpaired_ms = timed(ev, stream, step, 1)
+    capi.call("ya_pair_forward", 1)
+    step()
     capi.call("ya_graph_begin", stream)
     step()
     g = ctypes.c_void_p()
@@ -410,7 +415,8 @@ def rlwe_bench(stream, ev, batch=256, reps=5):
     for b in (sx, bx, ax, msgs, cb, ca, u, pl):
         b.close()
     return {"degree": n, "nmoduli": L, "hwt": 64, "sigma": 3.2, "batch": batch, "keygen_us": keygen_us,
-            "eager_step_ms": eager_ms, "graph_step_ms": ms, "value": batch / (ms / 1e3),
+            "eager_step_ms": eager_ms, "eager_step_unpaired_forward_ms": eager_unpaired_ms,
+            "graph_step_ms": ms, "value": batch / (ms / 1e3),
             "unit": "encrypt+decrypt/s", "launches_per_step": 12, "_first": first}
 
 

@@ -36,6 +36,44 @@ def test_many_transforms(logn):
         np.testing.assert_array_equal(got[s], orc.poly_backward(y[s], n, L))
 
 
+@pytest.mark.parametrize("logn", [12, 13, 14])
+@pytest.mark.parametrize("le", [3, 4])
+def test_paired_forward_many(logn, le):
+    """The paired forward (two same-prime residues per workgroup) against the
+    oracle per polynomial, for 2, 3 and 5 polynomials (the odd last one runs
+    alone), lazy and corrected, inputs up to 4p - 1; and the same words with
+    pairing off."""
+    if le == 3 and logn == 14:
+        pytest.skip("radix-8 passes stop at 2^13")
+    lib = device()
+    n, L = 1 << logn, 3
+    old = lib.ya_tune_get(0, logn)
+    assert lib.ya_tune_set(0, logn, le) == 0
+    try:
+        rng = np.random.default_rng(logn * 10 + le)
+        for count in (2, 3, 5):
+            x = np.concatenate([rand_bound(rng, n, L, 4, P) for _ in range(count)])
+            x[:3] = [4 * P[0] - 1, 0, 2 * P[0]]
+            for fn, ref in (("ya_poly_forward_lazy_many", orc.poly_forward_lazy),
+                            ("ya_poly_forward_many", orc.poly_forward)):
+                say("paired forward", logn, le, count, fn)
+                outs = []
+                for pair in (1, 0):
+                    capi.call("ya_pair_forward", pair)
+                    d = capi.DeviceBuffer.from_host(x)
+                    capi.call(fn, d.ptr, n, L, count, None)
+                    outs.append(d.download())
+                    d.close()
+                capi.call("ya_pair_forward", 1)
+                np.testing.assert_array_equal(outs[0], outs[1])
+                for k in range(count):
+                    s = slice(k * n * L, (k + 1) * n * L)
+                    np.testing.assert_array_equal(outs[0][s], ref(x[s], n, L), err_msg="poly %d" % k)
+    finally:
+        lib.ya_tune_set(0, logn, old)
+        capi.call("ya_pair_forward", 1)
+
+
 @pytest.mark.parametrize("npairs", [1, 3, 17])
 def test_add_products(npairs):
     device()

@@ -516,6 +516,10 @@ int ya_graph_destroy(void *graph) {
 }
 
 // ---- NTT ------------------------------------------------------------------
+// Paired forward transforms in the many-polynomial forms (on by default;
+// ya_pair_forward(0) runs one residue per workgroup, for comparison).
+static std::atomic<int> g_pair_forward{1};
+
 static int ntt_common(uint64_t *x, size_t degree, size_t cm0, size_t nres, size_t count, int dir,
                       void *stream) {
   int logn;
@@ -529,8 +533,19 @@ static int ntt_common(uint64_t *x, size_t degree, size_t cm0, size_t nres, size_
   const ModDesc *d = desc_ptr(st, logn, cm0);
   hipError_t e;
   const size_t total = nres * count;
-  if (dir == 2) e = launch_backward(x, d, logn, total, nres, tuned_shape(logn, 0), (hipStream_t)stream);
-  else e = launch_forward(x, d, logn, total, nres, dir == 1, tuned_shape(logn, 0), (hipStream_t)stream);
+  const NttShape sh = tuned_shape(logn, 0);
+  if (dir == 2) {
+    e = launch_backward(x, d, logn, total, nres, sh, (hipStream_t)stream);
+  } else if (count >= 2 && g_pair_forward.load() && forward_pairs_ok(logn, sh)) {
+    // same-prime residues of polynomials 2q, 2q+1 share a workgroup; an odd
+    // last polynomial runs on its own
+    const size_t np = count / 2;
+    e = launch_forward_pairs(x, d, logn, np, nres, dir == 1, sh, (hipStream_t)stream);
+    if (e == hipSuccess && (count & 1))
+      e = launch_forward(x + 2 * np * nres * degree, d, logn, nres, nres, dir == 1, sh, (hipStream_t)stream);
+  } else {
+    e = launch_forward(x, d, logn, total, nres, dir == 1, sh, (hipStream_t)stream);
+  }
   return e == hipSuccess ? YA_OK : hip_fail(e, "NTT launch");
 }
 
@@ -551,6 +566,10 @@ int ya_poly_forward_lazy(uint64_t *poly, size_t degree, size_t nmoduli, void *st
 }
 int ya_poly_backward(uint64_t *poly, size_t degree, size_t nmoduli, void *stream) {
   return ntt_common(poly, degree, 0, nmoduli, 1, 2, stream);
+}
+int ya_pair_forward(int on) {
+  g_pair_forward.store(on ? 1 : 0);
+  return YA_OK;
 }
 int ya_poly_forward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream) {
   return ntt_common(polys, degree, 0, nmoduli, count, 0, stream);

@@ -26,6 +26,16 @@ struct NttShape {
 // for one polynomial; nmod = L for nres / L polynomials of L residues).
 hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                           bool lazy, NttShape shape, hipStream_t st);
+// Forward transforms of polynomials 0 .. 2*npairs-1 of a many-polynomial
+// buffer (nmod residues each), two same-prime residues per workgroup; degrees
+// 2^kMinPairLog .. 2^kMaxBlockLog with radix 8 / 16 passes (shape.le <= 4).
+// Measured on MI355X: 3-6% faster than one residue per workgroup at 2^13 and
+// 2^14 (LDS-bound there), 9-19% slower at 2^10..2^12 (the single kernel runs
+// more waves per SIMD in fewer registers).
+constexpr int kMinPairLog = 13;
+hipError_t launch_forward_pairs(uint64_t *data, const ModDesc *descs, int logn, size_t npairs, size_t nmod,
+                                bool lazy, NttShape shape, hipStream_t st);
+bool forward_pairs_ok(int logn, NttShape shape);
 hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                            NttShape shape, hipStream_t st);
 

@@ -22,6 +22,34 @@ k_fwd_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint
   gmem_store<0, LE>(base, x, t);
 }
 
+// Residue r of polynomials 2q and 2q+1 of a many-polynomial launch (same
+// prime) in one workgroup: the dual pass sequence of the fused kernel, each
+// twiddle loaded once for both, twice the residues in flight per LDS buffer.
+// The same words as k_fwd_block on each.  Prime-major order over the XCDs.
+template <int LB, int LE, bool LAZY>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_fwd_block_pair(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod,
+                 uint32_t npairs) {
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
+  const uint32_t r = v / npairs, q = v % npairs;
+  const ModDesc d = descs[r];
+  uint64_t *px = data + ((size_t)(2 * q * nmod + r) << LB);
+  uint64_t *py = px + ((size_t)nmod << LB);
+  const uint32_t t = threadIdx.x;
+  uint64_t x[1 << LE], y[1 << LE];
+  constexpr int W0 = FirstWindow<LB, LE>::value;
+  gmem_load<W0, LE>(px, x, t);
+  gmem_load<W0, LE>(py, y, t);
+  fwd_block_dual<LB, LE>(x, y, lds, d.fwd, LB, 0u, d.p, t);
+  if constexpr (!LAZY) {
+    correct_all<LE>(x, d.p);
+    correct_all<LE>(y, d.p);
+  }
+  gmem_store<0, LE>(px, x, t);
+  gmem_store<0, LE>(py, y, t);
+}
+
 template <int LB, int LE, int TOP>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_inv_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nlog,
@@ -89,6 +117,25 @@ hipError_t fwd_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, s
 }
 
 template <int LB, int LE>
+hipError_t fwd_pair_lb(uint64_t *data, const ModDesc *descs, size_t npairs, size_t nmod, bool lazy,
+                       hipStream_t st) {
+  if constexpr (LB >= kMinPairLog && LE <= 4) {
+    const dim3 grid((unsigned)(npairs * nmod));
+    const dim3 block(1u << (LB - LE));
+    if (lazy) {
+      hipLaunchKernelGGL((k_fwd_block_pair<LB, LE, true>), grid, block, 0, st, data, descs, (uint32_t)nmod,
+                         (uint32_t)npairs);
+    } else {
+      hipLaunchKernelGGL((k_fwd_block_pair<LB, LE, false>), grid, block, 0, st, data, descs, (uint32_t)nmod,
+                         (uint32_t)npairs);
+    }
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
+template <int LB, int LE>
 hipError_t inv_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                   bool lazy, hipStream_t st) {
   const dim3 grid((unsigned)(nres << (logn - LB)));
@@ -125,6 +172,21 @@ hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t
 #define YA_FWD_CALL(LB, LE) fwd_lb<LB, LE>(data, descs, logn, nres, nmod, lazy, st)
   YA_LB_SWITCH(lb, le, YA_FWD_CALL)
 #undef YA_FWD_CALL
+}
+
+bool forward_pairs_ok(int logn, NttShape shape) {
+  return logn >= kMinPairLog && logn <= kMaxBlockLog && clamp_le(logn, shape.le) <= 4;
+}
+
+hipError_t launch_forward_pairs(uint64_t *data, const ModDesc *descs, int logn, size_t npairs, size_t nmod,
+                                bool lazy, NttShape shape, hipStream_t st) {
+  if (logn < kMinPairLog || logn > kMaxBlockLog || npairs == 0 || nmod == 0) return hipErrorInvalidValue;
+  if (npairs * nmod > 0x7fffffffu >> (logn - 3)) return hipErrorInvalidValue;  // grid in work-items
+  const int le = clamp_le(logn, shape.le);
+  if (le > 4) return hipErrorInvalidValue;
+#define YA_FWD_PAIR_CALL(LB, LE) fwd_pair_lb<LB, LE>(data, descs, npairs, nmod, lazy, st)
+  YA_LB_SWITCH(logn, le, YA_FWD_PAIR_CALL)
+#undef YA_FWD_PAIR_CALL
 }
 
 static hipError_t backward_impl(uint64_t *data, const ModDesc *descs, int logn, size_t nres,

@@ -57,9 +57,13 @@ __host__ __device__ __forceinline__ Philox4 stream_block(uint64_t seed, uint64_t
   return philox4x64_10(b, 0, 0, 0, seed, 0);
 }
 
-// 64-bit word j of the main stream.
+// 64-bit word j of the main stream (selects rather than a variable index,
+// which on the device would move the block out of registers).
 __host__ __device__ __forceinline__ uint64_t stream_word(uint64_t seed, uint64_t j) {
-  return stream_block(seed, j >> 2).v[j & 3];
+  const Philox4 r = stream_block(seed, j >> 2);
+  const uint32_t k = (uint32_t)(j & 3);
+  const uint64_t lo = (k & 1) ? r.v[1] : r.v[0], hi = (k & 1) ? r.v[3] : r.v[2];
+  return (k & 2) ? hi : lo;
 }
 
 // Reservoir step k of the hwt sampler: its own block.
