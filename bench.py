@@ -30,9 +30,11 @@ METRIC = "bit-exact poly-mults/s at N=2^14,L=8 batched vs plain; p50/p99 served 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes one poly-multiplication of the batched step moves, from the
 # FETCH_SIZE / WRITE_SIZE counter passes (calibrated against a 256 MiB copy)
-# in profiles/k_polymul_tab_dual_n16384_l8.md: 1.1535e9 B read + 5.369e8 B
-# written per launch of 512 poly-mults.
-TRAFFIC_BYTES_PER_POLYMUL = 3301576.0
+# in profiles/k_polymul_tab_dual_n16384_l8.md, per launch of 512 poly-mults:
+# k_polymul (the direct-addressed kernel of a plan over back-to-back
+# polynomials) 1.1584e9 B read + 5.369e8 B written; k_polymul_tab (the
+# residue-table kernel) 1.1535e9 + 5.369e8.
+TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3311411.0, "k_polymul_tab<14,4,dual>": 3301576.0}
 
 
 def log(*a):
@@ -545,6 +547,10 @@ def main():
                       for k in range(B)])
     form = ctypes.c_int()
     capi.call("ya_batch_plan_info", plan, None, None, None, ctypes.byref(form))
+    strided = ctypes.c_int()
+    capi.call("ya_batch_plan_is_strided", plan, ctypes.byref(strided))
+    step_kernel = ("k_polymul<14,4,dual>" if strided.value else "k_polymul_tab<14,4,dual>") \
+        if form.value == 2 else "k_batch_polymul"
 
     def batched():
         capi.call("ya_batch_plan_run_graph", plan, stream)
@@ -560,6 +566,15 @@ def main():
     for _ in range(args.warmup):
         batched()
     capi.call("ya_stream_synchronize", stream)
+    # settle: after the host-side input generation the GPU has idled for
+    # seconds and its clocks ramp back up over ~0.1 s of work; keep stepping
+    # (untimed) until 0.2 s of batched steps have run
+    settle = 0
+    t_settle = time.perf_counter()
+    while time.perf_counter() - t_settle < 0.2:
+        batched()
+        capi.call("ya_stream_synchronize", stream)
+        settle += 1
     log("timing batched:", args.steps, "steps")
     t0 = time.perf_counter()
     ms_step = timed(ev, stream, batched, args.steps)
@@ -581,7 +596,7 @@ def main():
     mm = modmuls_per_polymul(n, L)
     out = {
         "metric": METRIC, "value": value, "unit": "poly-mults/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "steps": args.steps, "warmup": args.warmup, "settle_steps": settle, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic (seeded uniform x Gaussian sigma=3.2 operands)",
         "config": {"workload": "RNS poly-multiplication N=16384 L=8 (P[0..7]), a uniform, "
@@ -595,9 +610,11 @@ def main():
         "grid_form": {"value": B / (ms_grid / 1e3), "ms_per_step": ms_grid,
                       "kernel": "k_polymul<14,4>, one workgroup per residue"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_POLYMUL * B,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES_PER_POLYMUL[step_kernel] * B
+                     if step_kernel in TRAFFIC_BYTES_PER_POLYMUL else None,
                      "traffic_source": "profiles/k_polymul_tab_dual_n16384_l8.md",
-                     "kernel": "k_polymul_tab<14,4>" if form.value == 2 else "k_batch_polymul",
+                     "kernel": step_kernel,
                      "bytes_per_polymul": bytes_per,
                      "polymuls_per_launch": B},
         "modmul_rate": {"modmuls_per_polymul": mm,

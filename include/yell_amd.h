@@ -226,6 +226,13 @@ int ya_batch_plan_info(const ya_batch_plan *plan, size_t *items, size_t *residue
 /* Forces the launch form of plans created afterwards (0 = automatic; GRID
  * and SPLIT only where the batch allows them). */
 int ya_batch_force_form(int form);
+/* Plans created while on (default) run a grid-form batch of back-to-back
+ * polynomials of one shape (job k at a_0 + k*N*L, likewise b and c, c apart
+ * from a and b) with the fused kernel addressing residues directly instead of
+ * through the residue table; same words.  0: always the residue table. */
+int ya_batch_strided(int on);
+/* *strided = 1 if the plan runs the direct-addressed path described above. */
+int ya_batch_plan_is_strided(const ya_batch_plan *plan, int *strided);
 int ya_batch_plan_destroy(ya_batch_plan *plan);
 /* One-shot: create, run on `stream`, wait, destroy. */
 int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);

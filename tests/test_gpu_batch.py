@@ -169,3 +169,41 @@ def test_both_launch_forms_of_a_uniform_batch(form):
         bt.close()
     finally:
         capi.call("ya_batch_force_form", 0)
+
+
+@pytest.mark.parametrize("logn,L,count", [(14, 8, 128), (12, 3, 200), (10, 1, 600)])
+def test_contiguous_plan_strided_and_table(logn, L, count):
+    """Back-to-back jobs of one shape: the direct-addressed path and the
+    residue-table path give the same words, and both the oracle's (first and
+    last job), over two graph replays."""
+    device()
+    n = 1 << logn
+    w = n * L
+    say("contiguous plan", logn, L, count)
+    a = np.concatenate([capi.sample_uniform(n, L, 10 + k) for k in range(count)])
+    b = np.concatenate([capi.sample_gaussian(n, L, 3.2, 5000 + k) for k in range(count)])
+    da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
+    outs = []
+    for strided in (1, 0):
+        capi.call("ya_batch_strided", strided)
+        dc = capi.DeviceBuffer(w * count)
+        arr = (capi.PolymulJob * count)(*[capi.PolymulJob(da.ptr + 8 * k * w, db.ptr + 8 * k * w,
+                                                          dc.ptr + 8 * k * w, n, L) for k in range(count)])
+        plan = ctypes.c_void_p()
+        capi.call("ya_batch_plan_create", ctypes.byref(plan), ctypes.cast(arr, ctypes.c_void_p), count)
+        form = ctypes.c_int()
+        capi.call("ya_batch_plan_info", plan, None, None, None, ctypes.byref(form))
+        for _ in range(2):
+            capi.call("ya_batch_plan_run_graph", plan, None)
+        capi.call("ya_device_synchronize")
+        outs.append((form.value, dc.download()))
+        capi.load().ya_batch_plan_destroy(plan)
+        dc.close()
+    capi.call("ya_batch_strided", 1)
+    assert outs[0][0] == outs[1][0]
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    for k in (0, count - 1):
+        s = slice(k * w, (k + 1) * w)
+        np.testing.assert_array_equal(outs[0][1][s], orc.polymul(a[s], b[s], n, L))
+    da.close()
+    db.close()

@@ -67,6 +67,8 @@ _SIGS = {
     "ya_poly_multiply": (i32, [vp, vp, vp, sz, sz, vp]),
     "ya_poly_forward_many": (i32, [vp, sz, sz, sz, vp]),
     "ya_pair_forward": (i32, [i32]),
+    "ya_batch_strided": (i32, [i32]),
+    "ya_batch_plan_is_strided": (i32, [vp, ctypes.POINTER(i32)]),
     "ya_poly_forward_lazy_many": (i32, [vp, sz, sz, sz, vp]),
     "ya_poly_backward_many": (i32, [vp, sz, sz, sz, vp]),
     "ya_poly_add_products": (i32, [vp, vp, vp, sz, sz, sz, vp]),

@@ -212,11 +212,19 @@ struct ya_batch_plan {
   int grid_logn = 0;               // degree class of the grid and split forms
   int split_k = 0;                 // split form: 2^k workgroups per residue
   uint64_t *d_split = nullptr;     // split form: scratch [nres][2^grid_logn]
+  // grid form over back-to-back polynomials of one shape: the fused kernel
+  // addresses residues from its launch index (no residue-table load)
+  bool strided = false;
+  const uint64_t *sa = nullptr, *sb = nullptr;
+  uint64_t *sc = nullptr;
+  size_t s_nmod = 0;
+  const ModDesc *s_desc = nullptr;
 };
 
 namespace {
 
 std::atomic<int> g_form_override{0};
+std::atomic<int> g_strided{1};  // ya_batch_strided(0): always the residue table (tests, comparison)
 
 int plan_free(ya_batch_plan *pl) {
   if (!pl) return YA_OK;
@@ -381,6 +389,26 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
     }
     pl->split_k = k;
   }
+  if (form == YA_BATCH_GRID) {
+    // jobs k = 0.. at a_0 + k*N*L (and likewise b, c), one shape, c apart
+    // from a and b: the direct-addressed fused kernel (measured ~5% faster
+    // than the residue-table kernel at N = 2^14, L = 8 on MI355X)
+    const size_t w = jobs[0].degree * jobs[0].nmoduli, all = w * njobs;
+    bool strided = true;
+    for (size_t j = 0; j < njobs && strided; ++j)
+      strided = jobs[j].degree == jobs[0].degree && jobs[j].nmoduli == jobs[0].nmoduli &&
+                jobs[j].a == jobs[0].a + j * w && jobs[j].b == jobs[0].b + j * w && jobs[j].c == jobs[0].c + j * w;
+    const uint64_t *c0 = jobs[0].c;
+    strided = strided && !(c0 < jobs[0].a + all && jobs[0].a < c0 + all) && !(c0 < jobs[0].b + all && jobs[0].b < c0 + all);
+    if (strided && g_strided.load()) {
+      pl->strided = true;
+      pl->sa = jobs[0].a;
+      pl->sb = jobs[0].b;
+      pl->sc = jobs[0].c;
+      pl->s_nmod = jobs[0].nmoduli;
+      pl->s_desc = desc_ptr(st, lg, 0);
+    }
+  }
   pl->form = form;
   pl->grid_logn = lg;
   *out = pl;
@@ -392,6 +420,9 @@ static hipError_t plan_launch(ya_batch_plan *pl, hipStream_t s) {
     const SplitArgs sa{pl->d_res, nullptr, nullptr, nullptr, nullptr, 0, pl->d_split};
     return launch_polymul_split(sa, pl->grid_logn, pl->nres, pl->split_k, s);
   }
+  if (pl->form == YA_BATCH_GRID && pl->strided)
+    return launch_polymul(pl->sc, pl->sa, pl->sb, pl->s_desc, pl->grid_logn, pl->nres, pl->s_nmod,
+                          tuned_shape(pl->grid_logn, 1), s);
   if (pl->form == YA_BATCH_GRID)
     return launch_polymul_table(pl->d_res, pl->nres, pl->grid_logn, tuned_shape(pl->grid_logn, 1), s);
   return launch_batch(pl->d_items, pl->segs, pl->d_res, pl->d_q, pl->d_scratch, pl->grid, s);
@@ -437,6 +468,17 @@ int ya_batch_plan_info(const ya_batch_plan *pl, size_t *items, size_t *residues,
                   : pl->form == YA_BATCH_SPLIT ? pl->nres << pl->split_k
                                                : pl->grid;
   if (form) *form = pl->form;
+  return YA_OK;
+}
+
+int ya_batch_plan_is_strided(const ya_batch_plan *pl, int *strided) {
+  if (!pl || !strided) return fail(YA_ERR_INVALID, "null argument");
+  *strided = pl->strided ? 1 : 0;
+  return YA_OK;
+}
+
+int ya_batch_strided(int on) {
+  g_strided.store(on ? 1 : 0);
   return YA_OK;
 }
 
