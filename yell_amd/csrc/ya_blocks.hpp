@@ -24,6 +24,20 @@ __device__ __forceinline__ void static_for(F &&f) {
   }
 }
 
+// A block's base address passed through the scalar unit as an opaque
+// wave-uniform value.  When the compiler sees the address as kernel argument
+// plus an offset from blockIdx it rebuilds the per-word addresses in vector
+// registers, and the 64-bit multiply chains then lose their zero-extended
+// register pairs to extra v_mov copies (the fused C2 kernel: 11,845 VALU
+// instructions per wave instead of 10,534).
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T *)(((uint64_t)hi << 32) | lo);
+}
+
 // XCD-aware order: workgroups b, b+8, b+16, ... run on one XCD (the hardware
 // deals consecutive workgroups to consecutive XCDs); give each XCD a
 // contiguous range [xcd*g/8, ...) of the g work units.

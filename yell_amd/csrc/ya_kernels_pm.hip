@@ -59,7 +59,8 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
   const uint32_t res = poly * nmod + cm;
   const ModDesc d = descs[cm];
   const size_t off = (size_t)res << LB;
-  polymul_residue<LB, LE, DUAL>(c + off, a + off, b + off, d, lds, threadIdx.x);
+  polymul_residue<LB, LE, DUAL>(uniform_ptr(c + off), uniform_ptr(a + off), uniform_ptr(b + off), d, lds,
+                                threadIdx.x);
 }
 
 // One residue per workgroup from a residue table (a batch plan of one degree
