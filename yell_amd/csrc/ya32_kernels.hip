@@ -42,7 +42,7 @@ k32_inv(uint32_t *__restrict__ data, const ModDesc32 *__restrict__ descs, uint32
 
 // Fused c = backward(forward(a) (*) forward(b)) of one residue per workgroup;
 // residue order prime-major over the XCDs, as the 64-bit k_polymul.
-template <int LB, int LE>
+template <int LB, int LE, bool DUAL>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k32_polymul(uint32_t *__restrict__ c, const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
             const ModDesc32 *__restrict__ descs, uint32_t nmod) {
@@ -54,22 +54,29 @@ k32_polymul(uint32_t *__restrict__ c, const uint32_t *__restrict__ a, const uint
   const uint32_t res = poly * nmod + cm;
   const ModDesc32 d = descs[cm];
   const size_t off = (size_t)res << LB;
+  uint32_t *cr = uniform_ptr(c + off);
+  const uint32_t *ar = uniform_ptr(a + off), *br = uniform_ptr(b + off);
   const uint32_t t = threadIdx.x;
   uint32_t x[1 << LE], y[1 << LE];
   constexpr int W0 = FirstWindow<LB, LE>::value;
-  gmem_load<W0, LE>(a + off, x, t);
-  gmem_load<W0, LE>(b + off, y, t);
-  fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd, LB, 0u, d.p, t);
-  gmem_store<0, LE>(c + off, x, t);  // park forward(a) at this thread's own words
+  gmem_load<W0, LE>(ar, x, t);
+  gmem_load<W0, LE>(br, y, t);
+  if constexpr (DUAL) {
+    // both forward transforms in one pass sequence (shared twiddles)
+    fwd_block_dual<LB, LE>(x, y, lds, d.fwd, LB, 0u, d.p, t);
+  } else {
+    fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd, LB, 0u, d.p, t);
+    gmem_store<0, LE>(cr, x, t);  // park forward(a) at this thread's own words
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) x[e] = y[e];
-  const uint32_t t2 = phase_v(t);
-  fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
-  gmem_load<0, LE>(c + off, y, t2);
+    for (int e = 0; e < (1 << LE); ++e) x[e] = y[e];
+    const uint32_t t2 = phase_v(t);
+    fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+    gmem_load<0, LE>(cr, y, t2);
+  }
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);
-  inv_block<LB, LE, TOP_MERGED>(x, c + off, lds, d.inv, LB, 0u, d, t3);
+  inv_block<LB, LE, TOP_MERGED>(x, cr, lds, d.inv, LB, 0u, d, t3);
 }
 
 // Element-wise operations over residue-major words (include/yell/ops.hpp:7-41,
@@ -134,11 +141,13 @@ hipError_t inv32(uint32_t *data, const ModDesc32 *descs, size_t nres, size_t nmo
   return hipGetLastError();
 }
 
+// The dual-forward form from blocks of 2^10 on (as the 64-bit build); below,
+// forward(a) parks in c.
 template <int LB, int LE>
 hipError_t pm32(uint32_t *c, const uint32_t *a, const uint32_t *b, const ModDesc32 *descs, size_t nres,
                 size_t nmod, hipStream_t st) {
-  hipLaunchKernelGGL((k32_polymul<LB, LE>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c, a, b, descs,
-                     (uint32_t)nmod);
+  hipLaunchKernelGGL((k32_polymul<LB, LE, (LB >= 10)>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c, a,
+                     b, descs, (uint32_t)nmod);
   return hipGetLastError();
 }
 
