@@ -30,11 +30,11 @@ METRIC = "bit-exact poly-mults/s at N=2^14,L=8 batched vs plain; p50/p99 served 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes one poly-multiplication of the batched step moves, from the
 # FETCH_SIZE / WRITE_SIZE counter passes (calibrated against a 256 MiB copy)
-# in profiles/k_polymul_tab_dual_n16384_l8.md, per launch of 512 poly-mults:
+# in profiles/k_polymul_dual_n16384_l8.md, per launch of 512 poly-mults:
 # k_polymul (the direct-addressed kernel of a plan over back-to-back
-# polynomials) 1.1584e9 B read + 5.369e8 B written; k_polymul_tab (the
+# polynomials) 1.1539e9 B read + 5.369e8 B written; k_polymul_tab (the
 # residue-table kernel) 1.1535e9 + 5.369e8.
-TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3311411.0, "k_polymul_tab<14,4,dual>": 3301576.0}
+TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3302192.0, "k_polymul_tab<14,4,dual>": 3301576.0}
 
 
 def log(*a):
@@ -613,7 +613,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": TRAFFIC_BYTES_PER_POLYMUL[step_kernel] * B
                      if step_kernel in TRAFFIC_BYTES_PER_POLYMUL else None,
-                     "traffic_source": "profiles/k_polymul_tab_dual_n16384_l8.md",
+                     "traffic_source": "profiles/k_polymul_dual_n16384_l8.md",
                      "kernel": step_kernel,
                      "bytes_per_polymul": bytes_per,
                      "polymuls_per_launch": B},
