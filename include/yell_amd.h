@@ -158,7 +158,10 @@ int ya_poly_forward_lazy_many(uint64_t *polys, size_t degree, size_t nmoduli, si
                               void *stream);
 int ya_poly_backward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream);
 /* The pointwise operations over `count` polynomials stored back to back (as
- * the *_many transforms): polynomial k of out is op(a_k, b_k), with b_k = b
+ * the *_many transforms; per polynomial the reference's operator* / + / - /
+ * add_product_of / negate, include/yell/ops.hpp:7-41,
+ * include/yell/details/poly.hpp:119-132, 152-158, applied polynomial after
+ * polynomial): polynomial k of out is op(a_k, b_k), with b_k = b
  * itself when b_shared != 0 (one operand for all, e.g. a key), else the k-th
  * polynomial of b.  Same words as `count` calls of the single forms; out may
  * alias a; a shared b must not overlap out.  add_product_of_many:
