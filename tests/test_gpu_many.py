@@ -68,3 +68,24 @@ def test_in_place_and_negate_many():
     assert lib.ya_poly_mul_many(da.ptr, da.ptr, db.ptr, n, L, 0, 0, None) == 1              # count 0
     da.close()
     db.close()
+
+
+def test_pointwise_many_chunks():
+    """count * nmoduli beyond one launch's 65,535 grid rows: 70 polynomials of
+    1000 residues (the launches are cut at 65 polynomials)."""
+    device()
+    n, L, count = 8, 1000, 70
+    rng = np.random.default_rng(70)
+    P = [O.prime(j) for j in range(L)]
+    a = np.concatenate([rand_residues(rng, n, L, P) for _ in range(count)])
+    b = rand_residues(rng, n, L, P)
+    acc = np.concatenate([rand_residues(rng, n, L, P, 4) for _ in range(count)])
+    da, db, dc = (capi.DeviceBuffer.from_host(x) for x in (a, b, acc))
+    say("pointwise many chunks")
+    capi.call("ya_poly_add_product_of_many", dc.ptr, da.ptr, db.ptr, n, L, count, 1, None)
+    got = dc.download().reshape(count, n * L)
+    w = n * L
+    for k in (0, 64, 65, count - 1):
+        np.testing.assert_array_equal(got[k], O.poly_add_product_of(acc[k * w:(k + 1) * w], a[k * w:(k + 1) * w], b, n, L))
+    for d in (da, db, dc):
+        d.close()

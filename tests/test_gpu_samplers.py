@@ -73,3 +73,37 @@ def test_device_sampler_errors():
     assert lib.ya_sample_uniform_device(buf.ptr, 12, 1, 1, None) == 1
     assert lib.ya32_sample_uniform_device(buf.ptr, 1 << 16, 1, 1, None) == 1
     buf.close()
+
+
+def test_many_chunks_and_extremes():
+    """More polynomials than one launch's grid rows (65,535) in one call; the
+    largest modulus count; the largest 64-bit degree; 32-bit at 291 moduli."""
+    lib = device()
+    say("sampler chunks")
+    n, count = 8, 70000
+    buf = capi.DeviceBuffer(n * count)
+    capi.call("ya_sample_zo_device_many", buf.ptr, n, 1, count, 0x7F, 3, None)
+    got = buf.download().reshape(count, n)
+    for k in (0, 65534, 65535, 65536, count - 1):
+        np.testing.assert_array_equal(got[k], capi.sample_host("zo", n, 1, 0x7F, 3 + k, 64))
+    buf.close()
+    say("sampler 1000 moduli")
+    n, L = 16, 1000
+    buf = capi.DeviceBuffer(n * L)
+    for kind, param in (("uniform", None), ("gaussian", 3.2), ("hwt", 16)):
+        capi.sample_device(kind, buf, n, L, param, 9)
+        np.testing.assert_array_equal(buf.download(), capi.sample_host(kind, n, L, param, 9, 64))
+    buf.close()
+    say("sampler 2^17")
+    n = 1 << 17
+    buf = capi.DeviceBuffer(n * 2)
+    for kind, param in (("uniform", None), ("hwt", 1000), ("zo", 200)):
+        capi.sample_device(kind, buf, n, 2, param, 10)
+        np.testing.assert_array_equal(buf.download(), capi.sample_host(kind, n, 2, param, 10, 64))
+    buf.close()
+    say("sampler 32-bit 291 moduli")
+    buf = capi.DeviceBuffer(64 * 291, np.uint32)
+    capi.sample_device("uniform", buf, 64, 291, None, 11, word=32)
+    np.testing.assert_array_equal(buf.download(), capi.sample_host("uniform", 64, 291, None, 11, 32))
+    buf.close()
+    assert lib.ya_sample_uniform_device_many(buf.ptr, 8, 1, 0, 1, None) == 1   # count 0
