@@ -190,28 +190,40 @@ int launch_slot(ya_server *s, int k) {
   hipError_t e = hipSuccess;
   if (has15 && !sl.d_scratch) e = hipMalloc(&sl.d_scratch, ((size_t)s->cus << 15) * 8);
   if (e != hipSuccess) return fail(YA_ERR_ALLOC, "slot scratch: %s", hipGetErrorString(e));
-  std::memcpy(sl.h_items, items.data(), items.size() * sizeof(BatchItem));
-  std::memcpy(sl.h_res, res.data(), res.size() * sizeof(BatchRes));
+  bool one_class = true;
+  for (const auto &it : items) one_class &= it.logn == items[0].logn;
+  const int lg = (int)items[0].logn;
+  const int split_k = one_class ? split_factor(s->st, lg, res.size()) : 0;
+  // a lone request on the split path addresses its residues directly (its a,
+  // b, c are contiguous in the arena), so no residue table goes up
+  const bool direct = split_k && jobs.size() == 1;
+  if (e == hipSuccess) e = hipEventRecord(sl.t0, sl.stream);
   for (size_t j = 0; j < jobs.size() && e == hipSuccess; ++j) {
     const Request &r = *sl.reqs[j];
-    e = hipMemcpyAsync((void *)jobs[j].a, r.a, r.words * 8, hipMemcpyHostToDevice, sl.stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync((void *)jobs[j].b, r.b, r.words * 8, hipMemcpyHostToDevice, sl.stream);
+    if (r.b == r.a + r.words) {  // a and b adjacent on the host too: one copy (b follows a in the arena)
+      e = hipMemcpyAsync((void *)jobs[j].a, r.a, 2 * r.words * 8, hipMemcpyHostToDevice, sl.stream);
+    } else {
+      e = hipMemcpyAsync((void *)jobs[j].a, r.a, r.words * 8, hipMemcpyHostToDevice, sl.stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync((void *)jobs[j].b, r.b, r.words * 8, hipMemcpyHostToDevice, sl.stream);
+    }
   }
-  if (e == hipSuccess) e = hipEventRecord(sl.t0, sl.stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(sl.d_items, sl.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, sl.stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(sl.d_res, sl.h_res, res.size() * sizeof(BatchRes), hipMemcpyHostToDevice, sl.stream);
+  if (!direct) {
+    std::memcpy(sl.h_items, items.data(), items.size() * sizeof(BatchItem));
+    std::memcpy(sl.h_res, res.data(), res.size() * sizeof(BatchRes));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sl.d_items, sl.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, sl.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sl.d_res, sl.h_res, res.size() * sizeof(BatchRes), hipMemcpyHostToDevice, sl.stream);
+  }
   if (e == hipSuccess) {
-    bool one_class = true;
-    for (const auto &it : items) one_class &= it.logn == items[0].logn;
-    const int k = one_class ? split_factor(s->st, (int)items[0].logn, res.size()) : 0;
-    if (k) {
+    if (split_k) {
       // few residues: spread each over 2^k workgroups; forward(b)'s top layers
       // go in place into the arena copy of b, which the server owns
-      const SplitArgs sa{sl.d_res, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
-      e = launch_polymul_split(sa, (int)items[0].logn, res.size(), k, sl.stream);
+      const SplitArgs sa = direct ? SplitArgs{nullptr, jobs[0].a, jobs[0].b, jobs[0].c, desc_ptr(s->st, lg, 0),
+                                              (uint32_t)jobs[0].nmoduli, nullptr}
+                                  : SplitArgs{sl.d_res, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
+      e = launch_polymul_split(sa, lg, res.size(), split_k, sl.stream);
     } else {
       const unsigned grid = (unsigned)std::min<size_t>(s->cus, items.size());
       e = launch_batch(sl.d_items, segs, sl.d_res, sl.d_q, sl.d_scratch, grid, sl.stream);
@@ -226,12 +238,28 @@ int launch_slot(ya_server *s, int k) {
   return e == hipSuccess ? YA_OK : hip_fail(e, "server launch");
 }
 
+// Threads that would block (the dispatcher, the completion thread, a
+// waiting client) first poll for up to kSpinUs, yielding: waking a blocked
+// thread costs tens of microseconds, as much as a small request's kernels.
+constexpr int kSpinUs = 500;
+
+template <class Pred>
+void spin_then_wait(std::unique_lock<std::mutex> &lk, std::condition_variable &cv, Pred pred) {
+  const auto until = clk::now() + std::chrono::microseconds(kSpinUs);
+  while (!pred() && clk::now() < until) {
+    lk.unlock();
+    std::this_thread::yield();
+    lk.lock();
+  }
+  cv.wait(lk, pred);
+}
+
 void dispatch_loop(ya_server *s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     int k = -1;
-    s->cv_dispatch.wait(lk, [&] {
+    spin_then_wait(lk, s->cv_dispatch, [&] {
       if (s->stop) return true;
       if (s->pending.empty()) return false;
       for (size_t i = 0; i < s->slots.size(); ++i)
@@ -289,7 +317,7 @@ void complete_loop(ya_server *s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    s->cv_done.wait(lk, [&] { return s->stop || !s->inflight.empty(); });
+    spin_then_wait(lk, s->cv_done, [&] { return s->stop || !s->inflight.empty(); });
     if (s->inflight.empty() && s->stop) return;
     std::vector<int> ready;
     std::vector<int> snapshot(s->inflight.begin(), s->inflight.end());
@@ -299,7 +327,7 @@ void complete_loop(ya_server *s) {
       if (q == hipSuccess) ready.push_back(k);
       else if (q != hipErrorNotReady) ready.push_back(-1 - k);  // failed
     }
-    if (ready.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    if (ready.empty()) std::this_thread::yield();  // a batch is in flight: poll its event again
     lk.lock();
     const auto now = clk::now();
     for (int code : ready) {
@@ -459,7 +487,7 @@ int ya_server_wait(ya_server *s, uint64_t ticket, int timeout_ms, double *latenc
   auto r = it->second;
   auto pred = [&] { return r->state >= 2; };
   if (timeout_ms < 0) {
-    s->cv_done.wait(lk, pred);
+    spin_then_wait(lk, s->cv_done, pred);
   } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred)) {
     return fail(YA_ERR_TIMEOUT, "ticket %llu not done after %d ms", (unsigned long long)ticket, timeout_ms);
   }
