@@ -179,24 +179,37 @@ int launch_slot(ya_server *s, int k) {
     uint64_t *base = s->arena_base + r->off;
     jobs.push_back(ya_polymul_job{base, base + r->words, base + 2 * r->words, r->degree, r->nmoduli});
   }
+  // A lone request whose launch takes the split path addresses its residues
+  // directly (its a, b, c are contiguous in the arena): no batch tables are
+  // built or uploaded.
+  int split_k = 0, lg = 0;
+  bool direct = false;
+  int rc;
+  if (jobs.size() == 1) {
+    lg = log2_degree(jobs[0].degree);
+    if (lg >= kBatchMinLog && lg <= kBatchMaxLog) {
+      if ((rc = ensure_tables(s->st, lg, 0, jobs[0].nmoduli))) return rc;
+      split_k = split_factor(s->st, lg, jobs[0].nmoduli);
+      direct = split_k != 0;
+    }
+  }
   std::vector<BatchItem> items;
   std::vector<BatchRes> res;
   bool has15 = false;
   BatchSegs segs{};
-  int rc = build_batch(s->st, jobs.data(), jobs.size(), items, res, &segs, &has15);
-  if (rc) return rc;
-  if (items.size() > sl.cap_items || res.size() > sl.cap_res)
-    return fail(YA_ERR_INVALID, "batch exceeds slot capacity");
   hipError_t e = hipSuccess;
-  if (has15 && !sl.d_scratch) e = hipMalloc(&sl.d_scratch, ((size_t)s->cus << 15) * 8);
-  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "slot scratch: %s", hipGetErrorString(e));
-  bool one_class = true;
-  for (const auto &it : items) one_class &= it.logn == items[0].logn;
-  const int lg = (int)items[0].logn;
-  const int split_k = one_class ? split_factor(s->st, lg, res.size()) : 0;
-  // a lone request on the split path addresses its residues directly (its a,
-  // b, c are contiguous in the arena), so no residue table goes up
-  const bool direct = split_k && jobs.size() == 1;
+  if (!direct) {
+    if ((rc = build_batch(s->st, jobs.data(), jobs.size(), items, res, &segs, &has15))) return rc;
+    if (items.size() > sl.cap_items || res.size() > sl.cap_res)
+      return fail(YA_ERR_INVALID, "batch exceeds slot capacity");
+    if (has15 && !sl.d_scratch) e = hipMalloc(&sl.d_scratch, ((size_t)s->cus << 15) * 8);
+    if (e != hipSuccess) return fail(YA_ERR_ALLOC, "slot scratch: %s", hipGetErrorString(e));
+    bool one_class = true;
+    for (const auto &it : items) one_class &= it.logn == items[0].logn;
+    lg = (int)items[0].logn;
+    split_k = one_class ? split_factor(s->st, lg, res.size()) : 0;
+  }
+  const size_t nres = direct ? jobs[0].nmoduli : res.size();
   if (e == hipSuccess) e = hipEventRecord(sl.t0, sl.stream);
   for (size_t j = 0; j < jobs.size() && e == hipSuccess; ++j) {
     const Request &r = *sl.reqs[j];
@@ -223,7 +236,7 @@ int launch_slot(ya_server *s, int k) {
       const SplitArgs sa = direct ? SplitArgs{nullptr, jobs[0].a, jobs[0].b, jobs[0].c, desc_ptr(s->st, lg, 0),
                                               (uint32_t)jobs[0].nmoduli, nullptr}
                                   : SplitArgs{sl.d_res, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
-      e = launch_polymul_split(sa, lg, res.size(), split_k, sl.stream);
+      e = launch_polymul_split(sa, lg, nres, split_k, sl.stream);
     } else {
       const unsigned grid = (unsigned)std::min<size_t>(s->cus, items.size());
       e = launch_batch(sl.d_items, segs, sl.d_res, sl.d_q, sl.d_scratch, grid, sl.stream);
