@@ -71,21 +71,6 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
   inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, d.inv, LB, 0u, d, t3);
 }
 
-// Top layer (index bit 14) of a 2^15 residue over global memory, src -> dst.
-__device__ __forceinline__ void top15_fwd(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
-                                          const ModDesc &d, uint32_t t) {
-  const Tw w = load_tw(d.fwd + 1);  // (N >> 15) + 0
-  const PrimeK k(d.p);
-#pragma unroll 4
-  for (uint32_t q = 0; q < 16; ++q) {
-    const uint32_t i = t + q * kBatchThreads;
-    uint64_t x0 = src[i], x1 = src[i + (1u << 14)];
-    ct_butterfly(x0, x1, w, k);
-    dst[i] = x0;
-    dst[i + (1u << 14)] = x1;
-  }
-}
-
 __device__ __forceinline__ void top15_inv_merged(uint64_t *__restrict__ c, const ModDesc &d,
                                                  uint32_t t) {
   const uint64_t p = d.p, p2 = p << 1;
@@ -98,26 +83,45 @@ __device__ __forceinline__ void top15_inv_merged(uint64_t *__restrict__ c, const
   }
 }
 
+// Top layer (index bit 14) of both operands of a 2^15 poly-multiplication
+// over global memory, one twiddle: a -> da, b -> db.
+__device__ __forceinline__ void top15_fwd2(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
+                                           uint64_t *__restrict__ da, uint64_t *__restrict__ db,
+                                           const ModDesc &d, uint32_t t) {
+  const Tw w = load_tw(d.fwd + 1);  // (N >> 15) + 0
+  const PrimeK k(d.p);
+#pragma unroll 4
+  for (uint32_t q = 0; q < 16; ++q) {
+    const uint32_t i = t + q * kBatchThreads;
+    uint64_t x0 = a[i], x1 = a[i + (1u << 14)], y0 = b[i], y1 = b[i + (1u << 14)];
+    ct_butterfly(x0, x1, w, k);
+    ct_butterfly(y0, y1, w, k);
+    da[i] = x0;
+    da[i + (1u << 14)] = x1;
+    db[i] = y0;
+    db[i + (1u << 14)] = y1;
+  }
+}
+
 // One 2^15 residue; `scr` is this workgroup's private 2^15-word scratch.
+// Bit 14 of both operands over global memory (a -> c, b -> scr), then per
+// 2^14 half the remaining forward layers of both at once (shared twiddles),
+// the product and the inverse layers below bit 14, written to c's half; last
+// the merged top inverse layer over global memory.
 __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__restrict__ res,
                                          uint64_t *lds, uint64_t *__restrict__ scr, uint32_t t) {
   constexpr int LE = 4;
+  constexpr int W0 = FirstWindow<14, LE>::value;
   const BatchRes r = res[it.first];
   const ModDesc d = load_desc(r.d);
   uint64_t x[1 << LE], y[1 << LE];
-  top15_fwd(r.a, scr, d, t);  // forward(a): bit 14, then each half
+  top15_fwd2(r.a, r.b, r.c, scr, d, t);
   __syncthreads();
   for (uint32_t h = 0; h < 2; ++h) {
     const uint32_t th = phase_v(t);
-    fwd_block<14, LE>(x, scr + (h << 14), lds, d.fwd + phase_zero_s(), 15, h << 14, d.p, th);
-    gmem_store<0, LE>(scr + (h << 14), x, th);
-  }
-  top15_fwd(r.b, r.c, d, t);  // forward(b) into c
-  __syncthreads();
-  for (uint32_t h = 0; h < 2; ++h) {
-    const uint32_t th = phase_v(t);
-    fwd_block<14, LE>(x, r.c + (h << 14), lds, d.fwd + phase_zero_s(), 15, h << 14, d.p, th);
-    gmem_load<0, LE>(scr + (h << 14), y, th);
+    gmem_load<W0, LE>(r.c + (h << 14), x, th);
+    gmem_load<W0, LE>(scr + (h << 14), y, th);
+    fwd_block_dual<14, LE>(x, y, lds, d.fwd + phase_zero_s(), 15, h << 14, d.p, th);
 #pragma unroll
     for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
     const uint32_t ti = phase_v(t);
