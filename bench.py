@@ -496,7 +496,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch, probes and tuner")
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[4000.0, 12000.0],
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[4000.0, 8000.0, 12000.0],
                     help="offered loads (requests/s) of the served-form runs")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--copy-calib-mb", type=int, default=0,

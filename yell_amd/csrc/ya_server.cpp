@@ -118,6 +118,9 @@ struct ya_server {
   std::unordered_map<uint64_t, std::shared_ptr<Request>> live;
   std::deque<int> inflight;  // slot indices in launch order
   bool stop = false;
+  // bumped wherever a waiting thread is notified, so that spinning threads
+  // can watch it without taking `mu`
+  std::atomic<uint64_t> epoch{0};
   uint64_t next_id = 1;
   int error = YA_OK;
 
@@ -141,6 +144,7 @@ int server_free(ya_server *s) {
     std::lock_guard<std::mutex> g(s->mu);
     s->stop = true;
   }
+  s->epoch.fetch_add(1, std::memory_order_release);
   s->cv_dispatch.notify_all();
   s->cv_done.notify_all();
   if (s->th_dispatch.joinable()) s->th_dispatch.join();
@@ -150,6 +154,7 @@ int server_free(ya_server *s) {
     for (auto &r : s->pending) r->state = 3;
     s->pending.clear();
   }
+  s->epoch.fetch_add(1, std::memory_order_release);
   s->cv_done.notify_all();
   (void)hipSetDevice(s->device);
   for (auto &sl : s->slots) {
@@ -257,22 +262,28 @@ int launch_slot(ya_server *s, int k) {
 constexpr int kSpinUs = 500;
 
 template <class Pred>
-void spin_then_wait(std::unique_lock<std::mutex> &lk, std::condition_variable &cv, Pred pred) {
+void spin_then_wait(std::unique_lock<std::mutex> &lk, std::condition_variable &cv,
+                    const std::atomic<uint64_t> &epoch, Pred pred) {
   const auto until = clk::now() + std::chrono::microseconds(kSpinUs);
-  while (!pred() && clk::now() < until) {
-    lk.unlock();
-    std::this_thread::yield();
+  while (!pred()) {
+    if (clk::now() >= until) {
+      cv.wait(lk, pred);
+      return;
+    }
+    const uint64_t seen = epoch.load(std::memory_order_acquire);
+    lk.unlock();  // poll the epoch, not the mutex
+    while (epoch.load(std::memory_order_acquire) == seen && clk::now() < until) std::this_thread::yield();
     lk.lock();
   }
-  cv.wait(lk, pred);
 }
+
 
 void dispatch_loop(ya_server *s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     int k = -1;
-    spin_then_wait(lk, s->cv_dispatch, [&] {
+    spin_then_wait(lk, s->cv_dispatch, s->epoch, [&] {
       if (s->stop) return true;
       if (s->pending.empty()) return false;
       for (size_t i = 0; i < s->slots.size(); ++i)
@@ -318,10 +329,12 @@ void dispatch_loop(ya_server *s) {
       }
       sl.reqs.clear();
       sl.busy = false;
+      s->epoch.fetch_add(1, std::memory_order_release);
       s->cv_done.notify_all();
       continue;
     }
     s->inflight.push_back(k);
+    s->epoch.fetch_add(1, std::memory_order_release);
     s->cv_done.notify_all();  // wake the completion thread
   }
 }
@@ -330,7 +343,7 @@ void complete_loop(ya_server *s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    spin_then_wait(lk, s->cv_done, [&] { return s->stop || !s->inflight.empty(); });
+    spin_then_wait(lk, s->cv_done, s->epoch, [&] { return s->stop || !s->inflight.empty(); });
     if (s->inflight.empty() && s->stop) return;
     std::vector<int> ready;
     std::vector<int> snapshot(s->inflight.begin(), s->inflight.end());
@@ -372,6 +385,7 @@ void complete_loop(ya_server *s) {
       s->inflight.erase(std::find(s->inflight.begin(), s->inflight.end(), k));
     }
     if (!ready.empty()) {
+      s->epoch.fetch_add(1, std::memory_order_release);
       s->cv_dispatch.notify_all();
       s->cv_done.notify_all();
     }
@@ -488,6 +502,7 @@ int ya_server_submit(ya_server *s, const uint64_t *a, const uint64_t *b, uint64_
   s->submitted++;
   s->max_pending = std::max<uint64_t>(s->max_pending, s->pending.size());
   *ticket = r->id;
+  s->epoch.fetch_add(1, std::memory_order_release);
   s->cv_dispatch.notify_one();
   return YA_OK;
 }
@@ -500,7 +515,7 @@ int ya_server_wait(ya_server *s, uint64_t ticket, int timeout_ms, double *latenc
   auto r = it->second;
   auto pred = [&] { return r->state >= 2; };
   if (timeout_ms < 0) {
-    spin_then_wait(lk, s->cv_done, pred);
+    spin_then_wait(lk, s->cv_done, s->epoch, pred);
   } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred)) {
     return fail(YA_ERR_TIMEOUT, "ticket %llu not done after %d ms", (unsigned long long)ticket, timeout_ms);
   }
@@ -649,33 +664,44 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   ya_server_reset_stats(s);
   std::vector<uint64_t> tickets(arr.size(), 0);
   std::vector<int> status(arr.size(), -1);  // -1 not yet submitted
+  std::vector<uint8_t> retired(arr.size(), 0);
   std::mutex wmu;
   std::condition_variable wcv;
   size_t submitted_upto = 0, retired_upto = 0;  // ring entry i % ring is free once i < retired_upto + ring
-  uint64_t mismatches = 0, failures = 0, backpressure = 0;
-  std::thread waiter([&]() {
-    for (size_t i = 0; i < arr.size(); ++i) {
-      int st;
-      {
-        std::unique_lock<std::mutex> lk(wmu);
-        wcv.wait(lk, [&] { return submitted_upto > i; });
-        st = status[i];
-      }
-      if (st == YA_OK) {
-        if (ya_server_wait(s, tickets[i], -1, nullptr) != YA_OK) {
-          ++failures;
-        } else if (spec->verify) {
-          const Shape &sh = shapes[arr[i].shape];
-          if (std::memcmp(results + (i % ring) * max_words, sh.want.data() + (size_t)arr[i].pair * sh.words,
-                          sh.words * 8) != 0)
-            ++mismatches;
+  std::atomic<uint64_t> mismatches{0}, failures{0};
+  uint64_t backpressure = 0;
+  // kWaiters threads take results (waiter w: requests w, w + kWaiters, ...),
+  // verify them (a 1 MiB compare per C2 request: one thread alone would limit
+  // the stream near 8 K req/s) and retire their ring entries; the ring
+  // advances over the contiguous prefix of retired requests.
+  constexpr size_t kWaiters = 4;
+  std::vector<std::thread> waiters;
+  for (size_t w = 0; w < kWaiters; ++w) {
+    waiters.emplace_back([&, w]() {
+      for (size_t i = w; i < arr.size(); i += kWaiters) {
+        int st;
+        {
+          std::unique_lock<std::mutex> lk(wmu);
+          wcv.wait(lk, [&] { return submitted_upto > i; });
+          st = status[i];
         }
+        if (st == YA_OK) {
+          if (ya_server_wait(s, tickets[i], -1, nullptr) != YA_OK) {
+            ++failures;
+          } else if (spec->verify) {
+            const Shape &sh = shapes[arr[i].shape];
+            if (std::memcmp(results + (i % ring) * max_words, sh.want.data() + (size_t)arr[i].pair * sh.words,
+                            sh.words * 8) != 0)
+              ++mismatches;
+          }
+        }
+        std::lock_guard<std::mutex> lk(wmu);
+        retired[i] = 1;
+        while (retired_upto < arr.size() && retired[retired_upto]) ++retired_upto;
+        wcv.notify_all();
       }
-      std::lock_guard<std::mutex> lk(wmu);
-      retired_upto = i + 1;
-      wcv.notify_all();
-    }
-  });
+    });
+  }
   const auto t0 = clk::now();
   for (size_t i = 0; i < arr.size(); ++i) {
     const auto due = t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(arr[i].t));
@@ -698,7 +724,7 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
     submitted_upto = i + 1;
     wcv.notify_all();
   }
-  waiter.join();
+  for (auto &t : waiters) t.join();
   const double wall = std::chrono::duration<double>(clk::now() - t0).count();
   ya_server_stats st;
   ya_server_get_stats(s, &st);
