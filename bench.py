@@ -32,9 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE counter passes (calibrated against a 256 MiB copy)
 # in profiles/k_polymul_dual_n16384_l8.md, per launch of 512 poly-mults:
 # k_polymul (the direct-addressed kernel of a plan over back-to-back
-# polynomials) 1.1539e9 B read + 5.369e8 B written; k_polymul_tab (the
+# polynomials) 1.1443e9 B read + 5.369e8 B written; k_polymul_tab (the
 # residue-table kernel) 1.1535e9 + 5.369e8.
-TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3302192.0, "k_polymul_tab<14,4,dual>": 3301576.0}
+TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3283596.0, "k_polymul_tab<14,4,dual>": 3301576.0}
 
 
 def log(*a):

@@ -28,13 +28,50 @@ __device__ __forceinline__ uint64_t mul32x32(uint32_t a, uint32_t b) {
   return (uint64_t)a * (uint64_t)b;
 }
 
-// High 64 bits of the 128-bit product a * b.
+// ---- Three short assembly sequences.  The compiler neither exposes the
+// carry-out of v_mad_u64_u32 nor chains a product into a full 64-bit addend
+// (it zero-extends 32-bit halves into fresh register pairs, one v_mov each),
+// and it rewrites x >= m ? x - m : x into a compare, two masked selects of m
+// and a borrow chain.  Operands constrained "s" are placed in a scalar
+// register where the value is wave-uniform and in a vector register where it
+// is not; both are legal VOP3 sources.  The sequences write no memory and
+// read nothing but their operands.
+
+// a * b + c (mod 2^64) and the carry out of bit 63 (0 or 1).  The select
+// that reads the carry mask waits two cycles after the multiply wrote it
+// (VALU-written SGPR read as a lane mask).
+__device__ __forceinline__ uint64_t mad64_carry(uint32_t a, uint32_t b, uint64_t c, uint32_t &carry) {
+  uint64_t d, mask;
+  asm("v_mad_u64_u32 %0, %2, %3, %4, %5\n\ts_nop 1\n\tv_cndmask_b32_e64 %1, 0, 1, %2"
+      : "=v"(d), "=v"(carry), "=s"(mask)
+      : "v"(a), "s"(b), "v"(c));
+  return d;
+}
+
+// a * b + c (mod 2^64) with c read as one 64-bit register pair.
+__device__ __forceinline__ uint64_t mad64_wrap(uint32_t a, uint32_t b, uint64_t c) {
+  uint64_t mask;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(c), "=s"(mask) : "v"(a), "s"(b));
+  return c;
+}
+
+// x >= m ? x - m : x, with nm = 2^64 - m: the difference as one opaque 64-bit
+// add, then one compare and two selects.
+__device__ __forceinline__ uint64_t sub_if_ge(uint64_t x, uint64_t m, uint64_t nm) {
+  uint64_t d;
+  asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(d) : "v"(x), "s"(nm));
+  return x >= m ? d : x;
+}
+
+// High 64 bits of the 128-bit product a * b: the middle column
+// a0*b1 + a1*b0 + hi32(a0*b0) is up to 2^65, so its carry out of bit 63 is
+// kept and added at bit 32 of the top product.
 __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
   const uint32_t a0 = lo32(a), a1 = hi32(a), b0 = lo32(b), b1 = hi32(b);
-  const uint64_t p00h = (uint64_t)__umulhi(a0, b0);
-  const uint64_t m1 = mul32x32(a1, b0) + p00h;        // < 2^64
-  const uint64_t m2 = mul32x32(a0, b1) + lo32(m1);    // < 2^64
-  return mul32x32(a1, b1) + hi32(m1) + hi32(m2);
+  const uint64_t m1 = mul32x32(a1, b0) + (uint64_t)__umulhi(a0, b0);  // < 2^64
+  uint32_t c;
+  const uint64_t m2 = mad64_carry(a0, b1, m1, c);
+  return mul32x32(a1, b1) + (((uint64_t)c << 32) | hi32(m2));
 }
 
 // Low 64 bits of a * b.
@@ -45,14 +82,42 @@ __device__ __forceinline__ uint64_t mullo64(uint64_t a, uint64_t b) {
   return p00 + ((uint64_t)cross << 32);
 }
 
-// 128-bit product as (hi, lo).
+// r with its high word replaced by h, written on the two 32-bit halves: the
+// equivalent (h << 32) | lo32(r), with h = hi32(r) + s, is rewritten by the
+// optimizer into the 64-bit add r + (s << 32), a zero-extension move and a
+// 64-bit add where one 32-bit add suffices.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint64_t with_hi(uint64_t r, uint32_t h) {
+  u32x2 v = __builtin_bit_cast(u32x2, r);
+  v.y = h;
+  return __builtin_bit_cast(uint64_t, v);
+}
+
+// 128-bit product as (hi, lo), the middle column as in mulhi64.
 __device__ __forceinline__ void mul128(uint64_t a, uint64_t b, uint64_t &hi, uint64_t &lo) {
   const uint32_t a0 = lo32(a), a1 = hi32(a), b0 = lo32(b), b1 = hi32(b);
   const uint64_t p00 = mul32x32(a0, b0);
   const uint64_t m1 = mul32x32(a1, b0) + hi32(p00);
-  const uint64_t m2 = mul32x32(a0, b1) + lo32(m1);
-  hi = mul32x32(a1, b1) + hi32(m1) + hi32(m2);
-  lo = ((uint64_t)lo32(m2) << 32) | lo32(p00);
+  uint32_t c;
+  const uint64_t m2 = mad64_carry(a0, b1, m1, c);
+  hi = mul32x32(a1, b1) + (((uint64_t)c << 32) | hi32(m2));
+  lo = with_hi(p00, lo32(m2));
+}
+
+// x*w - q*p (mod 2^64) as x*w + q*np with np = 2^64 - p: the two low
+// products in one multiply-add chain, the four cross products into the high
+// word.
+__device__ __forceinline__ uint64_t mul_sub_lo64(uint64_t x, uint64_t w, uint64_t q, uint64_t np) {
+  const uint32_t x0 = lo32(x), x1 = hi32(x), w0 = lo32(w), w1 = hi32(w);
+  const uint32_t q0 = lo32(q), q1 = hi32(q), n0 = lo32(np), n1 = hi32(np);
+  const uint64_t r = mad64_wrap(q0, n0, mul32x32(x0, w0));
+  return with_hi(r, hi32(r) + x0 * w1 + x1 * w0 + q0 * n1 + q1 * n0);
+}
+
+// acc + q*np (mod 2^64), acc the multiply-add's 64-bit addend.
+__device__ __forceinline__ uint64_t add_mul_lo64(uint64_t acc, uint64_t q, uint64_t np) {
+  const uint64_t r = mad64_wrap(lo32(q), lo32(np), acc);
+  return with_hi(r, hi32(r) + lo32(q) * hi32(np) + hi32(q) * lo32(np));
 }
 
 // Reference Barrett reduction of the 128-bit value (hi, lo) modulo p:
@@ -66,8 +131,9 @@ __device__ __forceinline__ uint64_t barrett_reduce(uint64_t hi, uint64_t lo, uin
   const uint64_t sh = (hi << 2) | (lo >> 62);
   const uint64_t s_lo = ql + sl;
   const uint64_t q_hi = qh + sh + (s_lo < ql ? 1u : 0u);
-  uint64_t r = lo - mullo64(q_hi, p);
-  return r >= p ? r - p : r;
+  const uint64_t np = 0 - p;
+  const uint64_t r = add_mul_lo64(lo, q_hi, np);  // lo - q_hi * p
+  return sub_if_ge(r, p, np);
 }
 
 // ops::mulmod (Barrett on the full product).
@@ -91,27 +157,28 @@ __device__ __forceinline__ uint64_t muladd_barrett(uint64_t acc, uint64_t x, uin
 // ops::mulmod_shoup: x < 4p, result in [0, p).
 __device__ __forceinline__ uint64_t mulmod_shoup(uint64_t x, uint64_t y, uint64_t ys,
                                                  uint64_t p) {
-  const uint64_t q = mulhi64(x, ys);
-  const uint64_t r = mullo64(x, y) - mullo64(q, p);
-  return r >= p ? r - p : r;
+  const uint64_t np = 0 - p;
+  const uint64_t r = mul_sub_lo64(x, y, mulhi64(x, ys), np);
+  return sub_if_ge(r, p, np);
 }
 
-// Per-prime constants of the butterflies (wave-uniform, scalar registers).
+// Per-prime constants of the butterflies (wave-uniform where the prime is).
 struct PrimeK {
   uint64_t p, p2;    // p, 2p
-  __device__ __forceinline__ explicit PrimeK(uint64_t q) : p(q), p2(q << 1) {}
+  uint64_t np, np2;  // 2^64 - p, 2^64 - 2p
+  __device__ __forceinline__ explicit PrimeK(uint64_t q)
+      : p(q), p2(q << 1), np(0 - q), np2(0 - (q << 1)) {}
 };
 
 // Shoup product without the final correction: x * w mod p in [0, 2p), the
 // word x*w - hi64(x*w')*p (mod 2^64) of the reference.
 __device__ __forceinline__ uint64_t mulmod_shoup_lazy(uint64_t x, const Tw &t, const PrimeK &k) {
-  const uint64_t q = mulhi64(x, t.ws);
-  return mullo64(x, t.w) - mullo64(q, k.p);
+  return mul_sub_lo64(x, t.w, mulhi64(x, t.ws), k.np);
 }
 
 // lib/ntt.cpp:69-83
 __device__ __forceinline__ void ct_butterfly(uint64_t &x0, uint64_t &x1, const Tw &t, const PrimeK &k) {
-  const uint64_t u0 = x0 >= k.p2 ? x0 - k.p2 : x0;
+  const uint64_t u0 = sub_if_ge(x0, k.p2, k.np2);
   const uint64_t v = mulmod_shoup_lazy(x1, t, k);
   x0 = u0 + v;
   x1 = u0 + k.p2 - v;
@@ -120,8 +187,7 @@ __device__ __forceinline__ void ct_butterfly(uint64_t &x0, uint64_t &x1, const T
 // lib/ntt.cpp:47-63
 __device__ __forceinline__ void gs_butterfly(uint64_t &x0, uint64_t &x1, const Tw &t, const PrimeK &k) {
   const uint64_t u0 = x0, u1 = x1;
-  uint64_t t0 = u0 + u1;
-  t0 = t0 >= k.p2 ? t0 - k.p2 : t0;
+  const uint64_t t0 = sub_if_ge(u0 + u1, k.p2, k.np2);
   const uint64_t t1 = u0 + k.p2 - u1;
   x0 = t0;
   x1 = mulmod_shoup_lazy(t1, t, k);
