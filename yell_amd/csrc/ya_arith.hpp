@@ -212,16 +212,23 @@ __device__ __forceinline__ uint64_t submod(uint64_t x, uint64_t y, uint64_t p) {
 // ---- 32-bit moduli build (YELL_USE_32BITS_MODULI, include/yell/params_32.hpp):
 // value_type uint32_t, gt_value_type uint64_t, primes < 2^30, so 4p < 2^32.
 // Every product is one native 32-bit multiply (v_mul_lo_u32 / v_mul_hi_u32)
-// or one v_mad_u64_u32.
+// or one v_mad_u64_u32; in the butterflies the remainder x*w - q*p (mod 2^32)
+// is the low word of the chain mad(q, 2^32 - p, mad(x, w, 0)), two
+// instructions where two low multiplies and a subtract would be three.
 
 struct __attribute__((aligned(8))) Tw32 {
   uint32_t w, ws;  // w' = floor(w * 2^32 / p)
 };
 
 struct PrimeK32 {
-  uint32_t p, p2;
-  __device__ __forceinline__ explicit PrimeK32(uint32_t q) : p(q), p2(q << 1) {}
+  uint32_t p, p2, np;  // p, 2p, 2^32 - p
+  __device__ __forceinline__ explicit PrimeK32(uint32_t q) : p(q), p2(q << 1), np(0 - q) {}
 };
+
+// x*w - q*p (mod 2^32).
+__device__ __forceinline__ uint32_t mul_sub_lo32(uint32_t x, uint32_t w, uint32_t q, uint32_t np) {
+  return lo32(mad64_wrap(q, np, mul32x32(x, w)));
+}
 
 // lib/ops.cpp:5-21 with shift = 32, delta = 32 - 30 = 2:
 // q = Pn * (x >> 32) + (x << 2) (mod 2^64); r = lo32(x) - hi32(q) * p; one -p.
@@ -249,7 +256,7 @@ __device__ __forceinline__ uint32_t mulmod_shoup(uint32_t x, uint32_t y, uint32_
 // lib/ntt.cpp:69-83 (bit_width = 32)
 __device__ __forceinline__ void ct_butterfly(uint32_t &x0, uint32_t &x1, const Tw32 &t, const PrimeK32 &k) {
   const uint32_t u0 = x0 >= k.p2 ? x0 - k.p2 : x0;
-  const uint32_t v = x1 * t.w - __umulhi(x1, t.ws) * k.p;
+  const uint32_t v = mul_sub_lo32(x1, t.w, __umulhi(x1, t.ws), k.np);
   x0 = u0 + v;
   x1 = u0 + k.p2 - v;
 }
@@ -261,7 +268,7 @@ __device__ __forceinline__ void gs_butterfly(uint32_t &x0, uint32_t &x1, const T
   t0 = t0 >= k.p2 ? t0 - k.p2 : t0;
   const uint32_t t1 = u0 + k.p2 - u1;
   x0 = t0;
-  x1 = t1 * t.w - __umulhi(t1, t.ws) * k.p;
+  x1 = mul_sub_lo32(t1, t.w, __umulhi(t1, t.ws), k.np);
 }
 
 __device__ __forceinline__ uint32_t correct4p(uint32_t v, uint32_t p, uint32_t p2) {
