@@ -6,7 +6,9 @@
 // transforms side by side: each residue gets a sub-group of 2^(LB-LE)
 // threads and its own slice of the workgroup's LDS, so a 2^10 residue and a
 // 2^14 residue keep every lane busy alike.  A 2^15 residue is one item: its
-// top layer runs over global memory and its two 2^14 halves through LDS.
+// top layer runs in registers as the words arrive, its two 2^14 halves
+// through LDS one after the other (the upper half's inputs parked in global
+// memory meanwhile).
 // Arithmetic and layer order are those of the plain kernels (the reference's
 // butterflies, lib/ntt.cpp:47-83), so each result equals ya_poly_multiply's.
 #include <algorithm>
@@ -71,64 +73,75 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
   inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, d.inv, LB, 0u, d, t3);
 }
 
-__device__ __forceinline__ void top15_inv_merged(uint64_t *__restrict__ c, const ModDesc &d,
-                                                 uint32_t t) {
-  const uint64_t p = d.p, p2 = p << 1;
-#pragma unroll 4
-  for (uint32_t q = 0; q < 16; ++q) {
-    const uint32_t i = t + q * kBatchThreads;
-    const uint64_t u = c[i], v = c[i + (1u << 14)];
-    c[i] = mulmod_shoup(u + v, d.inv_n, d.inv_n_s, p);
-    c[i + (1u << 14)] = mulmod_shoup(u - v + p2, d.inv_n_phi, d.inv_n_phi_s, p);
+// Bit 14 of one operand of a 2^15 poly-multiplication in registers: thread t
+// loads its first-window words t | e << 10 of the lower half and their
+// partners 2^14 above, keeps the lower outputs in x and parks the upper ones
+// at the same offsets in `park` (in two groups of eight, to bound registers).
+__device__ __forceinline__ void top15_fwd_keep(const uint64_t *__restrict__ src, uint64_t (&x)[16],
+                                               uint64_t *__restrict__ park, const Tw &w, const PrimeK &k,
+                                               uint32_t t) {
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    uint64_t up[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const uint32_t i = t | ((uint32_t)(g * 8 + e) << 10);
+      x[g * 8 + e] = src[i];
+      up[e] = src[i + (1u << 14)];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      ct_butterfly(x[g * 8 + e], up[e], w, k);
+      park[t | ((uint32_t)(g * 8 + e) << 10)] = up[e];
+    }
   }
 }
 
-// Top layer (index bit 14) of both operands of a 2^15 poly-multiplication
-// over global memory, one twiddle: a -> da, b -> db.
-__device__ __forceinline__ void top15_fwd2(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
-                                           uint64_t *__restrict__ da, uint64_t *__restrict__ db,
-                                           const ModDesc &d, uint32_t t) {
-  const Tw w = load_tw(d.fwd + 1);  // (N >> 15) + 0
-  const PrimeK k(d.p);
-#pragma unroll 4
-  for (uint32_t q = 0; q < 16; ++q) {
-    const uint32_t i = t + q * kBatchThreads;
-    uint64_t x0 = a[i], x1 = a[i + (1u << 14)], y0 = b[i], y1 = b[i + (1u << 14)];
-    ct_butterfly(x0, x1, w, k);
-    ct_butterfly(y0, y1, w, k);
-    da[i] = x0;
-    da[i + (1u << 14)] = x1;
-    db[i] = y0;
-    db[i + (1u << 14)] = y1;
-  }
-}
-
-// One 2^15 residue; `scr` is this workgroup's private 2^15-word scratch.
-// Bit 14 of both operands over global memory (a -> c, b -> scr), then per
-// 2^14 half the remaining forward layers of both at once (shared twiddles),
-// the product and the inverse layers below bit 14, written to c's half; last
-// the merged top inverse layer over global memory.
+// One 2^15 residue; `scr` is this workgroup's private 2^14-word scratch.
+// Bit 14 of both operands in registers (the upper outputs parked: a's in
+// c's upper half, b's in scr); the lower half's remaining forward layers of
+// both at once (shared twiddles), product and inverse layers below bit 14,
+// written to c's lower half; then the upper half from the parked words, its
+// inverse outputs kept in registers for the merged top inverse layer with
+// the lower half's words.  Every word parked in global memory is read back
+// by the thread that wrote it, at the same address (the first window of a
+// 2^14 block and the last window of its inverse are both t | e << 10).
 __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__restrict__ res,
                                          uint64_t *lds, uint64_t *__restrict__ scr, uint32_t t) {
   constexpr int LE = 4;
   constexpr int W0 = FirstWindow<14, LE>::value;
+  static_assert(W0 == 10, "first window of a 2^14 block");
   const BatchRes r = res[it.first];
   const ModDesc d = load_desc(r.d);
   uint64_t x[1 << LE], y[1 << LE];
-  top15_fwd2(r.a, r.b, r.c, scr, d, t);
-  __syncthreads();
-  for (uint32_t h = 0; h < 2; ++h) {
-    const uint32_t th = phase_v(t);
-    gmem_load<W0, LE>(r.c + (h << 14), x, th);
-    gmem_load<W0, LE>(scr + (h << 14), y, th);
-    fwd_block_dual<14, LE>(x, y, lds, d.fwd + phase_zero_s(), 15, h << 14, d.p, th);
-#pragma unroll
-    for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
-    const uint32_t ti = phase_v(t);
-    inv_block<14, LE, TOP_GS>(x, r.c + (h << 14), lds, d.inv + phase_zero_s(), 15, h << 14, d, ti);
+  {
+    const Tw w = load_tw(d.fwd + 1);  // (N >> 15) + 0
+    const PrimeK k(d.p);
+    top15_fwd_keep(r.a, x, r.c + (1u << 14), w, k, t);
+    top15_fwd_keep(r.b, y, scr, w, k, t);
   }
-  __syncthreads();
-  top15_inv_merged(r.c, d, t);
+  fwd_block_dual<14, LE>(x, y, lds, d.fwd + phase_zero_s(), 15, 0u, d.p, t);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+  const uint32_t t1 = phase_v(t);
+  inv_block<14, LE, TOP_GS>(x, r.c, lds, d.inv + phase_zero_s(), 15, 0u, d, t1);
+  const uint32_t t2 = phase_v(t);
+  gmem_load<W0, LE>(r.c + (1u << 14), x, t2);
+  gmem_load<W0, LE>(scr, y, t2);
+  fwd_block_dual<14, LE>(x, y, lds, d.fwd + phase_zero_s(), 15, 1u << 14, d.p, t2);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+  const uint32_t t3 = phase_v(t);
+  inv_block<14, LE, TOP_GS>(x, nullptr, lds, d.inv + phase_zero_s(), 15, 1u << 14, d, t3);
+  // merged top layer (include/yell/details/ntt.hpp:151-159) on words i, i + 2^14
+  const uint64_t p = d.p, p2 = p << 1;
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) {
+    const uint32_t i = t3 | ((uint32_t)e << W0);
+    const uint64_t u = r.c[i], v = x[e];
+    r.c[i] = mulmod_shoup(u + v, d.inv_n, d.inv_n_s, p);
+    r.c[i + (1u << 14)] = mulmod_shoup(u - v + p2, d.inv_n_phi, d.inv_n_phi_s, p);
+  }
 }
 
 __global__ void __launch_bounds__(kBatchThreads)
@@ -179,7 +192,7 @@ k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
       case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, t); break;
       case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, t); break;
       case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, t); break;
-      case 15 * 8 + 4: batch_15(it, res, lds, scratch + ((size_t)blockIdx.x << 15), t); break;
+      case 15 * 8 + 4: batch_15(it, res, lds, scratch + ((size_t)blockIdx.x << 14), t); break;
       default: break;  // rejected by the host
     }
     __syncthreads();  // LDS of this item is dead before the next item's first store
@@ -356,7 +369,7 @@ int ya_batch_plan_create(ya_batch_plan **out, const ya_polymul_job *jobs, size_t
   e = hipMalloc(&pl->d_items, items.size() * sizeof(BatchItem));
   if (e == hipSuccess) e = hipMalloc(&pl->d_res, res.size() * sizeof(BatchRes));
   if (e == hipSuccess) e = hipMalloc(&pl->d_q, sizeof(BatchQueue));
-  if (e == hipSuccess && has15) e = hipMalloc(&pl->d_scratch, ((size_t)pl->grid << 15) * 8);
+  if (e == hipSuccess && has15) e = hipMalloc(&pl->d_scratch, ((size_t)pl->grid << 14) * 8);
   if (e != hipSuccess) {
     plan_free(pl);
     return fail(YA_ERR_ALLOC, "batch plan allocation: %s", hipGetErrorString(e));
