@@ -74,6 +74,39 @@ def test_paired_forward_many(logn, le):
         capi.call("ya_pair_forward", 1)
 
 
+@pytest.mark.parametrize("logn", [15, 16, 17])
+def test_paired_forward_large_rings(logn):
+    """Above 2^14 the paired forward runs the top layers per residue, then
+    block b of residue r of polynomials 2q and 2q+1 in one workgroup: the
+    oracle's words per polynomial for 2 and 3 polynomials (the odd last one
+    alone), lazy and corrected, inputs up to 4p - 1, and the same words with
+    pairing off."""
+    device()
+    n, L = 1 << logn, 2
+    rng = np.random.default_rng(logn)
+    try:
+        for count in (2, 3):
+            x = np.concatenate([rand_bound(rng, n, L, 4, P) for _ in range(count)])
+            x[:3] = [4 * P[0] - 1, 0, 2 * P[0]]
+            for fn, ref in (("ya_poly_forward_lazy_many", orc.poly_forward_lazy),
+                            ("ya_poly_forward_many", orc.poly_forward)):
+                say("paired forward, large ring", logn, count, fn)
+                outs = []
+                for pair in (1, 0):
+                    capi.call("ya_pair_forward", pair)
+                    d = capi.DeviceBuffer.from_host(x)
+                    capi.call(fn, d.ptr, n, L, count, None)
+                    outs.append(d.download())
+                    d.close()
+                capi.call("ya_pair_forward", 1)
+                np.testing.assert_array_equal(outs[0], outs[1])
+                for k in range(count):
+                    s = slice(k * n * L, (k + 1) * n * L)
+                    np.testing.assert_array_equal(outs[0][s], ref(x[s], n, L), err_msg="poly %d" % k)
+    finally:
+        capi.call("ya_pair_forward", 1)
+
+
 @pytest.mark.parametrize("npairs", [1, 3, 17])
 def test_add_products(npairs):
     device()

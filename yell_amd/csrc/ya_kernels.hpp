@@ -28,7 +28,9 @@ hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t
                           bool lazy, NttShape shape, hipStream_t st);
 // Forward transforms of polynomials 0 .. 2*npairs-1 of a many-polynomial
 // buffer (nmod residues each), two same-prime residues per workgroup; degrees
-// 2^kMinPairLog .. 2^kMaxBlockLog with radix 8 / 16 passes (shape.le <= 4).
+// 2^kMinPairLog .. 2^kMaxLogN with radix 8 / 16 passes (shape.le <= 4).
+// Above 2^kMaxBlockLog the top layers run per residue first (k_fwd_top), then
+// block b of residue r of polynomials 2q and 2q+1 share a workgroup.
 // Measured on MI355X: 3-6% faster than one residue per workgroup at 2^13 and
 // 2^14 (LDS-bound there), 9-19% slower at 2^10..2^12 (the single kernel runs
 // more waves per SIMD in fewer registers).

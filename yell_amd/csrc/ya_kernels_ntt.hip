@@ -22,26 +22,31 @@ k_fwd_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint
   gmem_store<0, LE>(base, x, t);
 }
 
-// Residue r of polynomials 2q and 2q+1 of a many-polynomial launch (same
-// prime) in one workgroup: the dual pass sequence of the fused kernel, each
+// Block blk (2^LB words) of residue r of polynomials 2q and 2q+1 of a
+// many-polynomial launch in one workgroup: same prime and same block, so the
+// same twiddles, run as the dual pass sequence of the fused kernel, each
 // twiddle loaded once for both, twice the residues in flight per LDS buffer.
+// A residue of 2^nlog > 2^LB words has had its top layers done by k_fwd_top.
 // The same words as k_fwd_block on each.  Prime-major order over the XCDs.
 template <int LB, int LE, bool LAZY>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_fwd_block_pair(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod,
-                 uint32_t npairs) {
+                 uint32_t npairs, uint32_t nlog) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t nblk_log = nlog - LB;
   const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
-  const uint32_t r = v / npairs, q = v % npairs;
+  const uint32_t per_prime = npairs << nblk_log;
+  const uint32_t r = v / per_prime, rest = v % per_prime;
+  const uint32_t q = rest >> nblk_log, blk = rest & ((1u << nblk_log) - 1u);
   const ModDesc d = descs[r];
-  uint64_t *px = data + ((size_t)(2 * q * nmod + r) << LB);
-  uint64_t *py = px + ((size_t)nmod << LB);
+  uint64_t *px = data + ((size_t)(2 * q * nmod + r) << nlog) + ((size_t)blk << LB);
+  uint64_t *py = px + ((size_t)nmod << nlog);
   const uint32_t t = threadIdx.x;
   uint64_t x[1 << LE], y[1 << LE];
   constexpr int W0 = FirstWindow<LB, LE>::value;
   gmem_load<W0, LE>(px, x, t);
   gmem_load<W0, LE>(py, y, t);
-  fwd_block_dual<LB, LE>(x, y, lds, d.fwd, LB, 0u, d.p, t);
+  fwd_block_dual<LB, LE>(x, y, lds, d.fwd, nlog, blk << LB, d.p, t);
   if constexpr (!LAZY) {
     correct_all<LE>(x, d.p);
     correct_all<LE>(y, d.p);
@@ -117,17 +122,17 @@ hipError_t fwd_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, s
 }
 
 template <int LB, int LE>
-hipError_t fwd_pair_lb(uint64_t *data, const ModDesc *descs, size_t npairs, size_t nmod, bool lazy,
+hipError_t fwd_pair_lb(uint64_t *data, const ModDesc *descs, int logn, size_t npairs, size_t nmod, bool lazy,
                        hipStream_t st) {
   if constexpr (LB >= kMinPairLog && LE <= 4) {
-    const dim3 grid((unsigned)(npairs * nmod));
+    const dim3 grid((unsigned)((npairs * nmod) << (logn - LB)));
     const dim3 block(1u << (LB - LE));
     if (lazy) {
       hipLaunchKernelGGL((k_fwd_block_pair<LB, LE, true>), grid, block, 0, st, data, descs, (uint32_t)nmod,
-                         (uint32_t)npairs);
+                         (uint32_t)npairs, (uint32_t)logn);
     } else {
       hipLaunchKernelGGL((k_fwd_block_pair<LB, LE, false>), grid, block, 0, st, data, descs, (uint32_t)nmod,
-                         (uint32_t)npairs);
+                         (uint32_t)npairs, (uint32_t)logn);
     }
     return hipGetLastError();
   } else {
@@ -175,17 +180,33 @@ hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t
 }
 
 bool forward_pairs_ok(int logn, NttShape shape) {
-  return logn >= kMinPairLog && logn <= kMaxBlockLog && clamp_le(logn, shape.le) <= 4;
+  const int lb = logn < kMaxBlockLog ? logn : kMaxBlockLog;
+  return logn >= kMinPairLog && logn <= kMaxLogN && clamp_le(lb, shape.le) <= 4;
 }
 
 hipError_t launch_forward_pairs(uint64_t *data, const ModDesc *descs, int logn, size_t npairs, size_t nmod,
                                 bool lazy, NttShape shape, hipStream_t st) {
-  if (logn < kMinPairLog || logn > kMaxBlockLog || npairs == 0 || nmod == 0) return hipErrorInvalidValue;
+  if (logn < kMinPairLog || logn > kMaxLogN || npairs == 0 || nmod == 0) return hipErrorInvalidValue;
   if (npairs * nmod > 0x7fffffffu >> (logn - 3)) return hipErrorInvalidValue;  // grid in work-items
-  const int le = clamp_le(logn, shape.le);
+  const int lb = logn < kMaxBlockLog ? logn : kMaxBlockLog;
+  const int le = clamp_le(lb, shape.le);
   if (le > 4) return hipErrorInvalidValue;
-#define YA_FWD_PAIR_CALL(LB, LE) fwd_pair_lb<LB, LE>(data, descs, npairs, nmod, lazy, st)
-  YA_LB_SWITCH(logn, le, YA_FWD_PAIR_CALL)
+  if (logn > kMaxBlockLog) {
+    // top layers of every residue of the 2 * npairs polynomials, then the
+    // 2^kMaxBlockLog blocks in pairs
+    const int k = logn - kMaxBlockLog;
+    const dim3 grid((unsigned)((2 * npairs * nmod) << (kMaxBlockLog - 8)));
+    switch (k) {
+      case 1: hipLaunchKernelGGL(k_fwd_top<1>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+      case 2: hipLaunchKernelGGL(k_fwd_top<2>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+      case 3: hipLaunchKernelGGL(k_fwd_top<3>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+      default: return hipErrorInvalidValue;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+#define YA_FWD_PAIR_CALL(LB, LE) fwd_pair_lb<LB, LE>(data, descs, logn, npairs, nmod, lazy, st)
+  YA_LB_SWITCH(lb, le, YA_FWD_PAIR_CALL)
 #undef YA_FWD_PAIR_CALL
 }
 
