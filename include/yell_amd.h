@@ -150,9 +150,10 @@ int ya_poly_add_product_of(uint64_t *acc, const uint64_t *a, const uint64_t *b, 
 /* The same three transforms over `count` polynomials stored back to back
  * (polynomial k at words [k*L*N, (k+1)*L*N), residue r of each using P[r]). */
 int ya_poly_forward_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count, void *stream);
-/* The many-polynomial forward transforms of degrees 2^13 and 2^14 run residue r
- * of polynomials 2q and 2q+1 in one workgroup (shared twiddles); on = 0 runs
- * one residue per workgroup instead (same words; default 1). */
+/* The many-polynomial forward transforms of degree 2^13 and up run residue r
+ * of polynomials 2q and 2q+1 in one workgroup (shared twiddles; above 2^14,
+ * after the top layers, block b of that residue); on = 0 runs one residue
+ * (block) per workgroup instead (same words; default 1). */
 int ya_pair_forward(int on);
 int ya_poly_forward_lazy_many(uint64_t *polys, size_t degree, size_t nmoduli, size_t count,
                               void *stream);
