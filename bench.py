@@ -373,9 +373,12 @@ def rlwe_bench(stream, ev, batch=256, reps=5):
     """The reference example's RLWE flow (example.cpp:12-58) at its own
     parameters (N = 8192, L = 4, hwt 64, sigma 3.2): key generation once
     (eager), then a step of `batch` encryptions + decryptions under that key
-    (one launch per flow step over all messages: the *_many transforms,
-    samplers and pointwise forms), captured as one graph and replayed;
-    samples from the device samplers."""
+    (one launch per flow step over all messages: the samplers, the forward of
+    u, and the fused steps -- message addition, forward_lazy and
+    add_product_of in one pass per ciphertext half, add_product_of and
+    backward in one pass for the decryption), captured as one graph and
+    replayed; samples from the device samplers.  The eager step is also timed
+    with those steps as separate launches, and with pairing off."""
     from yell_amd.rlwe import Rlwe
     n, L = 8192, 4
     f = Rlwe(n, L, stream=stream)
@@ -391,14 +394,16 @@ def rlwe_bench(stream, ev, batch=256, reps=5):
     cb, ca, u, pl = (f.buffer(batch) for _ in range(4))
     seeds = (10, 10000, 20000)
 
-    def step():
-        f.encrypt_many(cb, ca, u, msgs, bx, ax, batch, seeds)
-        f.decrypt_many(pl, cb, ca, sx, batch)
+    def step(fused=True):
+        f.encrypt_many(cb, ca, u, msgs, bx, ax, batch, seeds, fused=fused)
+        f.decrypt_many(pl, cb, ca, sx, batch, fused=fused)
 
     step()
     capi.call("ya_stream_synchronize", stream)
     eager_ms = timed(ev, stream, step, 1)
-    capi.call("ya_pair_forward", 0)  # one residue per workgroup in the transforms, for comparison
+    step(False)  # the steps after each sampler as separate launches, for comparison
+    eager_unfused_ms = timed(ev, stream, lambda: step(False), 1)
+    capi.call("ya_pair_forward", 0)  # and one residue per workgroup in the transforms
     step()
     eager_unpaired_ms = timed(ev, stream, step, 1)
     capi.call("ya_pair_forward", 1)
@@ -417,9 +422,10 @@ def rlwe_bench(stream, ev, batch=256, reps=5):
     for b in (sx, bx, ax, msgs, cb, ca, u, pl):
         b.close()
     return {"degree": n, "nmoduli": L, "hwt": 64, "sigma": 3.2, "batch": batch, "keygen_us": keygen_us,
-            "eager_step_ms": eager_ms, "eager_step_unpaired_forward_ms": eager_unpaired_ms,
+            "eager_step_ms": eager_ms, "eager_step_unfused_ms": eager_unfused_ms,
+            "eager_step_unpaired_unfused_ms": eager_unpaired_ms,
             "graph_step_ms": ms, "value": batch / (ms / 1e3),
-            "unit": "encrypt+decrypt/s", "launches_per_step": 12, "_first": first}
+            "unit": "encrypt+decrypt/s", "launches_per_step": 7, "launches_per_step_unfused": 12, "_first": first}
 
 
 def rlwe_cpu_reference(rl, seconds=3.0):

@@ -181,6 +181,25 @@ int ya_poly_negate_many(uint64_t *x, size_t degree, size_t nmoduli, size_t count
  * arrays of npairs DEVICE pointers to polys of the acc's shape. */
 int ya_poly_add_products(uint64_t *acc, const uint64_t *const *a, const uint64_t *const *b,
                          size_t npairs, size_t degree, size_t nmoduli, void *stream);
+/* The reference example's encryption step (example.cpp:41-44 and 46-48:
+ * ct += msg; ct.forward_lazy(); ct.add_product_of(u, pk)) over `count`
+ * polynomials in one pass: acc_k = forward_lazy(acc_k + add_k) + u_k * key,
+ * with add null for no addition (ct_ax) and key one polynomial shared by all
+ * (the public key).  Same words as ya_poly_add_many, ya_poly_forward_lazy_many
+ * and ya_poly_add_product_of_many(.., b_shared = 1) in turn; fused at degrees
+ * 2^13 and 2^14 (two polynomials per workgroup), those launches otherwise.
+ * add, u and key must not overlap acc. */
+int ya_poly_forward_lazy_add_product_many(uint64_t *acc, const uint64_t *add, const uint64_t *u,
+                                          const uint64_t *key, size_t degree, size_t nmoduli, size_t count,
+                                          void *stream);
+/* The reference example's decryption step (example.cpp:55-57: m = ct_bx;
+ * m.add_product_of(sx, ct_ax); m.backward()) over `count` polynomials:
+ * out_k = backward(acc_k + x_k * key), key shared by all (the secret key).
+ * Same words as a copy, ya_poly_add_product_of_many(.., b_shared = 1) and
+ * ya_poly_backward_many in turn; fused up to degree 2^14.  out may be acc, or
+ * else must not overlap it; out must not overlap x or key. */
+int ya_poly_add_product_backward_many(uint64_t *out, const uint64_t *acc, const uint64_t *x, const uint64_t *key,
+                                      size_t degree, size_t nmoduli, size_t count, void *stream);
 
 /* One RNS poly-multiplication, fused: c = backward(forward(a) * forward(b)),
  * the call sequence a.forward(); b.forward(); c = a * b; c.backward() of the

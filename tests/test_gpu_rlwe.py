@@ -46,9 +46,11 @@ def test_device_flow_matches_oracle(word, n, L):
         b.close()
 
 
-def test_many_messages_match_single_flow():
+@pytest.mark.parametrize("fused", [True, False])
+def test_many_messages_match_single_flow(fused):
     """encrypt_many / decrypt_many of 6 messages = encrypt / decrypt of each
-    with seeds + k, bit-exact."""
+    with seeds + k, bit-exact, with the steps after each sampler fused and as
+    separate launches."""
     device()
     n, L, count = 8192, 4, 6
     f = Rlwe(n, L)
@@ -57,9 +59,9 @@ def test_many_messages_match_single_flow():
     msgs = capi.DeviceBuffer(n * L * count)
     capi.call("ya_sample_uniform_device_many", msgs.ptr, n, L, count, 500, None)
     cb, ca, u, pl = (f.buffer(count) for _ in range(4))
-    say("rlwe many", count)
-    f.encrypt_many(cb, ca, u, msgs, bx, ax, count, (10, 20, 30))
-    f.decrypt_many(pl, cb, ca, sx, count)
+    say("rlwe many", count, "fused", fused)
+    f.encrypt_many(cb, ca, u, msgs, bx, ax, count, (10, 20, 30), fused=fused)
+    f.decrypt_many(pl, cb, ca, sx, count, fused=fused)
     got = [x.download().reshape(count, n * L) for x in (cb, ca, pl)]
     w = n * L
     m_all = msgs.download()

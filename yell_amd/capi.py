@@ -94,6 +94,8 @@ _SIGS = {
     "ya_poly_sub_many": (i32, [vp, vp, vp, sz, sz, sz, i32, vp]),
     "ya_poly_add_product_of_many": (i32, [vp, vp, vp, sz, sz, sz, i32, vp]),
     "ya_poly_negate_many": (i32, [vp, sz, sz, sz, vp]),
+    "ya_poly_forward_lazy_add_product_many": (i32, [vp, vp, vp, vp, sz, sz, sz, vp]),
+    "ya_poly_add_product_backward_many": (i32, [vp, vp, vp, vp, sz, sz, sz, vp]),
     "ya_pointer_is_device": (i32, [vp, ctypes.POINTER(i32)]),
     "ya_probe_valu": (i32, [i32, ctypes.POINTER(ctypes.c_double)]),
     "ya_crt_limbs": (i32, [sz, ctypes.POINTER(ctypes.c_size_t)]),

@@ -726,6 +726,87 @@ int ya_poly_negate_many(uint64_t *x, size_t degree, size_t nmoduli, size_t count
   return pw_many(PW_NEGATE, x, x, nullptr, degree, nmoduli, count, 0, stream);
 }
 
+// Element-wise op over polynomials [k0, k0 + kc) in launches of at most 65535
+// residues; b_shared: b is one polynomial for all.
+static hipError_t pw_range(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b, const ModDesc *d,
+                           size_t degree, size_t nmoduli, size_t kc, bool b_shared, hipStream_t s) {
+  const size_t pw = degree * nmoduli, per = 65535 / nmoduli;
+  for (size_t k0 = 0; k0 < kc; k0 += per) {
+    const size_t n = std::min(per, kc - k0);
+    const hipError_t e = launch_pointwise_many(op, out + k0 * pw, a + k0 * pw, b_shared ? b : b + k0 * pw, d,
+                                               degree, nmoduli, n, b_shared ? 0 : pw, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+static bool overlaps(const uint64_t *a, size_t na, const uint64_t *b, size_t nb) {
+  return a && b && a < b + nb && b < a + na;
+}
+
+int ya_poly_forward_lazy_add_product_many(uint64_t *acc, const uint64_t *add, const uint64_t *u,
+                                          const uint64_t *key, size_t degree, size_t nmoduli, size_t count,
+                                          void *stream) {
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!acc || !u || !key) return fail(YA_ERR_INVALID, "null buffer");
+  if (count == 0 || count * nmoduli > (1u << 30)) return fail(YA_ERR_INVALID, "bad count %zu", count);
+  const size_t pw = degree * nmoduli, words = pw * count;
+  if (overlaps(acc, words, add, words) || overlaps(acc, words, u, words) || overlaps(acc, words, key, pw))
+    return fail(YA_ERR_INVALID, "add, u and key must not overlap acc");
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  const ModDesc *d = desc_ptr(st, logn, 0);
+  const hipStream_t s = (hipStream_t)stream;
+  const NttShape sh = tuned_shape(logn, 0);
+  hipError_t e = hipSuccess;
+  size_t done = 0;
+  if (count >= 2 && g_pair_forward.load() && logn <= kMaxBlockLog && forward_pairs_ok(logn, sh)) {
+    const size_t np = count / 2;
+    e = launch_forward_pairs_mac(acc, add, u, key, d, logn, np, nmoduli, sh, s);
+    done = 2 * np;
+  }
+  if (e == hipSuccess && done < count) {
+    // the remaining polynomial(s): the same steps as separate launches
+    const size_t kc = count - done;
+    uint64_t *ak = acc + done * pw;
+    if (add) e = pw_range(PW_ADD, ak, ak, add + done * pw, d, degree, nmoduli, kc, false, s);
+    if (e == hipSuccess) e = launch_forward(ak, d, logn, kc * nmoduli, nmoduli, true, sh, s);
+    if (e == hipSuccess) e = pw_range(PW_MULADD, ak, u + done * pw, key, d, degree, nmoduli, kc, true, s);
+  }
+  return e == hipSuccess ? YA_OK : hip_fail(e, "fused forward launch");
+}
+
+int ya_poly_add_product_backward_many(uint64_t *out, const uint64_t *acc, const uint64_t *x, const uint64_t *key,
+                                      size_t degree, size_t nmoduli, size_t count, void *stream) {
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!out || !acc || !x || !key) return fail(YA_ERR_INVALID, "null buffer");
+  if (count == 0 || count * nmoduli > (1u << 30)) return fail(YA_ERR_INVALID, "bad count %zu", count);
+  const size_t pw = degree * nmoduli, words = pw * count;
+  if ((out != acc && overlaps(out, words, acc, words)) || overlaps(out, words, x, words) ||
+      overlaps(out, words, key, pw))
+    return fail(YA_ERR_INVALID, "out must be acc or not overlap acc, and must not overlap x or key");
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  const ModDesc *d = desc_ptr(st, logn, 0);
+  const hipStream_t s = (hipStream_t)stream;
+  const NttShape sh = tuned_shape(logn, 0);
+  hipError_t e;
+  if (logn <= kMaxBlockLog) {
+    e = launch_mac_inverse(out, acc, x, key, d, logn, nmoduli * count, nmoduli, sh, s);
+  } else {
+    e = out == acc ? hipSuccess : hipMemcpyAsync(out, acc, words * 8, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = pw_range(PW_MULADD, out, x, key, d, degree, nmoduli, count, true, s);
+    if (e == hipSuccess) e = launch_backward(out, d, logn, nmoduli * count, nmoduli, sh, s);
+  }
+  return e == hipSuccess ? YA_OK : hip_fail(e, "fused backward launch");
+}
+
 int ya_poly_mul(uint64_t *out, const uint64_t *a, const uint64_t *b, size_t degree, size_t nmoduli,
                 void *stream) {
   return pw_common(PW_MUL, out, a, b, degree, nmoduli, stream);

@@ -41,6 +41,20 @@ bool forward_pairs_ok(int logn, NttShape shape);
 hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                            NttShape shape, hipStream_t st);
 
+// Fused steps of the reference example's RLWE flow, the words of the unfused
+// launches.  acc_k = u_k * key + forward_lazy(acc_k + add_k) for polynomials
+// 0 .. 2*npairs-1 (add may be null; key one polynomial shared by all), two
+// same-prime residues per workgroup, degrees 2^kMinPairLog .. 2^kMaxBlockLog
+// with shape.le <= 4; add, u and key must not alias acc.
+hipError_t launch_forward_pairs_mac(uint64_t *acc, const uint64_t *add, const uint64_t *u, const uint64_t *key,
+                                    const ModDesc *descs, int logn, size_t npairs, size_t nmod, NttShape shape,
+                                    hipStream_t st);
+// out_r = backward(acc_r + x_r * key_(r mod nmod)) for nres residues, degrees
+// up to 2^kMaxBlockLog; out may be acc.
+hipError_t launch_mac_inverse(uint64_t *out, const uint64_t *acc, const uint64_t *x, const uint64_t *key,
+                              const ModDesc *descs, int logn, size_t nres, size_t nmod, NttShape shape,
+                              hipStream_t st);
+
 // negacylic_backward_lazy (lib/ntt.cpp:151-210): every inverse layer except
 // the top one, outputs in [0, 2p).
 hipError_t launch_inverse_lazy(uint64_t *data, const ModDesc *descs, int logn, size_t nres,

@@ -55,6 +55,111 @@ k_fwd_block_pair(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs,
   gmem_store<0, LE>(py, y, t);
 }
 
+// ---- Fused steps of the reference example's RLWE flow over many polynomials
+// (each gives the words of the unfused sequence of launches).
+
+// Encryption step (example.cpp:41-44, 46-48) on residue r of polynomials 2q
+// and 2q+1: acc = acc + add (operator+=, include/yell/ops.hpp:25-41) as the
+// words are loaded, forward_lazy as k_fwd_block_pair, then each transform
+// goes back through LDS to the load layout (thread t: words t | e << W0,
+// coalesced) for acc = u * key + acc (add_product_of,
+// include/yell/details/poly.hpp:119-132) with coalesced reads of u and key,
+// and a coalesced store; key is one polynomial shared by all.  acc is read
+// before the first barrier, so add, u and key must not alias it.
+template <int LB, int LE, bool ADD>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_fwd_pair_mac(uint64_t *__restrict__ acc, const uint64_t *__restrict__ add, const uint64_t *__restrict__ u,
+               const uint64_t *__restrict__ key, const ModDesc *__restrict__ descs, uint32_t nmod,
+               uint32_t npairs) {
+  typedef const __attribute__((address_space(1))) uint64_t gw;
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
+  const uint32_t r = v / npairs, q = v % npairs;
+  const ModDesc d = descs[r];
+  const size_t ox = (size_t)(2 * q * nmod + r) << LB, oy = ox + ((size_t)nmod << LB);
+  const uint32_t t = threadIdx.x;
+  uint64_t x[1 << LE], y[1 << LE];
+  constexpr int W0 = FirstWindow<LB, LE>::value;
+  const uint32_t b0 = win_base<W0, LE>(t);
+  gmem_load<W0, LE>(acc + ox, x, t);
+  gmem_load<W0, LE>(acc + oy, y, t);
+  if constexpr (ADD) {
+    gw *ax = (gw *)(add + ox), *ay = (gw *)(add + oy);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) {
+      const uint32_t i = b0 | ((uint32_t)e << W0);
+      x[e] = addmod(x[e], ax[i], d.p);
+      y[e] = addmod(y[e], ay[i], d.p);
+    }
+  }
+  fwd_block_dual<LB, LE>(x, y, lds, d.fwd, LB, 0u, d.p, t);
+  gw *kr = (gw *)(key + ((size_t)r << LB));
+  // x: back to the load layout through LDS, then the product with u and key
+  // (thread indices through phase_v so that the reads of u and key stay in
+  // their phase instead of being hoisted, live, above the exchanges)
+  const uint32_t tx = phase_v(t);
+  __syncthreads();
+  lds_store<0, LE>(lds, x, tx);
+  __syncthreads();
+  lds_load<W0, LE>(lds, x, tx);
+  {
+    gw *ur = (gw *)(u + ox);
+    const uint32_t bx = win_base<W0, LE>(tx);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) {
+      const uint32_t i = bx | ((uint32_t)e << W0);
+      x[e] = muladd_barrett(x[e], ur[i], kr[i], d.p, d.pn);
+    }
+  }
+  gmem_store<W0, LE>(acc + ox, x, tx);
+  const uint32_t ty = phase_v(t);
+  __syncthreads();
+  lds_store<0, LE>(lds, y, ty);
+  __syncthreads();
+  lds_load<W0, LE>(lds, y, ty);
+  {
+    gw *ur = (gw *)(u + oy);
+    const uint32_t by = win_base<W0, LE>(ty);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) {
+      const uint32_t i = by | ((uint32_t)e << W0);
+      y[e] = muladd_barrett(y[e], ur[i], kr[i], d.p, d.pn);
+    }
+  }
+  gmem_store<W0, LE>(acc + oy, y, ty);
+}
+
+// Decryption step (example.cpp:55-57): out = backward(acc + x * key), the
+// product (add_product_of; key shared by all polynomials) formed with
+// coalesced reads (thread t: words t | e << W0) and passed through LDS to the
+// inverse's first layout; one residue per workgroup as k_inv_block.  Every
+// word of acc is read before the first barrier and out is written after the
+// last, so out may be acc.
+template <int LB, int LE>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_mac_inv(uint64_t *out, const uint64_t *acc, const uint64_t *__restrict__ xin, const uint64_t *__restrict__ key,
+          const ModDesc *__restrict__ descs, uint32_t nmod) {
+  typedef const __attribute__((address_space(1))) uint64_t gw;
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t res = blockIdx.x, r = res % nmod;
+  const ModDesc d = descs[r];
+  const size_t o = (size_t)res << LB;
+  const uint32_t t = threadIdx.x;
+  constexpr int W0 = FirstWindow<LB, LE>::value;
+  gw *ar = (gw *)(acc + o), *xr = (gw *)(xin + o), *kr = (gw *)(key + ((size_t)r << LB));
+  const uint32_t b0 = win_base<W0, LE>(t);
+  uint64_t x[1 << LE];
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) {
+    const uint32_t i = b0 | ((uint32_t)e << W0);
+    x[e] = muladd_barrett(ar[i], xr[i], kr[i], d.p, d.pn);
+  }
+  lds_store<W0, LE>(lds, x, t);
+  __syncthreads();
+  lds_load<0, LE>(lds, x, t);
+  inv_block<LB, LE, TOP_MERGED>(x, out + o, lds, d.inv, LB, 0u, d, t);
+}
+
 template <int LB, int LE, int TOP>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_inv_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nlog,
@@ -208,6 +313,59 @@ hipError_t launch_forward_pairs(uint64_t *data, const ModDesc *descs, int logn, 
 #define YA_FWD_PAIR_CALL(LB, LE) fwd_pair_lb<LB, LE>(data, descs, logn, npairs, nmod, lazy, st)
   YA_LB_SWITCH(lb, le, YA_FWD_PAIR_CALL)
 #undef YA_FWD_PAIR_CALL
+}
+
+namespace {
+
+template <int LB, int LE>
+hipError_t fwd_pair_mac_lb(uint64_t *acc, const uint64_t *add, const uint64_t *u, const uint64_t *key,
+                           const ModDesc *descs, size_t npairs, size_t nmod, hipStream_t st) {
+  if constexpr (LB >= kMinPairLog && LE <= 4) {
+    const dim3 grid((unsigned)(npairs * nmod));
+    const dim3 block(1u << (LB - LE));
+    if (add)
+      hipLaunchKernelGGL((k_fwd_pair_mac<LB, LE, true>), grid, block, 0, st, acc, add, u, key, descs, (uint32_t)nmod,
+                         (uint32_t)npairs);
+    else
+      hipLaunchKernelGGL((k_fwd_pair_mac<LB, LE, false>), grid, block, 0, st, acc, add, u, key, descs,
+                         (uint32_t)nmod, (uint32_t)npairs);
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
+template <int LB, int LE>
+hipError_t mac_inv_lb(uint64_t *out, const uint64_t *acc, const uint64_t *x, const uint64_t *key,
+                      const ModDesc *descs, size_t nres, size_t nmod, hipStream_t st) {
+  hipLaunchKernelGGL((k_mac_inv<LB, LE>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, out, acc, x, key,
+                     descs, (uint32_t)nmod);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_forward_pairs_mac(uint64_t *acc, const uint64_t *add, const uint64_t *u, const uint64_t *key,
+                                    const ModDesc *descs, int logn, size_t npairs, size_t nmod, NttShape shape,
+                                    hipStream_t st) {
+  if (logn < kMinPairLog || logn > kMaxBlockLog || npairs == 0 || nmod == 0) return hipErrorInvalidValue;
+  if (npairs * nmod > 0x7fffffffu >> (logn - 3)) return hipErrorInvalidValue;  // grid in work-items
+  const int le = clamp_le(logn, shape.le);
+  if (le > 4) return hipErrorInvalidValue;
+#define YA_FWD_PAIR_MAC_CALL(LB, LE) fwd_pair_mac_lb<LB, LE>(acc, add, u, key, descs, npairs, nmod, st)
+  YA_LB_SWITCH(logn, le, YA_FWD_PAIR_MAC_CALL)
+#undef YA_FWD_PAIR_MAC_CALL
+}
+
+hipError_t launch_mac_inverse(uint64_t *out, const uint64_t *acc, const uint64_t *x, const uint64_t *key,
+                              const ModDesc *descs, int logn, size_t nres, size_t nmod, NttShape shape,
+                              hipStream_t st) {
+  if (logn < kMinLogN || logn > kMaxBlockLog || nres == 0 || nmod == 0 || nres > 0x7fffffffu)
+    return hipErrorInvalidValue;
+  const int le = clamp_le(logn, shape.le);
+#define YA_MAC_INV_CALL(LB, LE) mac_inv_lb<LB, LE>(out, acc, x, key, descs, nres, nmod, st)
+  YA_LB_SWITCH(logn, le, YA_MAC_INV_CALL)
+#undef YA_MAC_INV_CALL
 }
 
 static hipError_t backward_impl(uint64_t *data, const ModDesc *descs, int logn, size_t nres,

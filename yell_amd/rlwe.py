@@ -12,8 +12,9 @@ graph (ya_graph_begin / ya_graph_end) once its calls have run once.
 
 encrypt_many / decrypt_many (64-bit build) run `count` messages under one key
 with one launch per step (the *_many transforms, samplers and pointwise
-forms, the key a shared operand); message k gets the words encrypt / decrypt
-give it with seeds + k.
+forms, the key a shared operand; the steps after each sampler fused into one
+pass by default); message k gets the words encrypt / decrypt give it with
+seeds + k.
 """
 import numpy as np
 
@@ -71,25 +72,40 @@ class Rlwe:
         extra = () if shared is None else (shared,)
         capi.call("ya_poly_" + fn, *[b.ptr for b in bufs], self.n, self.L, count, *extra, self.stream)
 
-    def encrypt_many(self, ct_bx, ct_ax, u, msgs, pk_bx, pk_ax, count, seeds):
+    def encrypt_many(self, ct_bx, ct_ax, u, msgs, pk_bx, pk_ax, count, seeds, fused=True):
         """encrypt() of messages 0..count-1 (polynomials back to back in msgs)
-        with seeds (seeds[0] + k, seeds[1] + k, seeds[2] + k)."""
+        with seeds (seeds[0] + k, seeds[1] + k, seeds[2] + k).  fused: each
+        ciphertext half is one ya_poly_forward_lazy_add_product_many call
+        (the addition of the message, forward_lazy and add_product_of in one
+        pass); else the separate launches.  The same words either way."""
         assert self.word == 64
         n, L, st = self.n, self.L, self.stream
         capi.call("ya_sample_zo_device_many", u.ptr, n, L, count, self.rho, seeds[0], st)
         self._many("forward_many", u, count=count)
         capi.call("ya_sample_gaussian_device_many", ct_bx.ptr, n, L, count, self.sigma, 1, seeds[1], st)
-        self._many("add_many", ct_bx, ct_bx, msgs, count=count, shared=0)
-        self._many("forward_lazy_many", ct_bx, count=count)
-        self._many("add_product_of_many", ct_bx, u, pk_bx, count=count, shared=1)
+        if fused:
+            capi.call("ya_poly_forward_lazy_add_product_many", ct_bx.ptr, msgs.ptr, u.ptr, pk_bx.ptr, n, L, count, st)
+        else:
+            self._many("add_many", ct_bx, ct_bx, msgs, count=count, shared=0)
+            self._many("forward_lazy_many", ct_bx, count=count)
+            self._many("add_product_of_many", ct_bx, u, pk_bx, count=count, shared=1)
         capi.call("ya_sample_gaussian_device_many", ct_ax.ptr, n, L, count, self.sigma, 1, seeds[2], st)
-        self._many("forward_lazy_many", ct_ax, count=count)
-        self._many("add_product_of_many", ct_ax, u, pk_ax, count=count, shared=1)
+        if fused:
+            capi.call("ya_poly_forward_lazy_add_product_many", ct_ax.ptr, None, u.ptr, pk_ax.ptr, n, L, count, st)
+        else:
+            self._many("forward_lazy_many", ct_ax, count=count)
+            self._many("add_product_of_many", ct_ax, u, pk_ax, count=count, shared=1)
 
-    def decrypt_many(self, plain, ct_bx, ct_ax, sx, count):
+    def decrypt_many(self, plain, ct_bx, ct_ax, sx, count, fused=True):
         """decrypt() of ciphertexts 0..count-1; sx * ct_ax is taken as
-        ct_ax * sx (the Barrett muladd of x * y + acc is symmetric in x, y)."""
+        ct_ax * sx (the Barrett muladd of x * y + acc is symmetric in x, y).
+        fused: one ya_poly_add_product_backward_many call; else a copy and
+        the separate launches."""
         assert self.word == 64
-        capi.call("ya_memcpy_dtod", plain.ptr, ct_bx.ptr, self.nbytes * count, self.stream)
+        n, L, st = self.n, self.L, self.stream
+        if fused:
+            capi.call("ya_poly_add_product_backward_many", plain.ptr, ct_bx.ptr, ct_ax.ptr, sx.ptr, n, L, count, st)
+            return
+        capi.call("ya_memcpy_dtod", plain.ptr, ct_bx.ptr, self.nbytes * count, st)
         self._many("add_product_of_many", plain, ct_ax, sx, count=count, shared=1)
         self._many("backward_many", plain, count=count)
