@@ -176,39 +176,106 @@ k_inv_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint
   inv_block<LB, LE, TOP>(x, base, lds, d.inv, nlog, blk << LB, d, t);
 }
 
-// Column kernels for N = 2^(14+K): one thread per column c in [0, 2^14)
-// holding the 2^K words c + e * 2^14.
+// Column kernels for N = 2^(B+K): one thread per column c in [0, 2^B) holding
+// the 2^K words c + e * 2^B (the layers on index bits >= B; the blocks of
+// 2^B words take the rest).
 constexpr int kTopThreads = 256;
 
-template <int K>
+template <int B, int K>
 __global__ void __launch_bounds__(kTopThreads)
 k_fwd_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod) {
-  constexpr int NLOG = kMaxBlockLog + K;
-  const uint32_t cols_per_res_log = kMaxBlockLog - 8;  // 2^14 / 256 workgroups per residue
+  constexpr int NLOG = B + K;
+  constexpr uint32_t cols_per_res_log = B - 8;  // 2^B / 256 workgroups per residue
   const uint32_t res = blockIdx.x >> cols_per_res_log;
   const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << 8) | threadIdx.x;
   const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << NLOG);
   uint64_t x[1 << K];
-  gmem_load<kMaxBlockLog, K>(base, x, t);
-  fwd_layers<kMaxBlockLog, K, NLOG, kMaxBlockLog>(x, d.fwd, NLOG, 0u, d.p);
-  gmem_store<kMaxBlockLog, K>(base, x, t);
+  gmem_load<B, K>(base, x, t);
+  fwd_layers<B, K, NLOG, B>(x, d.fwd, NLOG, 0u, d.p);
+  gmem_store<B, K>(base, x, t);
 }
 
-template <int K, bool MERGED>
+template <int B, int K, bool MERGED>
 __global__ void __launch_bounds__(kTopThreads)
 k_inv_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod) {
-  constexpr int NLOG = kMaxBlockLog + K;
-  const uint32_t cols_per_res_log = kMaxBlockLog - 8;
+  constexpr int NLOG = B + K;
+  constexpr uint32_t cols_per_res_log = B - 8;
   const uint32_t res = blockIdx.x >> cols_per_res_log;
   const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << 8) | threadIdx.x;
   const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << NLOG);
   uint64_t x[1 << K];
-  gmem_load<kMaxBlockLog, K>(base, x, t);
-  inv_layers<kMaxBlockLog, K, NLOG - 1, kMaxBlockLog>(x, d.inv, NLOG, 0u, d.p);
+  gmem_load<B, K>(base, x, t);
+  inv_layers<B, K, NLOG - 1, B>(x, d.inv, NLOG, 0u, d.p);
   if constexpr (MERGED) inv_last_layer<K, K - 1>(x, d);
-  gmem_store<kMaxBlockLog, K>(base, x, t);
+  gmem_store<B, K>(base, x, t);
+}
+
+// Block size of a ring above 2^kMaxBlockLog: 2^14 words, or 2^13 / 2^12 when
+// the launch would otherwise have fewer workgroups than the device has CUs
+// (a few residues: C4's single inverse is 24 residues of 2^16), keeping the
+// column pass to at most 4 layers.
+constexpr size_t kFillWorkgroups = 256;
+static int large_block_log(int logn, size_t nres) {
+  int lb = kMaxBlockLog;
+  while (lb > 12 && logn - lb < 4 && (nres << (logn - lb)) < kFillWorkgroups) --lb;
+  return lb;
+}
+
+template <int B>
+static hipError_t fwd_top_b(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
+                            hipStream_t st) {
+  const dim3 grid((unsigned)(nres << (B - 8)));
+  switch (k) {
+    case 1: hipLaunchKernelGGL((k_fwd_top<B, 1>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 2: hipLaunchKernelGGL((k_fwd_top<B, 2>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 3: hipLaunchKernelGGL((k_fwd_top<B, 3>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 4: hipLaunchKernelGGL((k_fwd_top<B, 4>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// The top K = logn - lb forward layers of nres residues (columns of 2^lb).
+static hipError_t launch_fwd_top(uint64_t *data, const ModDesc *descs, int logn, int lb, size_t nres, size_t nmod,
+                                 hipStream_t st) {
+  const int k = logn - lb;
+  switch (lb) {
+    case 12: return fwd_top_b<12>(data, descs, k, nres, nmod, st);
+    case 13: return fwd_top_b<13>(data, descs, k, nres, nmod, st);
+    case 14: return fwd_top_b<14>(data, descs, k, nres, nmod, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int B, bool MERGED>
+static hipError_t inv_top_b(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
+                            hipStream_t st) {
+  const dim3 grid((unsigned)(nres << (B - 8)));
+  switch (k) {
+    case 1: hipLaunchKernelGGL((k_inv_top<B, 1, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 2: hipLaunchKernelGGL((k_inv_top<B, 2, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 3: hipLaunchKernelGGL((k_inv_top<B, 3, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 4: hipLaunchKernelGGL((k_inv_top<B, 4, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// The top K = logn - lb inverse layers (the last merged with N^-1 unless lazy).
+static hipError_t launch_inv_top(uint64_t *data, const ModDesc *descs, int logn, int lb, size_t nres, size_t nmod,
+                                 bool lazy, hipStream_t st) {
+  const int k = logn - lb;
+  switch (lb * 2 + (lazy ? 0 : 1)) {
+    case 24: return inv_top_b<12, false>(data, descs, k, nres, nmod, st);
+    case 25: return inv_top_b<12, true>(data, descs, k, nres, nmod, st);
+    case 26: return inv_top_b<13, false>(data, descs, k, nres, nmod, st);
+    case 27: return inv_top_b<13, true>(data, descs, k, nres, nmod, st);
+    case 28: return inv_top_b<14, false>(data, descs, k, nres, nmod, st);
+    case 29: return inv_top_b<14, true>(data, descs, k, nres, nmod, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 namespace {
@@ -265,19 +332,11 @@ hipError_t inv_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, s
 hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                           bool lazy, NttShape shape, hipStream_t st) {
   if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0) return hipErrorInvalidValue;
-  if (logn > kMaxBlockLog) {
-    const int k = logn - kMaxBlockLog;
-    const dim3 grid((unsigned)(nres << (kMaxBlockLog - 8)));
-    switch (k) {
-      case 1: hipLaunchKernelGGL(k_fwd_top<1>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-      case 2: hipLaunchKernelGGL(k_fwd_top<2>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-      case 3: hipLaunchKernelGGL(k_fwd_top<3>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-      default: return hipErrorInvalidValue;
-    }
-    hipError_t e = hipGetLastError();
+  const int lb = logn <= kMaxBlockLog ? logn : large_block_log(logn, nres);
+  if (logn > lb) {
+    const hipError_t e = launch_fwd_top(data, descs, logn, lb, nres, nmod, st);
     if (e != hipSuccess) return e;
   }
-  const int lb = logn < kMaxBlockLog ? logn : kMaxBlockLog;
   const int le = clamp_le(lb, shape.le);
 #define YA_FWD_CALL(LB, LE) fwd_lb<LB, LE>(data, descs, logn, nres, nmod, lazy, st)
   YA_LB_SWITCH(lb, le, YA_FWD_CALL)
@@ -299,15 +358,7 @@ hipError_t launch_forward_pairs(uint64_t *data, const ModDesc *descs, int logn, 
   if (logn > kMaxBlockLog) {
     // top layers of every residue of the 2 * npairs polynomials, then the
     // 2^kMaxBlockLog blocks in pairs
-    const int k = logn - kMaxBlockLog;
-    const dim3 grid((unsigned)((2 * npairs * nmod) << (kMaxBlockLog - 8)));
-    switch (k) {
-      case 1: hipLaunchKernelGGL(k_fwd_top<1>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-      case 2: hipLaunchKernelGGL(k_fwd_top<2>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-      case 3: hipLaunchKernelGGL(k_fwd_top<3>, grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-      default: return hipErrorInvalidValue;
-    }
-    const hipError_t e = hipGetLastError();
+    const hipError_t e = launch_fwd_top(data, descs, logn, kMaxBlockLog, 2 * npairs * nmod, nmod, st);
     if (e != hipSuccess) return e;
   }
 #define YA_FWD_PAIR_CALL(LB, LE) fwd_pair_lb<LB, LE>(data, descs, logn, npairs, nmod, lazy, st)
@@ -371,7 +422,7 @@ hipError_t launch_mac_inverse(uint64_t *out, const uint64_t *acc, const uint64_t
 static hipError_t backward_impl(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
                                 size_t nmod, bool lazy, NttShape shape, hipStream_t st) {
   if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0) return hipErrorInvalidValue;
-  const int lb = logn < kMaxBlockLog ? logn : kMaxBlockLog;
+  const int lb = logn <= kMaxBlockLog ? logn : large_block_log(logn, nres);
   const int le = clamp_le(lb, shape.le);
   hipError_t e;
   {
@@ -379,20 +430,8 @@ static hipError_t backward_impl(uint64_t *data, const ModDesc *descs, int logn, 
     e = [&]() -> hipError_t { YA_LB_SWITCH(lb, le, YA_INV_CALL) }();
 #undef YA_INV_CALL
   }
-  if (e != hipSuccess || logn <= kMaxBlockLog) return e;
-  const int k = logn - kMaxBlockLog;
-  const dim3 grid((unsigned)(nres << (kMaxBlockLog - 8)));
-  const dim3 blk(kTopThreads);
-  switch (k * 2 + (lazy ? 0 : 1)) {
-    case 2: hipLaunchKernelGGL((k_inv_top<1, false>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
-    case 3: hipLaunchKernelGGL((k_inv_top<1, true>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
-    case 4: hipLaunchKernelGGL((k_inv_top<2, false>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
-    case 5: hipLaunchKernelGGL((k_inv_top<2, true>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
-    case 6: hipLaunchKernelGGL((k_inv_top<3, false>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
-    case 7: hipLaunchKernelGGL((k_inv_top<3, true>), grid, blk, 0, st, data, descs, (uint32_t)nmod); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  if (e != hipSuccess || logn <= lb) return e;
+  return launch_inv_top(data, descs, logn, lb, nres, nmod, lazy, st);
 }
 
 hipError_t launch_backward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
