@@ -89,3 +89,31 @@ def test_pointwise_many_chunks():
         np.testing.assert_array_equal(got[k], O.poly_add_product_of(acc[k * w:(k + 1) * w], a[k * w:(k + 1) * w], b, n, L))
     for d in (da, db, dc):
         d.close()
+
+
+def test_pointwise_range_edges_everywhere():
+    """Every word at the range edges, so that every lane takes the largest
+    128-bit products (the middle column's carry out of bit 63) and Barrett
+    quotients: a = b = p - 1, the accumulator at 4p - 1 (a forward_lazy
+    word), and the mixed edges 0, 1, p - 1 against each other."""
+    device()
+    n, L, count = 4096, 24, 2
+    P = [O.prime(j) for j in range(L)]
+    top = np.concatenate([np.full(n, p - 1, dtype=np.uint64) for p in P])
+    acc4 = np.concatenate([np.full(n, 4 * p - 1, dtype=np.uint64) for p in P])
+    mixed = np.tile(np.array([0, 1, 2, 3], dtype=np.uint64), n * L // 4)
+    mixed = np.concatenate([np.where(mixed[r * n:(r + 1) * n] == 3, P[r] - 1, mixed[r * n:(r + 1) * n]) for r in range(L)])
+    cases = [("mul", top, top, None), ("add_product_of", top, top, acc4), ("mul", mixed, top, None),
+             ("add_product_of", top, mixed, acc4)]
+    for op, a1, b1, acc1 in cases:
+        say("range edges", op)
+        a = np.concatenate([a1] * count)
+        da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b1)
+        dc = capi.DeviceBuffer.from_host(np.concatenate([acc1] * count) if acc1 is not None else np.zeros_like(a))
+        capi.call("ya_poly_%s_many" % op, dc.ptr, da.ptr, db.ptr, n, L, count, 1, None)
+        got = dc.download().reshape(count, n * L)
+        want = O.poly_add_product_of(acc1, a1, b1, n, L) if acc1 is not None else O.poly_mul(a1, b1, n, L)
+        for k in range(count):
+            np.testing.assert_array_equal(got[k], want, err_msg="%s polynomial %d" % (op, k))
+        for d in (da, db, dc):
+            d.close()
