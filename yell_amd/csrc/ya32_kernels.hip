@@ -24,7 +24,7 @@ k32_fwd(uint32_t *__restrict__ data, const ModDesc32 *__restrict__ descs, uint32
   uint32_t x[1 << LE];
   fwd_block<LB, LE>(x, base, lds, d.fwd, LB, 0u, d.p, t);
   if constexpr (!LAZY) correct_all<LE>(x, d.p);
-  gmem_store<0, LE>(base, x, t);
+  store_coalesced<LB, LE>(base, x, lds, t);
 }
 
 template <int LB, int LE>
@@ -36,7 +36,7 @@ k32_inv(uint32_t *__restrict__ data, const ModDesc32 *__restrict__ descs, uint32
   uint32_t *base = data + ((size_t)res << LB);
   const uint32_t t = threadIdx.x;
   uint32_t x[1 << LE];
-  gmem_load<0, LE>(base, x, t);
+  load_coalesced<LB, LE>(base, x, lds, t);
   inv_block<LB, LE, TOP_MERGED>(x, base, lds, d.inv, LB, 0u, d, t);
 }
 

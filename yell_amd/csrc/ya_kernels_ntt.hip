@@ -19,7 +19,7 @@ k_fwd_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint
   uint64_t x[1 << LE];
   fwd_block<LB, LE>(x, base, lds, d.fwd, nlog, blk << LB, d.p, t);
   if constexpr (!LAZY) correct_all<LE>(x, d.p);
-  gmem_store<0, LE>(base, x, t);
+  store_coalesced<LB, LE>(base, x, lds, t);
 }
 
 // Block blk (2^LB words) of residue r of polynomials 2q and 2q+1 of a
@@ -51,8 +51,8 @@ k_fwd_block_pair(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs,
     correct_all<LE>(x, d.p);
     correct_all<LE>(y, d.p);
   }
-  gmem_store<0, LE>(px, x, t);
-  gmem_store<0, LE>(py, y, t);
+  store_coalesced<LB, LE>(px, x, lds, t);
+  store_coalesced<LB, LE>(py, y, lds, t);
 }
 
 // ---- Fused steps of the reference example's RLWE flow over many polynomials
@@ -172,7 +172,7 @@ k_inv_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint
   uint64_t *base = data + ((size_t)res << nlog) + ((size_t)blk << LB);
   const uint32_t t = threadIdx.x;
   uint64_t x[1 << LE];
-  gmem_load<0, LE>(base, x, t);
+  load_coalesced<LB, LE>(base, x, lds, t);
   inv_block<LB, LE, TOP>(x, base, lds, d.inv, nlog, blk << LB, d, t);
 }
 
