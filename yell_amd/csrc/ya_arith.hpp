@@ -32,10 +32,13 @@ __device__ __forceinline__ uint64_t mul32x32(uint32_t a, uint32_t b) {
 // carry-out of v_mad_u64_u32 nor chains a product into a full 64-bit addend
 // (it zero-extends 32-bit halves into fresh register pairs, one v_mov each),
 // and it rewrites x >= m ? x - m : x into a compare, two masked selects of m
-// and a borrow chain.  Operands constrained "s" are placed in a scalar
-// register where the value is wave-uniform and in a vector register where it
-// is not; both are legal VOP3 sources.  The sequences write no memory and
-// read nothing but their operands.
+// and a borrow chain.  Each instruction reads at most one scalar operand (the
+// VOP3 constant-bus limit).  hipcc places an operand constrained "s" in a
+// scalar register where the value is wave-uniform and, where it is not (the
+// persistent batch kernel's sub-wave rings, whose lanes may hold different
+// primes), in a vector register -- both legal VOP3 sources; the batch tests
+// over mixed rings check those paths word for word.  The sequences write no
+// memory and read nothing but their operands.
 
 // a * b + c (mod 2^64) and the carry out of bit 63 (0 or 1).  The select
 // that reads the carry mask waits two cycles after the multiply wrote it
