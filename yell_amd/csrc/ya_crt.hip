@@ -338,22 +338,35 @@ struct ScratchLimbs {
   __device__ __forceinline__ uint32_t &operator[](int k) { return base[(size_t)k * stride]; }
 };
 
+// The output is coefficient-major (coefficient i at words [i * limbs, (i+1) *
+// limbs)), so a thread's own limbs are `limbs` consecutive words: stored
+// directly, one store instruction would touch one cache line per lane.  The
+// workgroup's coefficients are staged in LDS instead and written as one
+// contiguous range, consecutive words across the lanes.
 template <int WQ>
 __global__ void __launch_bounds__(kCrtThreads)
 k_crt_reg(uint64_t *__restrict__ out, const uint64_t *__restrict__ res, uint32_t n, uint32_t L,
           const uint32_t *__restrict__ lift, const uint32_t *__restrict__ q, uint32_t m0, uint32_t m1,
           uint32_t m2, uint32_t shift, uint32_t limbs) {
-  const uint32_t i = blockIdx.x * kCrtThreads + threadIdx.x;
-  if (i >= n) return;
-  RegLimbs<WQ> s, pr;
+  __shared__ uint64_t stage[kCrtThreads * (WQ / 2)];
+  const uint32_t c0 = blockIdx.x * kCrtThreads;
+  const uint32_t i = c0 + threadIdx.x;
+  if (i < n) {
+    RegLimbs<WQ> s, pr;
 #pragma unroll
-  for (int k = 0; k < WQ + 6; ++k) s[k] = pr[k] = 0;
-  const uint32_t m[3] = {m0, m1, m2};
-  crt_one<WQ>(s, pr, res + i, n, L, lift, q, m, shift, WQ);
-  uint64_t *o = out + (size_t)i * limbs;
+    for (int k = 0; k < WQ + 6; ++k) s[k] = pr[k] = 0;
+    const uint32_t m[3] = {m0, m1, m2};
+    crt_one<WQ>(s, pr, res + i, n, L, lift, q, m, shift, WQ);
+    uint64_t *o = stage + threadIdx.x * limbs;
 #pragma unroll
-  for (int t = 0; t < WQ / 2; ++t)
-    if ((uint32_t)t < limbs) o[t] = (uint64_t)s[2 * t] | ((uint64_t)s[2 * t + 1] << 32);
+    for (int t = 0; t < WQ / 2; ++t)
+      if ((uint32_t)t < limbs) o[t] = (uint64_t)s[2 * t] | ((uint64_t)s[2 * t + 1] << 32);
+  }
+  __syncthreads();
+  const uint32_t cnt = n - c0 < (uint32_t)kCrtThreads ? n - c0 : (uint32_t)kCrtThreads;
+  const uint32_t words = cnt * limbs;
+  uint64_t *dst = out + (size_t)c0 * limbs;
+  for (uint32_t j = threadIdx.x; j < words; j += kCrtThreads) dst[j] = stage[j];
 }
 
 // Scratch path: s and pr of coefficient i at scratch[k * chunk + i] and
