@@ -404,8 +404,8 @@ def rlwe_bench(stream, ev, batch=256, reps=5):
     step(False)  # the steps after each sampler as separate launches, for comparison
     eager_unfused_ms = timed(ev, stream, lambda: step(False), 1)
     capi.call("ya_pair_forward", 0)  # and one residue per workgroup in the transforms
-    step()
-    eager_unpaired_ms = timed(ev, stream, step, 1)
+    step(False)
+    eager_unpaired_ms = timed(ev, stream, lambda: step(False), 1)
     capi.call("ya_pair_forward", 1)
     step()
     capi.call("ya_graph_begin", stream)
