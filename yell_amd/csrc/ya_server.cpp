@@ -319,6 +319,10 @@ void dispatch_loop(ya_server *s) {
     lk.unlock();
     const int rc = launch_slot(s, k);
     const double lus = std::chrono::duration<double, std::micro>(clk::now() - sl.host_dispatch).count();
+    // a launch that failed part-way may have queued copies into the arena and
+    // the clients' buffers: they finish before that space is handed out again
+    // and before the clients learn of the failure
+    if (rc != YA_OK) (void)hipStreamSynchronize(sl.stream);
     lk.lock();
     sl.launch_us = lus;
     if (rc != YA_OK) {
