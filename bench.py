@@ -463,25 +463,27 @@ def rlwe_cpu_reference(rl, seconds=3.0):
                       "flow (N=%d, L=%d) through the reference's sources" % (reps, reps_t, th, n, L)}
 
 
-def served_bench(rates, duration, shapes):
-    """Poisson stream of host-operand requests (shape drawn from `shapes`):
-    continuous batching vs one request per launch, same seeded stream."""
-    out = {"shapes": [list(x) for x in shapes], "duration_s": duration, "runs": []}
+def served_bench(rates, duration, shapes, pool=8, max_pending=1024, batch_jobs=256):
+    """Poisson stream of host-operand requests (shape drawn uniformly from
+    `shapes`, `pool` operand pairs per shape): continuous batching vs one
+    request per launch, same seeded stream."""
+    out = {"shapes": [list(x) for x in shapes] if len(shapes) <= 4 else "%d shapes" % len(shapes),
+           "duration_s": duration, "runs": []}
     degrees = (ctypes.c_uint32 * len(shapes))(*[n for n, _ in shapes])
     mods = (ctypes.c_uint32 * len(shapes))(*[L for _, L in shapes])
-    for mode, cfgkw in (("batched", dict(max_batch_jobs=256, max_inflight=3)),
+    for mode, cfgkw in (("batched", dict(max_batch_jobs=batch_jobs, max_inflight=3)),
                         ("plain", dict(max_batch_jobs=1, max_inflight=1))):
-        cfg = capi.ServerConfig(hbm_budget_bytes=2 << 30, max_pending=1024, max_moduli=16, **cfgkw)
+        cfg = capi.ServerConfig(hbm_budget_bytes=2 << 30, max_pending=max_pending, max_moduli=16, **cfgkw)
         srv = ctypes.c_void_p()
         capi.call("ya_server_create", ctypes.byref(srv), ctypes.byref(cfg))
         # untimed warm-up stream (first-use costs of the server's streams,
         # arena and pinned pools stay out of the measured percentiles)
         warm = capi.PoissonSpec(rate_per_s=rates[0], duration_s=0.3, nshapes=len(shapes), degrees=degrees,
-                                nmoduli=mods, pool=8, verify=0, seed=7)
+                                nmoduli=mods, pool=pool, verify=0, seed=7)
         capi.call("ya_server_run_poisson", srv, ctypes.byref(warm), ctypes.byref(capi.ServeReport()))
         for rate in rates:
             spec = capi.PoissonSpec(rate_per_s=rate, duration_s=duration, nshapes=len(shapes),
-                                    degrees=degrees, nmoduli=mods, pool=8, verify=1, seed=99)
+                                    degrees=degrees, nmoduli=mods, pool=pool, verify=1, seed=99)
             rep = capi.ServeReport()
             log("served", mode, "rate", rate)
             capi.call("ya_server_run_poisson", srv, ctypes.byref(spec), ctypes.byref(rep))
@@ -653,6 +655,11 @@ def main():
             if r["mode"] == "batched" and r["offered_rate_per_s"] == args.serve_rates[0]:
                 out["served_p50_us"], out["served_p99_us"] = r["p50_us"], r["p99_us"]
                 out["served_offered_rate_per_s"] = r["offered_rate_per_s"]
+        # the same stream over the uneven batch's shapes (degree 2^10..2^15
+        # and 1..12 moduli, each uniform): 72 shapes, 2 operand pairs each
+        uneven = [(1 << lg, m) for lg in range(10, 16) for m in range(1, 13)]
+        out["served_uneven"] = served_bench(sorted({args.serve_rates[0], args.serve_rates[-1]}), args.serve_seconds,
+                                            uneven, pool=2, max_pending=256, batch_jobs=64)
     same = bool(np.array_equal(c0, c0_plain) and np.array_equal(c0, c0_grid))
     out["batched_equals_plain"] = same
     if not args.no_cpu:

@@ -39,7 +39,7 @@ def _submit(s, a, b, c, n, L):
 
 
 def _request(rng, k):
-    n, L = 1 << int(rng.integers(10, 15)), int(rng.integers(1, 9))
+    n, L = 1 << int(rng.integers(10, 16)), int(rng.integers(1, 13))
     a, b, c = Pinned(n * L), Pinned(n * L), Pinned(n * L)
     a.a[:] = capi.sample_uniform(n, L, 300 + 2 * k)
     b.a[:] = capi.sample_gaussian(n, L, 3.2, 301 + 2 * k)
