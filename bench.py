@@ -540,7 +540,8 @@ def main():
                              "chosen_batched_le": [capi.load().ya_tune_get(2, lg) for lg in range(10, 16)],
                              "chosen_fused_le": [capi.load().ya_tune_get(1, lg) for lg in range(10, 16)]}
         probes = {}
-        for kind, name in ((0, "mad_u64_u32_per_s"), (1, "mul_lo_u32_per_s"), (2, "ct_butterflies_per_s")):
+        for kind, name in ((0, "mad_u64_u32_per_s"), (1, "mul_lo_u32_per_s"), (2, "ct_butterflies_per_s"),
+                           (3, "ct32_butterflies_per_s")):
             v = ctypes.c_double()
             capi.call("ya_probe_valu", kind, ctypes.byref(v))
             probes[name] = v.value
@@ -645,6 +646,10 @@ def main():
         out["single_polymul_latency"] = latency_bench(stream, ev, n, L)
         log("32-bit moduli build")
         out["u32_moduli"] = u32_bench(stream, ev)
+        if "valu_probe" in extra:  # the 32-bit build against its own butterfly ceiling
+            pk32 = extra["valu_probe"]["ct32_butterflies_per_s"]
+            out["u32_moduli"]["butterfly_probe_T_per_s"] = pk32 / 1e12
+            out["u32_moduli"]["frac_of_probe"] = out["u32_moduli"]["modmuls_T_per_s"] * 1e12 / pk32
         log("device samplers")
         out["device_samplers"] = sampler_bench(stream, ev)
         log("RLWE flow")

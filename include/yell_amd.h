@@ -483,8 +483,9 @@ int ya32_sample_hwt_device(uint32_t *d_out, size_t degree, size_t nmoduli, uint3
  * No counterpart in the reference.  Device throughput of the VALU work this
  * engine is bound by, all operands in registers: kind 0 = v_mad_u64_u32
  * (32x32+64 -> 64), kind 1 = v_mul_lo_u32, kind 2 = this engine's lazy CT
- * butterfly (lib/ntt.cpp:69-83).  Result in operations (butterflies) per
- * second over the whole device. */
+ * butterfly (lib/ntt.cpp:69-83), kind 3 = the 32-bit moduli build's CT
+ * butterfly.  Result in operations (butterflies) per second over the whole
+ * device. */
 int ya_probe_valu(int kind, double *ops_per_second);
 
 #ifdef __cplusplus

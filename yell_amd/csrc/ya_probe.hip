@@ -1,7 +1,8 @@
 // ya_probe.hip -- VALU throughput probes used as the compute ceiling of the
 // roofline: the rate at which the device retires 32-bit integer multiplies
-// and this engine's exact CT butterfly with all operands in registers (no
-// memory traffic inside the timed loop).
+// and this engine's exact CT butterflies (64-bit words, and the 32-bit moduli
+// build's) with all operands in registers (no memory traffic inside the timed
+// loop).
 #include <hip/hip_runtime.h>
 
 #include "../../include/yell_amd.h"
@@ -35,6 +36,20 @@ k_probe(uint64_t *__restrict__ out, uint32_t iters, uint64_t seed, uint64_t p) {
     }
 #pragma unroll
     for (int k = 0; k < kChains; ++k) v[k] = w[k];
+  } else if constexpr (KIND == 3) {  // 32-bit moduli build: CT butterflies over 16 words
+    const uint32_t q = (uint32_t)p;
+    const PrimeK32 pk(q);
+    const uint32_t w0 = (uint32_t)(seed % q);
+    const Tw32 t{w0, (uint32_t)((((uint64_t)w0) << 32) / q)};
+    uint32_t x[kChains * 2];
+#pragma unroll
+    for (int k = 0; k < kChains * 2; ++k) x[k] = (uint32_t)(v[k] % ((uint64_t)q << 2));
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+      for (int k = 0; k < kChains; ++k) ct_butterfly(x[k], x[k + kChains], t, pk);
+    }
+#pragma unroll
+    for (int k = 0; k < kChains * 2; ++k) v[k] = x[k];
   } else {  // CT butterflies over 16 words, 8 per iteration
     const PrimeK pk(p);
     const Tw t{(seed % p), ((seed % p) * 3) | 1};
@@ -56,7 +71,7 @@ k_probe(uint64_t *__restrict__ out, uint32_t iters, uint64_t seed, uint64_t p) {
 using namespace ya;
 
 extern "C" int ya_probe_valu(int kind, double *ops_per_second) {
-  if (!ops_per_second || kind < 0 || kind > 2) return fail(YA_ERR_INVALID, "bad probe kind");
+  if (!ops_per_second || kind < 0 || kind > 3) return fail(YA_ERR_INVALID, "bad probe kind");
   DeviceState *st;
   int rc = current_device(&st);
   if (rc) return rc;
@@ -65,21 +80,22 @@ extern "C" int ya_probe_valu(int kind, double *ops_per_second) {
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return hip_fail(e, "device attributes");
   const unsigned blocks = (unsigned)cus * 16u;
-  const uint32_t iters = kind == 2 ? 2048u : 8192u;
+  const uint32_t iters = kind >= 2 ? 2048u : 8192u;
   uint64_t *out = nullptr;
   e = hipMalloc(&out, (size_t)blocks * kProbeThreads * 8);
   if (e != hipSuccess) return fail(YA_ERR_ALLOC, "probe buffer");
   hipEvent_t a, b;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
-  const uint64_t p = prime(0);
+  const uint64_t p = kind == 3 ? (uint64_t)ya32_prime(0) : prime(0);
   float best = 1e30f;
   for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
     (void)hipEventRecord(a, nullptr);
     switch (kind) {
       case 0: hipLaunchKernelGGL(k_probe<0>, dim3(blocks), dim3(kProbeThreads), 0, nullptr, out, iters, 0x5eedULL + rep, p); break;
       case 1: hipLaunchKernelGGL(k_probe<1>, dim3(blocks), dim3(kProbeThreads), 0, nullptr, out, iters, 0x5eedULL + rep, p); break;
-      default: hipLaunchKernelGGL(k_probe<2>, dim3(blocks), dim3(kProbeThreads), 0, nullptr, out, iters, 0x5eedULL + rep, p); break;
+      case 2: hipLaunchKernelGGL(k_probe<2>, dim3(blocks), dim3(kProbeThreads), 0, nullptr, out, iters, 0x5eedULL + rep, p); break;
+      default: hipLaunchKernelGGL(k_probe<3>, dim3(blocks), dim3(kProbeThreads), 0, nullptr, out, iters, 0x5eedULL + rep, p); break;
     }
     e = hipGetLastError();
     (void)hipEventRecord(b, nullptr);
