@@ -24,6 +24,7 @@
 #include <array>
 #include <atomic>
 #include <cstring>
+#include <ostream>
 #include <stdexcept>
 #include <vector>
 
@@ -341,6 +342,22 @@ poly<N> multiply(const poly<N> &a, const poly<N> &b) {
   poly<N> r(L);
   if (L) ya_check(detail::api_poly_multiply(r.data(), a.data(), b.data(), N, L, nullptr), "poly multiply");
   return r;
+}
+
+// include/yell/poly.hpp:142 (text format of details/poly.hpp:336-353): one
+// brace-enclosed row of "<word>ULL" per residue, rows separated by newlines,
+// all inside one more pair of braces.  One copy of the polynomial to the host.
+template <size_t N>
+std::ostream &operator<<(std::ostream &os, const poly<N> &p) {
+  const size_t L = p.moduli_count();
+  const auto h = p.to_host();
+  os << "{";
+  for (size_t cm = 0; cm < L; ++cm) {
+    os << "{";
+    for (size_t i = 0; i < N; ++i) os << h[cm * N + i] << "ULL" << (i + 1 < N ? ", " : "}");
+    if (cm + 1 < L) os << "\n";
+  }
+  return os << "}";
 }
 
 }  // namespace yell

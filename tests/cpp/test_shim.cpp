@@ -5,6 +5,8 @@
 // (test_shim32), as the reference's compile-time switch.
 #include <cstdio>
 #include <cstdlib>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "yell/ntt.hpp"
@@ -172,6 +174,20 @@ static void test_device_samplers() {
   }
 }
 
+// operator<< prints the reference's text form (details/poly.hpp:336-353).
+static void test_stream_output() {
+  constexpr size_t N = 8, L = 2;
+  yell::poly<N> p(L);
+  std::vector<T> h(N * L);
+  for (size_t k = 0; k < h.size(); ++k) h[k] = (T)(k % N + 10 * (k / N));
+  p.from_host(h);
+  std::ostringstream os;
+  os << p;
+  const std::string want = "{{0ULL, 1ULL, 2ULL, 3ULL, 4ULL, 5ULL, 6ULL, 7ULL}\n"
+                           "{10ULL, 11ULL, 12ULL, 13ULL, 14ULL, 15ULL, 16ULL, 17ULL}}";
+  CHECK(os.str() == want, "operator<< printed %s", os.str().c_str());
+}
+
 int main() {
   if (ya_init(0) != YA_OK) {
     std::fprintf(stderr, "no device: %s\n", ya_last_error());
@@ -183,6 +199,7 @@ int main() {
   test_polymul();
   test_ntt_host_pointer_and_tables();
   test_rlwe_example();
+  test_stream_output();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "ok", g_fail);
   return g_fail ? 1 : 0;
 }
