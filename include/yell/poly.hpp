@@ -344,6 +344,28 @@ poly<N> multiply(const poly<N> &a, const poly<N> &b) {
   return r;
 }
 
+// include/yell/poly.hpp:144-161: residue cm viewed as one std::array.  Here,
+// as ptr_at(cm), the pointer is a device address: for device code, or for
+// copies through the C ABI.
+template <typename value_type, size_t degree>
+struct ArrayPointer {
+  using T = std::array<value_type, degree> *;
+  using cT = const T;
+};
+
+template <size_t degree>
+typename ArrayPointer<params::value_type, degree>::T recast_as_array(poly<degree> &op, size_t cm) {
+  if (cm >= op.moduli_count()) throw std::out_of_range("modulus index");
+  return reinterpret_cast<typename ArrayPointer<params::value_type, degree>::T>(op.ptr_at(cm));
+}
+
+template <size_t degree>
+typename ArrayPointer<params::value_type, degree>::cT recast_as_array(const poly<degree> &op, size_t cm) {
+  if (cm >= op.moduli_count()) throw std::out_of_range("modulus index");
+  return reinterpret_cast<typename ArrayPointer<params::value_type, degree>::cT>(
+      const_cast<params::value_type *>(op.cptr_at(cm)));
+}
+
 // include/yell/poly.hpp:142 (text format of details/poly.hpp:336-353): one
 // brace-enclosed row of "<word>ULL" per residue, rows separated by newlines,
 // all inside one more pair of braces.  One copy of the polynomial to the host.

@@ -3,6 +3,7 @@
 // (benchmark_ntt.cc, example.cpp).  Exit status 0 = every check passed.
 // Built twice: 64-bit moduli (test_shim) and -DYELL_USE_32BITS_MODULI
 // (test_shim32), as the reference's compile-time switch.
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <sstream>
@@ -186,6 +187,13 @@ static void test_stream_output() {
   const std::string want = "{{0ULL, 1ULL, 2ULL, 3ULL, 4ULL, 5ULL, 6ULL, 7ULL}\n"
                            "{10ULL, 11ULL, 12ULL, 13ULL, 14ULL, 15ULL, 16ULL, 17ULL}}";
   CHECK(os.str() == want, "operator<< printed %s", os.str().c_str());
+  // recast_as_array views residue cm in place (a device address here)
+  const yell::poly<N> &cp = p;
+  CHECK((void *)yell::recast_as_array(p, 1) == (void *)p.ptr_at(1), "recast_as_array");
+  CHECK((const void *)yell::recast_as_array(cp, 1) == (const void *)cp.cptr_at(1), "recast_as_array const");
+  std::array<T, N> back;
+  CHECK(ya_memcpy_dtoh(back.data(), yell::recast_as_array(cp, 1), sizeof back, nullptr) == YA_OK, "copy");
+  CHECK(back[3] == 13, "recast_as_array residue 1 word 3 = %llu", (unsigned long long)back[3]);
 }
 
 int main() {
