@@ -55,10 +55,19 @@ class ntt {
   using T = params::value_type;
 
  public:
-  // The reference's ntt_precomputed (details/ntt.hpp:27-77), a host copy of
-  // the device-resident table of (degree, cm).
+  // The reference's ntt_precomputed (include/yell/ntt.hpp:23-45, filled by
+  // details/ntt.hpp:27-77): a host copy of the device-resident table of
+  // (degree, cm), the same members as fixed arrays in bit-reversed order.
   struct ntt_precomputed {
-    std::vector<T> phiTbl, shoupPhiTbl, invphiTbl, shoupInvphiTbl;
+    void init(size_t cm) {
+      T sc[4];
+      ya_check(detail::api_table(degree, cm, phiTbl, shoupPhiTbl, invphiTbl, shoupInvphiTbl, sc), "ntt table");
+      invDegree = sc[0];
+      shoupInvDegree = sc[1];
+      invphiInvDegree = sc[2];
+      shoupInvphiInvDegree = sc[3];
+    }
+    T phiTbl[degree], shoupPhiTbl[degree], invphiTbl[degree], shoupInvphiTbl[degree];
     T invDegree, shoupInvDegree, invphiInvDegree, shoupInvphiInvDegree;
   };
 
@@ -69,18 +78,7 @@ class ntt {
     auto it = cache.find(cm);
     if (it != cache.end()) return it->second.get();
     std::unique_ptr<ntt_precomputed> t(new ntt_precomputed());
-    t->phiTbl.resize(degree);
-    t->shoupPhiTbl.resize(degree);
-    t->invphiTbl.resize(degree);
-    t->shoupInvphiTbl.resize(degree);
-    T sc[4];
-    ya_check(detail::api_table(degree, cm, t->phiTbl.data(), t->shoupPhiTbl.data(), t->invphiTbl.data(),
-                               t->shoupInvphiTbl.data(), sc),
-             "ntt table");
-    t->invDegree = sc[0];
-    t->shoupInvDegree = sc[1];
-    t->invphiInvDegree = sc[2];
-    t->shoupInvphiInvDegree = sc[3];
+    t->init(cm);
     auto *raw = t.get();
     cache.emplace(cm, std::move(t));
     return raw;

@@ -74,6 +74,15 @@ static void test_ntt_host_pointer_and_tables() {
   CHECK(mul(t->invDegree, N, 1) == 1, "invDegree * N != 1");
   CHECK(t->shoupPhiTbl[5] == yell::ops::shoupify(t->phiTbl[5], 1), "Shoup companion");
   CHECK(mul(t->phiTbl[1], t->phiTbl[1], 1) == yell::params::P[1] - 1, "phi^(N/2)^2 != -1");
+#ifndef YELL_USE_32BITS_MODULI
+  // the tables passed as the reference passes them to its out-of-line seam
+  // (details/ntt.hpp:111-118: the arrays decay to pointers)
+  const auto t0 = yell::ntt<N>::init_table(0);
+  std::vector<T> r1(h.begin(), h.begin() + N);
+  CHECK(ya_negacylic_forward_lazy(r1.data(), N, t0->phiTbl, t0->shoupPhiTbl, yell::params::P[0]) == YA_OK,
+        "seam call");
+  CHECK(r1 == r0, "seam forward_lazy differs from ntt<N>::forward_lazy");
+#endif
 }
 
 // The toy RLWE scheme of the reference's example.cpp:12-58 at N = 8192,
