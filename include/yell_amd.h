@@ -215,6 +215,21 @@ int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t d
 int ya_poly_multiply_many(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
                           size_t nmoduli, size_t count, void *stream);
 
+/* ya_poly_multiply for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same words as ya_poly_multiply; it calls no
+ * allocating routine: the device must have been set up by ya_init and the
+ * tables of (degree, P[0..nmoduli)) built by ya_ntt_init_tables(degree,
+ * nmoduli) beforehand (else YA_ERR_INVALID), and the scratch of the split and
+ * large-ring paths is one static device array of 2^21 words (16 MiB) per
+ * device, held until the process ends; a call that needs more (nmoduli *
+ * degree > 2^21 above degree 2^14, or a split of more than 2^21 words)
+ * returns YA_ERR_INVALID.  Every call on one device shares that array, so
+ * calls that overlap on one device (two streams, two threads) race: order
+ * them on one stream or serialise them. */
+int ya_poly_multiply_noalloc(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                             size_t nmoduli, void *stream);
+
 /* ---- batched form (uneven degrees and modulus counts) ------------------- *
  * No single counterpart in the reference: a batch of the reference's
  * poly-multiplications (the sequence above), possibly of different degrees

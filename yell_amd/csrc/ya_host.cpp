@@ -124,9 +124,18 @@ struct DeviceState {
   int cus = 256;
   hipStream_t tstream = nullptr;
   ModDesc *d_desc = nullptr;  // [(kMaxLogN + 1) * YA_NUM_MODULI]
+  uint64_t *scratch = nullptr;  // this device's copy of ya::scratch (device address)
   std::mutex mu;
   std::unique_ptr<Slot[]> slots;  // same indexing as d_desc
 };
+
+// Scratch of ya_poly_multiply_noalloc: one static array per device, loaded
+// with the code object, so that entry point never allocates.  Host code must
+// not pass `scratch` itself to a kernel (on the host it names a stand-in, not
+// device memory): DeviceState::scratch holds the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchWords = (size_t)1 << 21;  // 16 MiB
+static __device__ __align__(256) uint64_t scratch[kScratchWords];
 
 static std::mutex g_dev_mu;
 static std::vector<std::unique_ptr<DeviceState>> g_devs;
@@ -171,12 +180,41 @@ int current_device(DeviceState **out) {
     e = hipMemset(st->d_desc, 0, nslots * sizeof(ModDesc));
     if (e != hipSuccess) return hip_fail(e, "hipMemset(desc)");
     st->slots.reset(new Slot[nslots]);
+    e = hipGetSymbolAddress((void **)&st->scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !st->scratch)
+      return fail(YA_ERR_NO_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
       st->cus = cus;
     g_devs[dev] = std::move(st);
   }
   *out = g_devs[dev].get();
+  return YA_OK;
+}
+
+// The calling thread's device state if ya_init (or an earlier call) created
+// it; never creates it (that allocates).
+static int existing_device(DeviceState **out) {
+  int dev = t_device;
+  if (dev < 0) {
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(YA_ERR_NO_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+  }
+  std::lock_guard<std::mutex> g(g_dev_mu);
+  if ((int)g_devs.size() <= dev || !g_devs[dev])
+    return fail(YA_ERR_INVALID, "device %d is not initialised: call ya_init(%d) first", dev, dev);
+  *out = g_devs[dev].get();
+  return YA_OK;
+}
+
+// YA_OK if the tables of (2^logn, cm) for cm in [cm0, cm0 + count) are all
+// resident (builds nothing).
+static int tables_resident(DeviceState *st, int logn, size_t cm0, size_t count) {
+  for (size_t cm = cm0; cm < cm0 + count; ++cm)
+    if (!st->slots[slot_index(logn, cm)].ready.load(std::memory_order_acquire))
+      return fail(YA_ERR_INVALID,
+                  "tables of degree 2^%d, P[%zu] are not resident: call ya_ntt_init_tables(%zu, %zu) first", logn,
+                  cm, (size_t)1 << logn, cm0 + count);
   return YA_OK;
 }
 
@@ -827,8 +865,11 @@ int ya_poly_add_product_of(uint64_t *acc, const uint64_t *a, const uint64_t *b, 
   return pw_common(PW_MULADD, acc, a, b, degree, nmoduli, stream);
 }
 
+// noalloc (ya_poly_multiply_noalloc): the device state and the tables must
+// already exist, and the scratch is the device's static array instead of a
+// stream-ordered allocation.
 static int polymul_impl(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
-                        size_t nmoduli, size_t count, void *stream) {
+                        size_t nmoduli, size_t count, void *stream, bool noalloc) {
   int logn;
   int rc = check_shape(degree, 0, nmoduli, &logn);
   if (rc) return rc;
@@ -838,50 +879,66 @@ static int polymul_impl(uint64_t *c, const uint64_t *a, const uint64_t *b, size_
   if ((c < a + words && a < c + words) || (c < b + words && b < c + words))
     return fail(YA_ERR_INVALID, "c must not overlap a or b");
   DeviceState *st;
-  if ((rc = current_device(&st))) return rc;
-  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  if (noalloc) {
+    if ((rc = existing_device(&st))) return rc;
+    if ((rc = tables_resident(st, logn, 0, nmoduli))) return rc;
+  } else {
+    if ((rc = current_device(&st))) return rc;
+    if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  }
+  // scratch: the split launch keeps forward(b)'s top layers of every residue,
+  // a large ring one polynomial's forward(b); the fused kernel needs none
+  const size_t nres = nmoduli * count;
+  const int ksplit = logn <= kMaxBlockLog ? split_factor(st, logn, nres) : 0;
+  const size_t need = logn > kMaxBlockLog ? nmoduli << logn : ksplit ? nres << logn : 0;
+  if (noalloc && need > kScratchWords)
+    return fail(YA_ERR_INVALID,
+                "degree %zu x %zu moduli needs %zu words of scratch; the static scratch holds %zu words", degree,
+                nmoduli, need, kScratchWords);
   const ModDesc *d = desc_ptr(st, logn, 0);
   hipStream_t s = (hipStream_t)stream;
-  if (logn <= kMaxBlockLog) {
-    const size_t nres = nmoduli * count;
-    if (const int k = split_factor(st, logn, nres)) {
-      // few residues: spread each over 2^k workgroups; forward(b)'s top
-      // layers go to stream-ordered scratch (a and b are left untouched)
-      uint64_t *tmp = nullptr;
-      hipError_t e = hipMallocAsync((void **)&tmp, (nres << logn) * 8, s);
-      if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMallocAsync: %s", hipGetErrorString(e));
-      const SplitArgs sa{nullptr, a, b, c, d, (uint32_t)nmoduli, tmp};
-      e = launch_polymul_split(sa, logn, nres, k, s);
-      const hipError_t ef = hipFreeAsync(tmp, s);
-      if (e == hipSuccess) e = ef;
-      return e == hipSuccess ? YA_OK : hip_fail(e, "split polymul launch");
-    }
-    hipError_t e = launch_polymul(c, a, b, d, logn, nres, nmoduli, tuned_shape(logn, 1), s);
-    return e == hipSuccess ? YA_OK : hip_fail(e, "polymul launch");
-  }
-  // Large rings: forward(a) into c, forward(b) into stream-ordered scratch,
-  // c = c * scratch, backward(c); polynomial by polynomial.
-  const size_t pw = degree * nmoduli;
   uint64_t *tmp = nullptr;
-  hipError_t e = hipMallocAsync((void **)&tmp, pw * 8, s);
-  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipMallocAsync: %s", hipGetErrorString(e));
-  const NttShape sh = tuned_shape(logn, 0);
-  for (size_t k = 0; e == hipSuccess && k < count; ++k) {
-    uint64_t *ck = c + k * pw;
-    e = hipMemcpyAsync(ck, a + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(tmp, b + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = launch_forward(ck, d, logn, nmoduli, nmoduli, true, sh, s);
-    if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, nmoduli, true, sh, s);
-    for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
-      const size_t nr = std::min<size_t>(65535, nmoduli - r0);
-      e = launch_pointwise(PW_MUL, ck + r0 * degree, ck + r0 * degree, tmp + r0 * degree, d + r0,
-                           degree, nr, s);
+  hipError_t e = hipSuccess;
+  if (need) {
+    if (noalloc) {
+      tmp = st->scratch;
+    } else if ((e = hipMallocAsync((void **)&tmp, need * 8, s)) != hipSuccess) {
+      return fail(YA_ERR_ALLOC, "hipMallocAsync: %s", hipGetErrorString(e));
     }
-    if (e == hipSuccess) e = launch_backward(ck, d, logn, nmoduli, nmoduli, sh, s);
   }
-  hipError_t ef = hipFreeAsync(tmp, s);
-  if (e == hipSuccess) e = ef;
-  return e == hipSuccess ? YA_OK : hip_fail(e, "polymul (large ring)");
+  if (logn <= kMaxBlockLog) {
+    if (ksplit) {
+      // few residues: spread each over 2^k workgroups; forward(b)'s top
+      // layers go to the scratch (a and b are left untouched)
+      const SplitArgs sa{nullptr, a, b, c, d, (uint32_t)nmoduli, tmp};
+      e = launch_polymul_split(sa, logn, nres, ksplit, s);
+    } else {
+      e = launch_polymul(c, a, b, d, logn, nres, nmoduli, tuned_shape(logn, 1), s);
+    }
+  } else {
+    // Large rings: forward(a) into c, forward(b) into the scratch,
+    // c = c * scratch, backward(c); polynomial by polynomial.
+    const size_t pw = degree * nmoduli;
+    const NttShape sh = tuned_shape(logn, 0);
+    for (size_t k = 0; e == hipSuccess && k < count; ++k) {
+      uint64_t *ck = c + k * pw;
+      e = hipMemcpyAsync(ck, a + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(tmp, b + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = launch_forward(ck, d, logn, nmoduli, nmoduli, true, sh, s);
+      if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, nmoduli, true, sh, s);
+      for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
+        const size_t nr = std::min<size_t>(65535, nmoduli - r0);
+        e = launch_pointwise(PW_MUL, ck + r0 * degree, ck + r0 * degree, tmp + r0 * degree, d + r0,
+                             degree, nr, s);
+      }
+      if (e == hipSuccess) e = launch_backward(ck, d, logn, nmoduli, nmoduli, sh, s);
+    }
+  }
+  if (tmp && !noalloc) {
+    const hipError_t ef = hipFreeAsync(tmp, s);
+    if (e == hipSuccess) e = ef;
+  }
+  return e == hipSuccess ? YA_OK : hip_fail(e, "polymul launch");
 }
 
 int ya_split_policy(int mode) {
@@ -892,12 +949,17 @@ int ya_split_policy(int mode) {
 
 int ya_poly_multiply(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
                      size_t nmoduli, void *stream) {
-  return polymul_impl(c, a, b, degree, nmoduli, 1, stream);
+  return polymul_impl(c, a, b, degree, nmoduli, 1, stream, false);
+}
+
+int ya_poly_multiply_noalloc(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
+                             size_t nmoduli, void *stream) {
+  return polymul_impl(c, a, b, degree, nmoduli, 1, stream, true);
 }
 
 int ya_poly_multiply_many(uint64_t *c, const uint64_t *a, const uint64_t *b, size_t degree,
                           size_t nmoduli, size_t count, void *stream) {
-  return polymul_impl(c, a, b, degree, nmoduli, count, stream);
+  return polymul_impl(c, a, b, degree, nmoduli, count, stream, false);
 }
 
 }  // extern "C"
