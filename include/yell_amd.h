@@ -20,8 +20,12 @@
  * Preconditions (as the reference, lib/ntt.cpp:45-46,66-68, ops.hpp:64,79):
  * forward inputs < 4p, backward inputs < 2p, add/sub inputs < p.
  *
- * Supported sizes: degree N = 2^k with 3 <= k <= 17; nmoduli L >= 1 with
- * (first prime index + L) <= ya_num_moduli() (1000).
+ * Supported sizes: degree N = 2^k with 2 <= k <= 19 (the reference's poly<N>
+ * takes N < kMaxPolyDegree = 2^20, include/yell/poly.hpp:45,
+ * params_64.hpp:21; its forward loop needs N >= 4, lib/ntt.cpp:123-135;
+ * at N = 4 ntt<N>::backward returns the reference's own words, which apply
+ * the top layer twice, lib/ntt.cpp:161-192); the samplers need N >= 8;
+ * nmoduli L >= 1 with (first prime index + L) <= ya_num_moduli() (1000).
  */
 #ifndef YELL_AMD_H
 #define YELL_AMD_H

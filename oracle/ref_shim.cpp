@@ -154,6 +154,16 @@ struct Ops {
   }
 };
 
+// 2^18 and 2^19 exist only in the 64-bit build (kMaxPolyDegree 2^20,
+// params_64.hpp:21; the 32-bit build stops at 2^15, params_32.hpp).
+#ifdef YELL_USE_32BITS_MODULI
+#define YREF_LARGE_CASES(EXPR)
+#else
+#define YREF_LARGE_CASES(EXPR)                                                    \
+    case 18: { using O = Ops<(1u << 18)>; EXPR; } break;                          \
+    case 19: { using O = Ops<(1u << 19)>; EXPR; } break;
+#endif
+
 #define YREF_SWITCH(logn, EXPR)                                                   \
   switch (logn) {                                                                 \
     case 2: { using O = Ops<(1u << 2)>; EXPR; } break;                            \
@@ -172,6 +182,7 @@ struct Ops {
     case 15: { using O = Ops<(1u << 15)>; EXPR; } break;                          \
     case 16: { using O = Ops<(1u << 16)>; EXPR; } break;                          \
     case 17: { using O = Ops<(1u << 17)>; EXPR; } break;                          \
+    YREF_LARGE_CASES(EXPR)                                                        \
     default: return -1;                                                           \
   }
 

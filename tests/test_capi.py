@@ -72,12 +72,14 @@ def test_invalid_arguments_rejected():
     lib = capi.load()
     buf = np.zeros(16, dtype=np.uint64)
     assert lib.ya_sample_uniform(capi.ptr(buf), 12, 1, 0) == 1        # not a power of two
-    assert lib.ya_sample_uniform(capi.ptr(buf), 4, 1, 0) == 1         # below 2^3
+    assert lib.ya_sample_uniform(capi.ptr(buf), 4, 1, 0) == 1         # samplers: below 2^3
     assert lib.ya_sample_uniform(capi.ptr(buf), 8, 1001, 0) == 1      # too many moduli
     assert lib.ya_sample_uniform(None, 8, 1, 0) == 1
     assert lib.ya_sample_gaussian(capi.ptr(buf), 8, 1, -1.0, 0) == 1
     assert b"degree" in lib.ya_last_error() or b"sigma" in lib.ya_last_error()
-    assert lib.ya_poly_forward(None, 1 << 18, 1, None) == 1           # 2^18 unsupported
+    assert lib.ya_poly_forward(None, 1 << 20, 1, None) == 1           # 2^20 unsupported (N < 2^20)
+    assert b"degree" in lib.ya_last_error()
+    assert lib.ya_poly_forward(None, 2, 1, None) == 1                 # 2 unsupported
     assert lib.ya_ntt_table_host(8, 1000, capi.ptr(buf), capi.ptr(buf), capi.ptr(buf),
                                  capi.ptr(buf), capi.ptr(buf)) == 1
     assert lib.ya_status_string(8) == b"tables do not match"

@@ -15,7 +15,7 @@ from yell_amd import capi
 pytestmark = pytest.mark.gpu
 O = Oracle(64)
 
-SHAPES = [(4096, 3, 3), (8192, 4, 5), (16384, 2, 2), (65536, 2, 3)]
+SHAPES = [(4, 3, 3), (4096, 3, 3), (8192, 4, 5), (16384, 2, 2), (65536, 2, 3), (1 << 19, 1, 2)]
 
 
 def _polys(rng, n, L, P, count, mult=1):

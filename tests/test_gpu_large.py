@@ -14,10 +14,10 @@ orc = Oracle()
 P = [orc.prime(i) for i in range(32)]
 
 
-@pytest.mark.parametrize("logn", [10, 14, 15, 16])
+@pytest.mark.parametrize("logn", [2, 10, 14, 15, 16, 18, 19])
 def test_many_transforms(logn):
     device()
-    n, L, count = 1 << logn, 3, 3
+    n, L, count = 1 << logn, (3 if logn <= 16 else 1), 3
     rng = np.random.default_rng(logn)
     x = np.concatenate([rand_bound(rng, n, L, 4, P) for _ in range(count)])
     d = capi.DeviceBuffer.from_host(x)
@@ -74,7 +74,7 @@ def test_paired_forward_many(logn, le):
         capi.call("ya_pair_forward", 1)
 
 
-@pytest.mark.parametrize("logn", [15, 16, 17])
+@pytest.mark.parametrize("logn", [15, 16, 17, 18, 19])
 def test_paired_forward_large_rings(logn):
     """Above 2^14 the paired forward runs the top layers per residue, then
     block b of residue r of polynomials 2q and 2q+1 in one workgroup: the
@@ -82,7 +82,7 @@ def test_paired_forward_large_rings(logn):
     alone), lazy and corrected, inputs up to 4p - 1, and the same words with
     pairing off."""
     device()
-    n, L = 1 << logn, 2
+    n, L = 1 << logn, (2 if logn <= 17 else 1)
     rng = np.random.default_rng(logn)
     try:
         for count in (2, 3):

@@ -15,10 +15,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = np.load(os.path.join(HERE, "golden", "ntt_ref.npz"))
 orc = Oracle()
 P = [orc.prime(i) for i in range(32)]
-LOGNS = list(range(3, 18))
+# every supported degree: 2^2 (the reference's own N = 4 words, including its
+# backward there) .. 2^19 (the reference requires N < 2^20)
+LOGNS = list(range(2, 20))
 
 
-@pytest.mark.parametrize("logn", [3, 10, 14, 16, 17])
+@pytest.mark.parametrize("logn", [2, 3, 10, 14, 16, 17, 18, 19])
 def test_resident_tables(logn):
     device()
     n = 1 << logn
@@ -51,7 +53,7 @@ def test_poly_backward(logn):
                                   orc.poly_backward(x, n, L))
 
 
-@pytest.mark.parametrize("logn", [4, 12, 15])
+@pytest.mark.parametrize("logn", [2, 4, 12, 15, 19])
 def test_single_residue_with_modulus_index(logn):
     n, cm = 1 << logn, 9
     rng = np.random.default_rng(7)
@@ -64,7 +66,7 @@ def test_single_residue_with_modulus_index(logn):
     np.testing.assert_array_equal(run_inplace("ya_ntt_backward", y, n, cm), orc.backward(y, n, cm))
 
 
-@pytest.mark.parametrize("logn", [3, 8, 13, 14, 15, 17])
+@pytest.mark.parametrize("logn", [2, 3, 8, 13, 14, 15, 17, 18, 19])
 def test_host_seam(logn):
     device()
     n, cm = 1 << logn, 4
@@ -86,14 +88,14 @@ def test_host_seam(logn):
     assert lib.ya_negacylic_forward_lazy(capi.ptr(y), n, capi.ptr(iphi), capi.ptr(iphis), p) == 8
 
 
-@pytest.mark.parametrize("logn", [3, 11, 14, 16])
+@pytest.mark.parametrize("logn", [2, 3, 11, 14, 16, 19])
 def test_pointwise(logn):
     n, L = 1 << logn, 3
     rng = np.random.default_rng(300 + logn)
     a = rand_bound(rng, n, L, 1, P)
     b = rand_bound(rng, n, L, 1, P)
     acc = rand_bound(rng, n, L, 1, P)
-    a[:5] = 0
+    a[:3] = 0
     say("pointwise", logn)
     np.testing.assert_array_equal(run_binary("ya_poly_mul", a, b, n, L), orc.poly_mul(a, b, n, L))
     np.testing.assert_array_equal(run_binary("ya_poly_add", a, b, n, L), orc.poly_add(a, b, n, L))
@@ -143,7 +145,7 @@ def test_polymul_c1_schoolbook():
 
 
 def test_golden_vectors():
-    for n in (8, 64, 1024):
+    for n in (4, 8, 64, 1024):
         for cm in (0, 7):
             k = "n%d_cm%d" % (n, cm)
             say("golden", k)
@@ -159,7 +161,7 @@ def test_golden_vectors():
                                       GOLD["pm_c_" + k])
 
 
-@pytest.mark.parametrize("logn", [3, 14, 16])
+@pytest.mark.parametrize("logn", [2, 3, 14, 16, 19])
 def test_extreme_values(logn):
     n, L = 1 << logn, 2
     say("extremes", logn)

@@ -835,7 +835,7 @@ int ya_poly_add_product_backward_many(uint64_t *out, const uint64_t *acc, const 
   const hipStream_t s = (hipStream_t)stream;
   const NttShape sh = tuned_shape(logn, 0);
   hipError_t e;
-  if (logn <= kMaxBlockLog) {
+  if (logn >= 3 && logn <= kMaxBlockLog) {
     e = launch_mac_inverse(out, acc, x, key, d, logn, nmoduli * count, nmoduli, sh, s);
   } else {
     e = out == acc ? hipSuccess : hipMemcpyAsync(out, acc, words * 8, hipMemcpyDeviceToDevice, s);

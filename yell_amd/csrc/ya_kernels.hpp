@@ -12,8 +12,11 @@ namespace ya {
 
 // Largest block handled in one workgroup's LDS (2^14 words = 128 KiB + padding).
 constexpr int kMaxBlockLog = 14;
-constexpr int kMinLogN = 3;
-constexpr int kMaxLogN = 17;
+// Degrees 2^2 .. 2^19: the reference's poly<N> requires N < kMaxPolyDegree =
+// 2^20 (include/yell/poly.hpp:45, params_64.hpp:21), and its forward loop
+// reads past a residue of 2 words (lib/ntt.cpp:123-135 assumes h = 2 exists).
+constexpr int kMinLogN = 2;
+constexpr int kMaxLogN = 19;
 
 // Elements per thread (log2) chosen for a block of 2^lb words; 0 = default.
 struct NttShape {

@@ -176,6 +176,50 @@ k_inv_block(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint
   inv_block<LB, LE, TOP>(x, base, lds, d.inv, nlog, blk << LB, d, t);
 }
 
+// Residues of 4 words, one per thread, in the reference's exact layer order
+// at that size: forward (lib/ntt.cpp:122-147; its main loop is empty) runs
+// the layers on bits 1 and 0.  The reference's inverse at N = 4 runs both of
+// its special-cased loops (:161-187, bits 0 and 1) and its main loop not at
+// all, so negacylic_backward_lazy does not skip the top layer there, and
+// ntt<N>::backward then applies the merged layer on bit 1 once more
+// (include/yell/details/ntt.hpp:151-159): not the inverse transform, but the
+// reference's words, reproduced.  MODE 0 forward_lazy, 1 forward, 2 backward,
+// 3 backward_lazy.
+constexpr int kSmallThreads = 64;
+
+template <int MODE>
+__global__ void __launch_bounds__(kSmallThreads)
+k_ntt4(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nres, uint32_t nmod) {
+  const uint32_t v = blockIdx.x * kSmallThreads + threadIdx.x;
+  if (v >= nres) return;
+  const ModDesc d = descs[v % nmod];
+  uint64_t *base = data + ((size_t)v << 2);
+  uint64_t x[4];
+  gmem_load<0, 2>(base, x, 0u);
+  if constexpr (MODE <= 1) {
+    fwd_layers<0, 2, 2, 0>(x, d.fwd, 2u, 0u, d.p);
+    if constexpr (MODE == 1) correct_all<2>(x, d.p);
+  } else {
+    inv_layers<0, 2, 2, 0>(x, d.inv, 2u, 0u, d.p);
+    if constexpr (MODE == 2) inv_last_layer<2, 1>(x, d);
+  }
+  gmem_store<0, 2>(base, x, 0u);
+}
+
+static hipError_t launch_ntt4(uint64_t *data, const ModDesc *descs, size_t nres, size_t nmod, int mode,
+                              hipStream_t st) {
+  if (nres > 0xffffffffu) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((nres + kSmallThreads - 1) / kSmallThreads));
+  const uint32_t n32 = (uint32_t)nres, m32 = (uint32_t)nmod;
+  switch (mode) {
+    case 0: hipLaunchKernelGGL(k_ntt4<0>, grid, dim3(kSmallThreads), 0, st, data, descs, n32, m32); break;
+    case 1: hipLaunchKernelGGL(k_ntt4<1>, grid, dim3(kSmallThreads), 0, st, data, descs, n32, m32); break;
+    case 2: hipLaunchKernelGGL(k_ntt4<2>, grid, dim3(kSmallThreads), 0, st, data, descs, n32, m32); break;
+    default: hipLaunchKernelGGL(k_ntt4<3>, grid, dim3(kSmallThreads), 0, st, data, descs, n32, m32); break;
+  }
+  return hipGetLastError();
+}
+
 // Column kernels for N = 2^(B+K): one thread per column c in [0, 2^B) holding
 // the 2^K words c + e * 2^B (the layers on index bits >= B; the blocks of
 // 2^B words take the rest).
@@ -215,7 +259,7 @@ k_inv_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32
 // Block size of a ring above 2^kMaxBlockLog: 2^14 words, or 2^13 / 2^12 when
 // the launch would otherwise have fewer workgroups than the device has CUs
 // (a few residues: C4's single inverse is 24 residues of 2^16), keeping the
-// column pass to at most 4 layers.
+// column pass to at most 4 layers (5 at 2^19: 32 words per thread).
 constexpr size_t kFillWorkgroups = 256;
 static int large_block_log(int logn, size_t nres) {
   int lb = kMaxBlockLog;
@@ -232,6 +276,7 @@ static hipError_t fwd_top_b(uint64_t *data, const ModDesc *descs, int k, size_t 
     case 2: hipLaunchKernelGGL((k_fwd_top<B, 2>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
     case 3: hipLaunchKernelGGL((k_fwd_top<B, 3>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
     case 4: hipLaunchKernelGGL((k_fwd_top<B, 4>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 5: hipLaunchKernelGGL((k_fwd_top<B, 5>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -258,6 +303,7 @@ static hipError_t inv_top_b(uint64_t *data, const ModDesc *descs, int k, size_t 
     case 2: hipLaunchKernelGGL((k_inv_top<B, 2, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
     case 3: hipLaunchKernelGGL((k_inv_top<B, 3, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
     case 4: hipLaunchKernelGGL((k_inv_top<B, 4, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 5: hipLaunchKernelGGL((k_inv_top<B, 5, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -332,6 +378,7 @@ hipError_t inv_lb(uint64_t *data, const ModDesc *descs, int logn, size_t nres, s
 hipError_t launch_forward(uint64_t *data, const ModDesc *descs, int logn, size_t nres, size_t nmod,
                           bool lazy, NttShape shape, hipStream_t st) {
   if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0) return hipErrorInvalidValue;
+  if (logn == 2) return launch_ntt4(data, descs, nres, nmod, lazy ? 0 : 1, st);
   const int lb = logn <= kMaxBlockLog ? logn : large_block_log(logn, nres);
   if (logn > lb) {
     const hipError_t e = launch_fwd_top(data, descs, logn, lb, nres, nmod, st);
@@ -411,7 +458,7 @@ hipError_t launch_forward_pairs_mac(uint64_t *acc, const uint64_t *add, const ui
 hipError_t launch_mac_inverse(uint64_t *out, const uint64_t *acc, const uint64_t *x, const uint64_t *key,
                               const ModDesc *descs, int logn, size_t nres, size_t nmod, NttShape shape,
                               hipStream_t st) {
-  if (logn < kMinLogN || logn > kMaxBlockLog || nres == 0 || nmod == 0 || nres > 0x7fffffffu)
+  if (logn < 3 || logn > kMaxBlockLog || nres == 0 || nmod == 0 || nres > 0x7fffffffu)
     return hipErrorInvalidValue;
   const int le = clamp_le(logn, shape.le);
 #define YA_MAC_INV_CALL(LB, LE) mac_inv_lb<LB, LE>(out, acc, x, key, descs, nres, nmod, st)
@@ -422,6 +469,7 @@ hipError_t launch_mac_inverse(uint64_t *out, const uint64_t *acc, const uint64_t
 static hipError_t backward_impl(uint64_t *data, const ModDesc *descs, int logn, size_t nres,
                                 size_t nmod, bool lazy, NttShape shape, hipStream_t st) {
   if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0) return hipErrorInvalidValue;
+  if (logn == 2) return launch_ntt4(data, descs, nres, nmod, lazy ? 3 : 2, st);
   const int lb = logn <= kMaxBlockLog ? logn : large_block_log(logn, nres);
   const int le = clamp_le(lb, shape.le);
   hipError_t e;

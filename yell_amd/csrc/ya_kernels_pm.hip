@@ -75,6 +75,33 @@ k_polymul_tab(const BatchRes *__restrict__ res, uint32_t nres) {
   polymul_residue<LB, LE, DUAL>(r.c, r.a, r.b, d, lds, threadIdx.x);
 }
 
+// Poly-multiplication of residues of 4 words, one per thread: forward (the
+// reference's layers on bits 1, 0, corrected), the Barrett product, and the
+// reference's backward at that size (both GS layers, then the merged layer on
+// bit 1 again; see k_ntt4), so the words are the reference's sequence
+// a.forward(); b.forward(); c = a * b; c.backward() at N = 4.
+constexpr int kPm4Threads = 64;
+
+__global__ void __launch_bounds__(kPm4Threads)
+k_polymul4(uint64_t *__restrict__ c, const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
+           const ModDesc *__restrict__ descs, uint32_t nres, uint32_t nmod) {
+  const uint32_t v = blockIdx.x * kPm4Threads + threadIdx.x;
+  if (v >= nres) return;
+  const ModDesc d = descs[v % nmod];
+  const size_t off = (size_t)v << 2;
+  uint64_t x[4], y[4];
+  gmem_load<0, 2>(a + off, x, 0u);
+  gmem_load<0, 2>(b + off, y, 0u);
+  fwd_layers2<0, 2, 2, 0>(x, y, d.fwd, 2u, 0u, d.p);
+  correct_all<2>(x, d.p);
+  correct_all<2>(y, d.p);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) x[e] = mulmod_barrett(x[e], y[e], d.p, d.pn);
+  inv_layers<0, 2, 2, 0>(x, d.inv, 2u, 0u, d.p);
+  inv_last_layer<2, 1>(x, d);
+  gmem_store<0, 2>(c + off, x, 0u);
+}
+
 namespace {
 
 // The dual-forward form is instantiated for the shapes where it can pay
@@ -128,6 +155,12 @@ hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, con
   if (logn < kMinLogN || logn > kMaxLogN || nres == 0 || nmod == 0 || nres % nmod)
     return hipErrorInvalidValue;
   if (logn > kMaxBlockLog) return hipErrorNotSupported;  // composed by the caller
+  if (logn == 2) {
+    if (nres > 0xffffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_polymul4, dim3((unsigned)((nres + kPm4Threads - 1) / kPm4Threads)), dim3(kPm4Threads), 0,
+                       st, c, a, b, descs, (uint32_t)nres, (uint32_t)nmod);
+    return hipGetLastError();
+  }
   const int le = clamp_le(logn, shape.le);
 #define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, nmod, shape.dual != 0, st)
   YA_LB_SWITCH(logn, le, YA_PM_CALL)

@@ -156,10 +156,7 @@ int sample_hwt(W *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t see
 uint64_t prime64(size_t cm) { return ya::prime(cm); }
 uint64_t prime32(size_t cm) { return ya32_prime(cm); }
 
-int shape64(size_t degree, size_t nmoduli) {
-  int logn;
-  return ya::check_shape(degree, 0, nmoduli, &logn);
-}
+int shape64(size_t degree, size_t nmoduli) { return sample_shape(degree, nmoduli, ya::kMaxLogN, ya_num_moduli()); }
 int shape32(size_t degree, size_t nmoduli) { return sample_shape(degree, nmoduli, 15, ya32_num_moduli()); }
 
 }  // namespace

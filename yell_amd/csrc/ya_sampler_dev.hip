@@ -264,7 +264,9 @@ int dev_shape(const W *out, size_t degree, size_t nmoduli, int *logn) {
   if (!out) return fail(YA_ERR_INVALID, "null output");
   if (((uintptr_t)out) % 16) return fail(YA_ERR_INVALID, "output must be 16-byte aligned");
   if constexpr (sizeof(W) == 8) {
-    return check_shape(degree, 0, nmoduli, logn);
+    const int rc = check_shape(degree, 0, nmoduli, logn);
+    if (rc == YA_OK && *logn < 3) return fail(YA_ERR_INVALID, "degree %zu: the samplers need 8 words or more", degree);
+    return rc;
   } else {
     const int l = log2_degree(degree);
     if (l < 3 || l > 15) return fail(YA_ERR_INVALID, "degree %zu: a power of two in [8, 2^15] is required", degree);
