@@ -4,33 +4,38 @@
 //
 // Same constructors, samplers, NTT methods, operators and accessors as the
 // reference; the differences a caller sees:
-//   * storage is one residue-major device buffer [L][N] (ptr_at(cm) returns a
-//     device pointer; data() the base); raw_data() returns the residue
-//     pointers as a vector instead of T**;
-//   * element access operator()(cm, i) reads / writes one word across PCIe;
-//     to_host() / from_host() move whole polynomials;
-//   * samplers are seeded (yell::set_seed); gaussian takes sigma directly
-//     (the reference wraps an MPFR-built FastGaussianNoise table);
+//   * storage is one residue-major device buffer [L][N]: ptr_at(cm) and the
+//     residue pointers of raw_data() are device pointers (data() is the
+//     base), so host code must not dereference them;
+//   * element access operator()(cm, i) reads one word across PCIe; on a
+//     non-const poly it returns a reference-like proxy, so p(cm, i) = v also
+//     writes one word; to_host() / from_host() move whole polynomials;
+//   * samplers are seeded (yell::set_seed); yell::gaussian<In, Out, D> wraps
+//     a yell::FastGaussianNoise<In, Out, D> as in the reference, drawn on
+//     the device from that sampler's sigma (see yell/prng/FastGaussianNoise.hpp);
 //   * with -DYELL_USE_32BITS_MODULI the 32-bit build (uint32 words, 30-bit
-//     primes, degrees up to 2^14) is used, as in the reference;
+//     primes, degrees 2^3 .. 2^14) is used, as in the reference;
 //   * poly2mpz (the CRT lift, include/yell/gmp.hpp:41-70) runs on the GPU
 //     (64-bit build);
 //     it is declared when <gmp.h> is on the include path (link -lgmp), and
 //     to_limbs() gives the same integers as little-endian 64-bit limbs
 //     without GMP;
-//   * failures throw std::runtime_error (the reference asserts).
+//   * failures throw std::runtime_error, out-of-range indices
+//     std::out_of_range (the reference asserts).
 // Every arithmetic result is bit-identical to the reference's.
 #pragma once
 #include <array>
 #include <atomic>
 #include <cstring>
-#include <ostream>
+#include <iostream>
 #include <stdexcept>
 #include <vector>
 
 #include "yell/ntt.hpp"
 #include "yell/ops.hpp"
 #include "yell/params.hpp"
+#include "yell/prng/FastGaussianNoise.hpp"
+#include "yell/prng/seed.hpp"
 #include "yell_amd.h"
 #if __has_include(<gmp.h>)
 #include <gmp.h>
@@ -97,29 +102,32 @@ struct ZO_dist {
   uint8_t rho;
   explicit ZO_dist(uint8_t r = 0x7F) : rho(r) {}
 };
+// include/yell/poly.hpp:32-41: the Gaussian generator, a sampler plus the
+// amplifier the signed samples are multiplied by.
+template <class in_class, class out_class, unsigned _lu_depth>
 struct gaussian {
-  double sigma;
+  FastGaussianNoise<in_class, out_class, _lu_depth> *fg_prng;
   uint64_t amplifier;
-  explicit gaussian(double s = 3.2, uint64_t amp = 1) : sigma(s), amplifier(amp) {}
+  explicit gaussian(FastGaussianNoise<in_class, out_class, _lu_depth> *prng) : fg_prng{prng}, amplifier{1} {}
+  gaussian(FastGaussianNoise<in_class, out_class, _lu_depth> *prng, uint64_t amp) : fg_prng{prng}, amplifier{amp} {}
 };
 
 namespace detail {
-inline std::atomic<uint64_t> &seed_counter() {
-  static std::atomic<uint64_t> s{0x5eed5eed5eedULL};
-  return s;
-}
+constexpr size_t log2_of(size_t n) { return n <= 1 ? 0 : 1 + log2_of(n >> 1); }
+// Smallest degree of this build: the 64-bit engine runs the reference's
+// N = 4 (its forward needs N >= 4, lib/ntt.cpp:123-135), the 32-bit one 2^3.
+#ifdef YELL_USE_32BITS_MODULI
+constexpr size_t kMinDegree = 8;
+#else
+constexpr size_t kMinDegree = 4;
+#endif
 }  // namespace detail
-
-// Seeds the samplers used by poly constructors / set(); each sampling call
-// draws the next seed, so a program's inputs are reproducible.
-inline void set_seed(uint64_t s) { detail::seed_counter().store(s); }
-inline uint64_t next_seed() { return detail::seed_counter().fetch_add(0x9E3779B97F4A7C15ULL); }
 
 template <size_t degree_>
 class poly {
-  static_assert(degree_ >= 8 && degree_ <= (1u << 17) && degree_ < params::kMaxPolyDegree &&
-                    (degree_ & (degree_ - 1)) == 0,
-                "degree must be a power of two in [2^3, 2^17] below params::kMaxPolyDegree");
+  // the reference asserts degree_ < kMaxPolyDegree (include/yell/poly.hpp:45)
+  static_assert(degree_ >= detail::kMinDegree && degree_ < params::kMaxPolyDegree && (degree_ & (degree_ - 1)) == 0,
+                "degree must be a power of two of at least 4 (8 with 32-bit moduli) below params::kMaxPolyDegree");
 
  public:
   using value_type = params::value_type;
@@ -130,14 +138,22 @@ class poly {
   using iterator = pointer_type;
   using const_iterator = const_pointer_type;
   static constexpr size_t degree = degree_;
+  static constexpr size_t logn = detail::log2_of(degree_);
 
   explicit poly(size_t nmoduli) : nmoduli_(0), d_(nullptr) { resize_moduli_count(nmoduli); clear(); }
-  poly(size_t nmoduli, const uniform &m) : poly(nmoduli) { set(m); }
-  poly(size_t nmoduli, const hwt_dist &m) : poly(nmoduli) { set(m); }
-  poly(size_t nmoduli, const ZO_dist &m) : poly(nmoduli) { set(m); }
-  poly(size_t nmoduli, const gaussian &m) : poly(nmoduli) { set(m); }
+  explicit poly(size_t nmoduli, const uniform &m) : poly(nmoduli) { set(m); }
+  explicit poly(size_t nmoduli, const hwt_dist &m) : poly(nmoduli) { set(m); }
+  explicit poly(size_t nmoduli, const ZO_dist &m) : poly(nmoduli) { set(m); }
+  template <class in_class, unsigned _lu_depth>
+  explicit poly(size_t nmoduli, const gaussian<in_class, value_type, _lu_depth> &m) : poly(nmoduli) {
+    set(m);
+  }
   poly(const poly &o) : poly(o.nmoduli_) { copy_from(o); }
-  poly(poly &&o) noexcept : nmoduli_(o.nmoduli_), d_(o.d_) { o.d_ = nullptr; o.nmoduli_ = 0; }
+  poly(poly &&o) noexcept : nmoduli_(o.nmoduli_), d_(o.d_), ptrs_(std::move(o.ptrs_)) {
+    o.d_ = nullptr;
+    o.nmoduli_ = 0;
+    o.ptrs_.clear();
+  }
   poly &operator=(const poly &o) {
     if (this != &o) {
       resize_moduli_count(o.nmoduli_);
@@ -150,6 +166,7 @@ class poly {
   friend void swap(poly &a, poly &b) {
     std::swap(a.d_, b.d_);
     std::swap(a.nmoduli_, b.nmoduli_);
+    std::swap(a.ptrs_, b.ptrs_);
   }
 
   // ---- samplers (details/poly.hpp:200-330), drawn on the device into the
@@ -163,11 +180,15 @@ class poly {
   void set(const hwt_dist &m) {
     sample([&](uint64_t s) { return detail::api_sample_hwt(d_, degree, nmoduli_, m.hwt, s); });
   }
-  // r *= amplifier on the signed sample (details/poly.hpp:248-251), then r < 0 ? p + r : r.
-  void set(const gaussian &m) {
-    sample([&](uint64_t s) {
-      return detail::api_sample_gaussian(d_, degree, nmoduli_, m.sigma, (uint64_t)m.amplifier, s);
-    });
+  // details/poly.hpp:236-260: one signed sample per coefficient from the
+  // generator's distribution, r *= amplifier in params::signed_type, then
+  // r < 0 ? p + r : r in every residue; drawn on the device with the
+  // sampler's sigma and the next seed.
+  template <class in_class, unsigned _lu_depth>
+  void set(const gaussian<in_class, value_type, _lu_depth> &m) {
+    if (!m.fg_prng) throw std::invalid_argument("gaussian: null FastGaussianNoise");
+    const double sigma = m.fg_prng->sigma();
+    sample([&](uint64_t s) { return detail::api_sample_gaussian(d_, degree, nmoduli_, sigma, m.amplifier, s); });
   }
 
   // ---- access
@@ -177,22 +198,41 @@ class poly {
   const_iterator cptr_end(size_t cm) const { return cptr_at(cm) + degree; }
   value_type *data() { return d_; }
   const value_type *data() const { return d_; }
-  std::vector<value_type *> raw_data() {
-    std::vector<value_type *> v(nmoduli_);
-    for (size_t cm = 0; cm < nmoduli_; ++cm) v[cm] = d_ + cm * degree;
-    return v;
-  }
+  // include/yell/poly.hpp:119-121: the residue pointers (device addresses).
+  value_type **raw_data() { return ptrs_.data(); }
+  value_type *const *raw_data() const { return ptrs_.data(); }
   size_t moduli_count() const { return nmoduli_; }
-  value_type get_modulus(size_t cm) const { return params::P[cm]; }
+  value_type get_modulus(size_t cm) const { check_cm(cm); return params::P[cm]; }
 
+  // One word of residue cm, as the reference's value_type& of
+  // include/yell/poly.hpp:105-108: reads on conversion, writes on assignment.
+  class word_ref {
+   public:
+    word_ref(value_type *dev) : dev_(dev) {}
+    operator value_type() const {
+      value_type v;
+      ya_check(ya_memcpy_dtoh(&v, dev_, sizeof v, nullptr), "read word");
+      return v;
+    }
+    word_ref &operator=(value_type v) {
+      ya_check(ya_memcpy_htod(dev_, &v, sizeof v, nullptr), "write word");
+      return *this;
+    }
+    word_ref &operator=(const word_ref &o) { return *this = static_cast<value_type>(o); }
+    word_ref &operator+=(value_type v) { return *this = static_cast<value_type>(*this) + v; }
+    word_ref &operator-=(value_type v) { return *this = static_cast<value_type>(*this) - v; }
+    friend std::ostream &operator<<(std::ostream &os, const word_ref &r) { return os << static_cast<value_type>(r); }
+
+   private:
+    value_type *dev_;
+  };
+  word_ref operator()(size_t cm, size_t i) { return word_ref(ptr_at(cm) + check_i(i)); }
   value_type operator()(size_t cm, size_t i) const {
     value_type v;
-    ya_check(ya_memcpy_dtoh(&v, cptr_at(cm) + i, sizeof v, nullptr), "read word");
+    ya_check(ya_memcpy_dtoh(&v, cptr_at(cm) + check_i(i), sizeof v, nullptr), "read word");
     return v;
   }
-  void set_word(size_t cm, size_t i, value_type v) {
-    ya_check(ya_memcpy_htod(ptr_at(cm) + i, &v, sizeof v, nullptr), "write word");
-  }
+  void set_word(size_t cm, size_t i, value_type v) { (*this)(cm, i) = v; }
   std::vector<value_type> to_host() const {
     std::vector<value_type> h(degree * nmoduli_);
     if (!h.empty()) ya_check(ya_memcpy_dtoh(h.data(), d_, h.size() * sizeof(value_type), nullptr), "to_host");
@@ -253,7 +293,8 @@ class poly {
     return *this;
   }
   void negate() { if (nmoduli_) ya_check(detail::api_poly_negate(d_, degree, nmoduli_), "negate"); }
-  // a * s computed as a * r + a * (s - r) with a fresh uniform r (details/poly.hpp:134-150).
+  // a * s computed as a * r + a * (s - r) with a fresh uniform r; the masks
+  // are cleared before their memory is released (details/poly.hpp:134-150).
   poly &mul_sensitive_poly(const poly &s) {
     same(s);
     poly r0(nmoduli_, uniform{});
@@ -262,6 +303,8 @@ class poly {
     r1 *= *this;
     *this *= r0;
     *this += r1;
+    r0.clear();
+    r1.clear();
     return *this;
   }
   void clear() { if (nmoduli_) ya_check(ya_memset_zero(d_, bytes(), nullptr), "clear"); }
@@ -281,6 +324,8 @@ class poly {
     if (d_) ya_free(d_);
     d_ = nd;
     nmoduli_ = n;
+    ptrs_.resize(n);
+    for (size_t cm = 0; cm < n; ++cm) ptrs_[cm] = d_ + cm * degree;
   }
   void check_vaild_range() const {
     const auto h = to_host();
@@ -299,12 +344,17 @@ class poly {
   }
   size_t bytes() const { return nmoduli_ * degree * sizeof(value_type); }
   void check_cm(size_t cm) const { if (cm >= nmoduli_) throw std::out_of_range("modulus index"); }
+  static size_t check_i(size_t i) {
+    if (i >= degree) throw std::out_of_range("coefficient index");
+    return i;
+  }
   void same(const poly &o) const {
     if (o.nmoduli_ != nmoduli_) throw std::runtime_error("moduli counts differ");
   }
 
   size_t nmoduli_;
   value_type *d_;
+  std::vector<value_type *> ptrs_;  // residue cm at d_ + cm * degree (raw_data)
 };
 
 // Element-wise operators (include/yell/ops.hpp:7-41, poly.hpp:163-168): the

@@ -24,7 +24,7 @@
  * takes N < kMaxPolyDegree = 2^20, include/yell/poly.hpp:45,
  * params_64.hpp:21; its forward loop needs N >= 4, lib/ntt.cpp:123-135;
  * at N = 4 ntt<N>::backward returns the reference's own words, which apply
- * the top layer twice, lib/ntt.cpp:161-192); the samplers need N >= 8;
+ * the top layer twice, lib/ntt.cpp:161-192);
  * nmoduli L >= 1 with (first prime index + L) <= ya_num_moduli() (1000).
  */
 #ifndef YELL_AMD_H
@@ -394,6 +394,11 @@ int ya_sample_zo(uint64_t *out, size_t degree, size_t nmoduli, uint32_t rho, uin
  * slot pos if pos < hwt.  The j-th smallest hit position is +1 if stream
  * word j has bit 1 set, else p-1. */
 int ya_sample_hwt(uint64_t *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t seed);
+/* FastGaussianNoise::getNoise (include/yell/prng/FastGaussianNoise.hpp:484-602)
+ * over this library's stream: out[i] = the signed sample of coefficient i of
+ * the stream of `seed` (the values ya_sample_gaussian maps to residues), for
+ * any count.  The C++ drop-in's FastGaussianNoise<In, Out, D> draws from it. */
+int ya_gaussian_noise(int64_t *out, size_t count, double sigma, uint64_t seed);
 /* The Gaussian samplers' table: cdf[k] = floor(P(|X| <= k) * 2^64) for
  * k < tail = ceil(13 sigma), cdf[tail] = 2^64-1; *count = tail + 1.  out may
  * be NULL to query the count. */
@@ -405,8 +410,10 @@ int ya_gaussian_cdt(double sigma, uint64_t *out, size_t capacity, size_t *count)
  * with the same seed returns the same words as ya_sample_*.  No counterpart in
  * the reference (its samplers are host-only); they replace host sampling plus
  * the upload.  gaussian: the signed sample r is multiplied by `amplifier`
- * (gaussian::amplifier, details/poly.hpp:248-251; wrapping 64-bit product a)
- * and mapped a < 0 ? p + a : a; amplifier 1 is ya_sample_gaussian.  hwt uses
+ * (gaussian::amplifier, details/poly.hpp:248-251) in the reference's
+ * params::signed_type -- int64_t here, int32_t in the ya32 build
+ * (params_32.hpp), i.e. the product wraps modulo 2^64 (2^32) -- and mapped
+ * a < 0 ? p + a : a modulo 2^64 (2^32); amplifier 1 is ya_sample_gaussian.  hwt uses
  * stream-ordered scratch (hipMallocAsync) of hwt + 2*ceil(degree/32) words. */
 int ya_sample_uniform_device(uint64_t *d_out, size_t degree, size_t nmoduli, uint64_t seed, void *stream);
 int ya_sample_gaussian_device(uint64_t *d_out, size_t degree, size_t nmoduli, double sigma, uint64_t amplifier,

@@ -109,12 +109,19 @@ def gaussian_signed(degree, cdf, seed):
 
 
 def gaussian(primes, degree, cdf, seed, amplifier=1, word=64):
+    """details/poly.hpp:238-257: r *= amplifier in params::signed_type (int64_t,
+    or int32_t in the 32-bit build, params_32.hpp), so the product wraps
+    modulo 2^word; then r < 0 ? p + r : r modulo 2^word."""
     r = gaussian_signed(degree, cdf, seed)
     a = r.astype(np.uint64) * np.uint64(amplifier)  # wrapping 64-bit product
-    neg = a >= np.uint64(1 << 63)
+    if word == 32:
+        a = a & np.uint64(0xFFFFFFFF)               # the int32_t value's bits
+    neg = a >= np.uint64(1 << (word - 1))
     out = np.empty(len(primes) * degree, dtype=_dtype(word))
     for cm, p in enumerate(primes):
         v = np.where(neg, a + np.uint64(int(p)), a)
+        if word == 32:
+            v = v & np.uint64(0xFFFFFFFF)
         out[cm * degree:(cm + 1) * degree] = v.astype(out.dtype)
     return out
 

@@ -27,12 +27,17 @@ static int g_fail = 0;
 
 using T = yell::params::value_type;
 
+// The reference's Gaussian generator, as example.cpp:5-9 declares it.
+using gauss_t = yell::FastGaussianNoise<uint16_t, T, 2>;
+using gauss_struct = yell::gaussian<uint16_t, T, 2>;
+static gauss_t g_prng(3.2, 128, 1 << 14);
+
 // Poly-multiplication through the reference's call sequence equals the fused
 // entry point and a schoolbook negacyclic convolution on sampled coefficients.
 static void test_polymul() {
   constexpr size_t N = 4096;
   const size_t L = 3;
-  yell::poly<N> a(L, yell::uniform{}), b(L, yell::gaussian(3.2));
+  yell::poly<N> a(L, yell::uniform{}), b(L, gauss_struct(&g_prng));
   yell::poly<N> fa(a), fb(b);
   fa.forward();
   fb.forward();
@@ -92,7 +97,7 @@ static void test_rlwe_example() {
   yell::poly<Deg> sx(NdM, yell::hwt_dist(64));  // SK
   sx.forward();
   yell::poly<Deg> pbx(NdM), pax(NdM, yell::uniform{});  // PK
-  pbx.set(yell::gaussian(3.2));
+  pbx.set(gauss_struct(&g_prng));
   pbx.forward();
   pbx.add_product_of(pax, sx);
   pbx.negate();
@@ -100,11 +105,11 @@ static void test_rlwe_example() {
   yell::poly<Deg> cbx(NdM), cax(NdM);  // CTXT
   yell::poly<Deg> u(NdM, yell::ZO_dist{});
   u.forward();
-  cbx.set(yell::gaussian(3.2));
+  cbx.set(gauss_struct(&g_prng));
   cbx += msg;
   cbx.forward_lazy();
   cbx.add_product_of(u, pbx);
-  cax.set(yell::gaussian(3.2));
+  cax.set(gauss_struct(&g_prng));
   cax.forward_lazy();
   cax.add_product_of(u, pax);
   yell::poly<Deg> plain(NdM);  // decrypt
@@ -165,7 +170,7 @@ static void test_device_samplers() {
   const uint64_t step = yell::next_seed() - first;  // the seed increment
   yell::set_seed(s0);
   yell::poly<N> u(L, yell::uniform{}), z(L, yell::ZO_dist{}), w(L, yell::hwt_dist(64));
-  yell::poly<N> g(L, yell::gaussian(3.2));
+  yell::poly<N> g(L, gauss_struct(&g_prng));
 #ifdef YELL_USE_32BITS_MODULI
   auto host = [&](int k, uint64_t seed) {
     return k == 0 ? ya32_sample_uniform(h.data(), N, L, seed) : k == 1 ? ya32_sample_zo(h.data(), N, L, 0x7F, seed)
@@ -205,6 +210,69 @@ static void test_stream_output() {
   CHECK(back[3] == 13, "recast_as_array residue 1 word 3 = %llu", (unsigned long long)back[3]);
 }
 
+// Element access through the proxy of operator()(cm, i), raw_data(), logn,
+// the index checks, getNoise and the amplifier of gaussian<In, Out, D>.
+static void test_accessors() {
+  constexpr size_t N = 1024;
+  static_assert(yell::poly<N>::logn == 10, "logn");
+  yell::poly<N> p(2);
+  p(1, 5) = 123;
+  p(0, 0) = p(1, 5);
+  p(0, 1) += 7;
+  const yell::poly<N> &cp = p;
+  CHECK(cp(1, 5) == 123 && (T)p(0, 0) == 123 && cp(0, 1) == 7, "proxy write / read");
+  CHECK(p.raw_data()[1] == p.ptr_at(1) && cp.raw_data()[0] == cp.cptr_at(0), "raw_data");
+  yell::poly<N> q(std::move(p));
+  CHECK(q.raw_data()[1] == q.ptr_at(1) && q(1, 5) == 123, "raw_data after a move");
+  int thrown = 0;
+  try { (void)q(0, N); } catch (const std::out_of_range &) { ++thrown; }
+  try { (void)cp(0, N); } catch (const std::out_of_range &) { ++thrown; }
+  try { q(2, 0) = 1; } catch (const std::out_of_range &) { ++thrown; }
+  CHECK(thrown == 3, "out-of-range indices: %d of 3 thrown", thrown);
+
+  // getNoise: the signed samples of the stream of the next seed
+  yell::set_seed(99);
+  std::vector<T> noise(100);
+  g_prng.getNoise(noise.data(), noise.size());
+  std::vector<int64_t> want(100);
+  CHECK(ya_gaussian_noise(want.data(), want.size(), 3.2, 99) == YA_OK, "ya_gaussian_noise");
+  bool same = true;
+  for (size_t i = 0; i < noise.size(); ++i) same = same && noise[i] == (T)want[i];
+  CHECK(same, "getNoise differs from ya_gaussian_noise");
+
+  // gaussian(prng, amplifier): r * amplifier mapped to each residue
+  yell::set_seed(5);
+  yell::poly<N> g(2, gauss_struct(&g_prng, 3));
+  std::vector<int64_t> r(N);
+  CHECK(ya_gaussian_noise(r.data(), N, 3.2, 5) == YA_OK, "ya_gaussian_noise");
+  const auto h = g.to_host();
+  int bad = 0;
+  for (size_t cm = 0; cm < 2; ++cm)
+    for (size_t i = 0; i < N; ++i) {
+      const yell::params::signed_type a = (yell::params::signed_type)(r[i] * 3);
+      const T w = a < 0 ? (T)(yell::params::P[cm] + a) : (T)a;
+      bad += h[cm * N + i] != w;
+    }
+  CHECK(bad == 0, "amplified gaussian: %d words differ", bad);
+}
+
+#ifndef YELL_USE_32BITS_MODULI
+// The reference's smallest and largest rings: N = 4 (its own layer order,
+// including its backward there) and N = 2^19, the fused entry point against
+// the call sequence.
+template <size_t N>
+static void check_sequence_vs_fused() {
+  const size_t L = 2;
+  yell::poly<N> a(L, yell::uniform{}), b(L, gauss_struct(&g_prng));
+  yell::poly<N> fa(a), fb(b);
+  fa.forward();
+  fb.forward();
+  yell::poly<N> c = fa * fb;
+  c.backward();
+  CHECK(c == yell::multiply(a, b), "N=%zu: sequence vs fused poly-multiplication differ", N);
+}
+#endif
+
 int main() {
   if (ya_init(0) != YA_OK) {
     std::fprintf(stderr, "no device: %s\n", ya_last_error());
@@ -217,6 +285,11 @@ int main() {
   test_ntt_host_pointer_and_tables();
   test_rlwe_example();
   test_stream_output();
+  test_accessors();
+#ifndef YELL_USE_32BITS_MODULI
+  check_sequence_vs_fused<4>();
+  check_sequence_vs_fused<(1u << 19)>();
+#endif
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "ok", g_fail);
   return g_fail ? 1 : 0;
 }

@@ -72,7 +72,7 @@ def test_invalid_arguments_rejected():
     lib = capi.load()
     buf = np.zeros(16, dtype=np.uint64)
     assert lib.ya_sample_uniform(capi.ptr(buf), 12, 1, 0) == 1        # not a power of two
-    assert lib.ya_sample_uniform(capi.ptr(buf), 4, 1, 0) == 1         # samplers: below 2^3
+    assert lib.ya_sample_uniform(capi.ptr(buf), 2, 1, 0) == 1         # below 2^2
     assert lib.ya_sample_uniform(capi.ptr(buf), 8, 1001, 0) == 1      # too many moduli
     assert lib.ya_sample_uniform(None, 8, 1, 0) == 1
     assert lib.ya_sample_gaussian(capi.ptr(buf), 8, 1, -1.0, 0) == 1

@@ -40,10 +40,23 @@ def test_device_equals_host(word, kind, n, L, param):
         np.testing.assert_array_equal(got, capi.sample_host(kind, n, L, param, seed, word))
 
 
+@pytest.mark.parametrize("kind,param", [("uniform", None), ("gaussian", 3.2), ("zo", 127), ("hwt", 1),
+                                        ("hwt", 4)])
+def test_device_equals_host_degree4(kind, param):
+    """The reference's smallest ring, N = 4 (64-bit build): the device
+    samplers give the host samplers' words, which the oracle pins."""
+    say("sampler N=4", kind, param)
+    got = run_device(kind, 4, 3, param, 9, 64)
+    np.testing.assert_array_equal(got, capi.sample_host(kind, 4, 3, param, 9))
+
+
 @pytest.mark.parametrize("word", [64, 32])
-def test_gaussian_amplifier(word):
-    n, L, amp = 4096, 3, 7
-    say("sampler amplifier", word)
+@pytest.mark.parametrize("amp", [7, 2 ** 28 + 3, 2 ** 30 - 1, 2 ** 31, 2 ** 62 + 5])
+def test_gaussian_amplifier(word, amp):
+    """Amplified samples wrap in params::signed_type (int32_t in the 32-bit
+    build), as the oracle states; large amplifiers reach the wrap."""
+    n, L = 4096, 3
+    say("sampler amplifier", word, amp)
     ps = [int(p) for p in capi.primes(L)] if word == 64 else [O32.prime(j) for j in range(L)]
     got = run_device("gaussian", n, L, 3.2, 21, word, amplifier=amp)
     np.testing.assert_array_equal(got, S.gaussian(ps, n, capi.gaussian_cdt(3.2), 21, amp, word))

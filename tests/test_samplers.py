@@ -47,6 +47,63 @@ def test_gaussian_matches_oracle(word, n, L, sigma, seed):
     np.testing.assert_array_equal(got, S.gaussian(primes(L, word), n, cdf, seed, 1, word))
 
 
+def test_degree4_samplers_match_oracle():
+    """The 64-bit build's samplers at the reference's smallest ring, N = 4."""
+    P = primes(3, 64)
+    np.testing.assert_array_equal(capi.sample_host("uniform", 4, 3, None, 3), S.uniform(P, 4, 3, 64))
+    np.testing.assert_array_equal(capi.sample_host("gaussian", 4, 3, 3.2, 3),
+                                  S.gaussian(P, 4, capi.gaussian_cdt(3.2), 3, 1, 64))
+    np.testing.assert_array_equal(capi.sample_host("zo", 4, 3, 127, 3), S.zo(P, 4, 127, 3, 64))
+    for h in (1, 3, 4):
+        np.testing.assert_array_equal(capi.sample_host("hwt", 4, 3, h, 3), S.hwt(P, 4, h, 3, 64))
+
+
+def test_gaussian_noise_is_the_signed_sample():
+    """ya_gaussian_noise (FastGaussianNoise::getNoise of the drop-in) returns
+    the signed samples that the Gaussian sampler maps to residues, for any
+    count, as a prefix of one stream."""
+    cdf = capi.gaussian_cdt(3.2)
+    for count in (1, 7, 4096):
+        out = np.empty(count, dtype=np.int64)
+        capi.call("ya_gaussian_noise", capi.ptr(out), count, 3.2, 11)
+        np.testing.assert_array_equal(out, S.gaussian_signed(count, cdf, 11))
+    p = [int(v) for v in capi.primes(1)]
+    res = capi.sample_host("gaussian", 1024, 1, 3.2, 11)
+    out = np.empty(1024, dtype=np.int64)
+    capi.call("ya_gaussian_noise", capi.ptr(out), 1024, 3.2, 11)
+    np.testing.assert_array_equal(np.where(out < 0, out + p[0], out).astype(np.uint64), res)
+
+
+@pytest.mark.parametrize("word", [64, 32])
+def test_amplifier_wraps_in_signed_type(word):
+    """The amplified sample wraps in the reference's params::signed_type
+    (int64_t; int32_t in the 32-bit build, params_32.hpp) before the sign test
+    r < 0 ? p + r : r (details/poly.hpp:248-257).  Closed forms from C's
+    conversions, 32-bit: 4 * (2^30 - 1) = 2^32 - 4 is -4 as int32_t, so p - 4;
+    -4 * (2^30 - 1) is 4; 1 * 2^31 is INT32_MIN, so p + 2^31 (mod 2^32).  In
+    64 bits the same products are positive / negative as integers."""
+    cdf = capi.gaussian_cdt(3.2)
+    r = S.gaussian_signed(4096, cdf, 5)
+    p = primes(1, word)[0]
+    for amp in (2 ** 30 - 1, 2 ** 31, 3 ** 20):
+        out = S.gaussian([p], 4096, cdf, 5, amp, word).astype(np.uint64)
+        for s_val in (4, -4, 1, -1, 0):
+            idx = np.flatnonzero(r == s_val)
+            if idx.size == 0:
+                continue
+            prod = s_val * amp
+            if word == 32:
+                a = ((prod + 2 ** 31) % 2 ** 32) - 2 ** 31      # the int32_t value
+                want = (p + a) % 2 ** 32 if a < 0 else a
+            else:
+                a = ((prod + 2 ** 63) % 2 ** 64) - 2 ** 63
+                want = (p + a) % 2 ** 64 if a < 0 else a
+            assert set(int(v) for v in out[idx]) == {want}, (word, amp, s_val)
+    if word == 32:
+        idx = np.flatnonzero(r == 4)[0]
+        assert int(S.gaussian([p], 4096, cdf, 5, 2 ** 30 - 1, 32)[idx]) == p - 4
+
+
 @pytest.mark.parametrize("word", [64, 32])
 @pytest.mark.parametrize("n,L,rho,seed", [(8, 1, 127, 1), (1024, 3, 0, 2), (4096, 2, 255, 3), (1 << 14, 4, 100, 4)])
 def test_zo_matches_oracle(word, n, L, rho, seed):

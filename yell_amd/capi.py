@@ -82,6 +82,7 @@ _SIGS = {
     "ya_graph_end": (i32, [vp, ctypes.POINTER(vp)]),
     "ya_graph_launch": (i32, [vp, vp]),
     "ya_graph_destroy": (i32, [vp]),
+    "ya_gaussian_noise": (i32, [vp, sz, ctypes.c_double, u64]),
     "ya_gaussian_cdt": (i32, [ctypes.c_double, vp, sz, ctypes.POINTER(ctypes.c_size_t)]),
     "ya_sample_uniform_device": (i32, [vp, sz, sz, u64, vp]),
     "ya_sample_gaussian_device": (i32, [vp, sz, sz, ctypes.c_double, u64, u64, vp]),

@@ -92,20 +92,28 @@ int sample_uniform(W *out, size_t degree, size_t nmoduli, uint64_t seed, PF prim
 // takes stream words 2i (magnitude, from the table) and 2i+1 (sign, top
 // bit); the signed sample r is shared by all residues, r < 0 ? p + r : r.
 // |r| <= ceil(13 sigma) is below every prime of both builds.
-template <class W, class PF>
-int sample_gaussian(W *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed, PF prime) {
-  if (!out) return ya::fail(YA_ERR_INVALID, "null output");
+// Signed samples 0..count-1 of the stream of `seed`: word 2i against the
+// cumulative table (the magnitude), the top bit of word 2i+1 (the sign).
+int gaussian_signed(int64_t *r, size_t count, double sigma, uint64_t seed) {
   std::vector<uint64_t> cdf;
   int rc = ya::gaussian_cdt(sigma, cdf);
   if (rc) return rc;
   const size_t tail = cdf.size() - 1;
-  std::vector<int64_t> r(degree);
-  for (size_t i = 0; i < degree; ++i) {
+  for (size_t i = 0; i < count; ++i) {
     const uint64_t u = ya::stream_word(seed, 2 * i), s = ya::stream_word(seed, 2 * i + 1);
     size_t k = 0;
     while (k < tail && u >= cdf[k]) ++k;
     r[i] = (k != 0 && (s >> 63)) ? -(int64_t)k : (int64_t)k;
   }
+  return YA_OK;
+}
+
+template <class W, class PF>
+int sample_gaussian(W *out, size_t degree, size_t nmoduli, double sigma, uint64_t seed, PF prime) {
+  if (!out) return ya::fail(YA_ERR_INVALID, "null output");
+  std::vector<int64_t> r(degree);
+  int rc = gaussian_signed(r.data(), degree, sigma, seed);
+  if (rc) return rc;
   for (size_t cm = 0; cm < nmoduli; ++cm) {
     const int64_t p = (int64_t)prime(cm);
     W *dst = out + cm * degree;
@@ -156,7 +164,10 @@ int sample_hwt(W *out, size_t degree, size_t nmoduli, uint32_t hwt, uint64_t see
 uint64_t prime64(size_t cm) { return ya::prime(cm); }
 uint64_t prime32(size_t cm) { return ya32_prime(cm); }
 
-int shape64(size_t degree, size_t nmoduli) { return sample_shape(degree, nmoduli, ya::kMaxLogN, ya_num_moduli()); }
+int shape64(size_t degree, size_t nmoduli) {
+  int logn;
+  return ya::check_shape(degree, 0, nmoduli, &logn);
+}
 int shape32(size_t degree, size_t nmoduli) { return sample_shape(degree, nmoduli, 15, ya32_num_moduli()); }
 
 }  // namespace
@@ -173,6 +184,11 @@ int ya_gaussian_cdt(double sigma, uint64_t *out, size_t capacity, size_t *count)
     std::copy(cdf.begin(), cdf.end(), out);
   }
   return YA_OK;
+}
+
+int ya_gaussian_noise(int64_t *out, size_t count, double sigma, uint64_t seed) {
+  if (!out && count) return ya::fail(YA_ERR_INVALID, "null output");
+  return gaussian_signed(out, count, sigma, seed);
 }
 
 int ya_sample_uniform(uint64_t *out, size_t degree, size_t nmoduli, uint64_t seed) {

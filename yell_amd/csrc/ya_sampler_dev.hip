@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -55,7 +56,7 @@ __device__ __forceinline__ void store2(W *dst, uint64_t a, uint64_t b) {
 // Uniform: one stream block (32 bytes = 32 / sizeof(W) words) per thread
 // iteration -- the Philox block (~380 instructions) is the cost at large
 // sizes, so each is computed once; a block never straddles two residues
-// (degree >= 8).  Polynomial blockIdx.y of a launch over several:
+// (degree >= 4 words of 64 bits, >= 8 of 32).  Polynomial blockIdx.y of a launch over several:
 // seed + blockIdx.y, at out + blockIdx.y * nblocks * E.
 template <class W>
 __global__ void __launch_bounds__(kThreads) k_sample_uniform(W *__restrict__ out, uint32_t logn, uint64_t nblocks,
@@ -93,10 +94,15 @@ __device__ __forceinline__ uint32_t cdt_search(const uint64_t *__restrict__ cdf,
   return lo;
 }
 
-// Signed sample times the amplifier (wrapping 64-bit product), mapped
-// a < 0 ? p + a : a.
-__device__ __forceinline__ uint64_t gaussian_residue(int64_t a, uint64_t p) {
-  return a < 0 ? p + (uint64_t)a : (uint64_t)a;
+// a = signed sample times the amplifier in the reference's signed_type
+// (details/poly.hpp:248-251: int64_t, or int32_t with 32-bit moduli,
+// params_32.hpp), so the product wraps modulo 2^(8 sizeof W); then
+// a < 0 ? p + a : a in W (details/poly.hpp:252-257).
+template <class W>
+__device__ __forceinline__ W gaussian_residue(int64_t s, uint64_t amplifier, uint64_t p) {
+  typedef typename std::conditional<sizeof(W) == 8, int64_t, int32_t>::type S;
+  const S a = (S)(W)((uint64_t)s * amplifier);
+  return a < 0 ? (W)((W)p + (W)a) : (W)a;
 }
 
 // Gaussian: one stream block per two coefficients (words 2i, 2i+1 of
@@ -113,10 +119,10 @@ __global__ void __launch_bounds__(kThreads) k_sample_gaussian(W *__restrict__ ou
     const uint32_t k0 = cdt_search(cdf, tail, r.v[0]), k1 = cdt_search(cdf, tail, r.v[2]);
     const int64_t s0 = (k0 != 0 && (r.v[1] >> 63)) ? -(int64_t)k0 : (int64_t)k0;
     const int64_t s1 = (k1 != 0 && (r.v[3] >> 63)) ? -(int64_t)k1 : (int64_t)k1;
-    const int64_t a0 = (int64_t)((uint64_t)s0 * amplifier), a1 = (int64_t)((uint64_t)s1 * amplifier);
     for (uint32_t cm = 0; cm < nmod; ++cm) {
       const uint64_t p = sampler_prime<W>(cm);
-      store2(out + ((uint64_t)cm << logn) + 2 * b, gaussian_residue(a0, p), gaussian_residue(a1, p));
+      store2(out + ((uint64_t)cm << logn) + 2 * b, gaussian_residue<W>(s0, amplifier, p),
+             gaussian_residue<W>(s1, amplifier, p));
     }
   }
 }
@@ -264,9 +270,7 @@ int dev_shape(const W *out, size_t degree, size_t nmoduli, int *logn) {
   if (!out) return fail(YA_ERR_INVALID, "null output");
   if (((uintptr_t)out) % 16) return fail(YA_ERR_INVALID, "output must be 16-byte aligned");
   if constexpr (sizeof(W) == 8) {
-    const int rc = check_shape(degree, 0, nmoduli, logn);
-    if (rc == YA_OK && *logn < 3) return fail(YA_ERR_INVALID, "degree %zu: the samplers need 8 words or more", degree);
-    return rc;
+    return check_shape(degree, 0, nmoduli, logn);
   } else {
     const int l = log2_degree(degree);
     if (l < 3 || l > 15) return fail(YA_ERR_INVALID, "degree %zu: a power of two in [8, 2^15] is required", degree);
