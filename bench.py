@@ -35,6 +35,15 @@ HBM_PEAK_GBS = 8000.0
 # polynomials) 1.1443e9 B read + 5.369e8 B written; k_polymul_tab (the
 # residue-table kernel) 1.1535e9 + 5.369e8.
 TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3283596.0, "k_polymul_tab<14,4,dual>": 3301576.0}
+# The binding resource of the fused kernel is VALU issue: one wave64 VALU
+# instruction per CU per cycle (4 SIMDs of 16 lanes, 4 cycles per wave64
+# instruction), 256 CUs at the 2.4 GHz peak engine clock.  VALU wave-
+# instructions per poly-multiplication from SQ_INSTS_VALU at --batch 512
+# (profiles/k_polymul_dual_n16384_l8.md): 8,668 per wave x 16 waves per
+# residue x 8 residues.
+CUS, ENGINE_CLOCK_GHZ = 256, 2.4
+VALU_PEAK_G_PER_S = CUS * ENGINE_CLOCK_GHZ
+VALU_INSTS_PER_POLYMUL = {"k_polymul<14,4,dual>": 8668 * 16 * 8}
 
 
 def log(*a):
@@ -71,17 +80,19 @@ def make_inputs(n, L, batch, seed):
     return a, b
 
 
-def cpu_baseline(a0, b0, n, L, gpu_c0, seconds):
+def cpu_baseline(a, b, n, L, gpu_c, sample_polys, seconds):
     """Time the reference's poly-mult on the host cores (bounded sample) and
-    check the GPU result of polynomial 0 against it."""
+    check the GPU results of polynomials `sample_polys` against it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle  # checker only: this leg measures and verifies, it is never the product
 
+    w = n * L
+    a0, b0 = a[:w].copy(), b[:w].copy()
     threads = max(1, min(16, os.cpu_count() or 1))
     try:
         ref = _oracle.RefLib()
         kind = "reference"
-        want = ref.polymul(a0, b0, n, L)
+        polymul = ref.polymul
         lg = n.bit_length() - 1
         t1 = ref.lib.yref_bench_polymul(lg, capi.ptr(a0), capi.ptr(b0), L, 1, 1)
         reps = max(1, int(seconds / max(t1, 1e-6) / 1.0))
@@ -98,8 +109,9 @@ def cpu_baseline(a0, b0, n, L, gpu_c0, seconds):
         orc = _oracle.Oracle()
         kind = "port"
         threads = 1
+        polymul = orc.polymul
         t0 = time.perf_counter()
-        want = orc.polymul(a0, b0, n, L)
+        orc.polymul(a0, b0, n, L)
         t1 = time.perf_counter() - t0
         reps = max(1, int(seconds / max(t1, 1e-6)))
         t0 = time.perf_counter()
@@ -116,9 +128,30 @@ def cpu_baseline(a0, b0, n, L, gpu_c0, seconds):
                 break
     except OSError:
         pass
+    exact = all(np.array_equal(polymul(a[k * w:(k + 1) * w].copy(), b[k * w:(k + 1) * w].copy(), n, L),
+                               gpu_c[k * w:(k + 1) * w]) for k in sample_polys)
     return {"value": value, "unit": "poly-mults/s", "cores": threads, "kind": kind,
             "sample": sample, "single_core_value": single, "cpu_model": model,
-            "host_cpus": os.cpu_count()}, bool(np.array_equal(want, gpu_c0))
+            "host_cpus": os.cpu_count()}, exact
+
+
+def roofline(kernel, ms, polys, hbm_achieved, bytes_per):
+    """The step kernel against the resource that binds it (VALU issue), with
+    the HBM side as a secondary block."""
+    traffic = TRAFFIC_BYTES_PER_POLYMUL[kernel] * polys if kernel in TRAFFIC_BYTES_PER_POLYMUL else None
+    hbm = {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_achieved / HBM_PEAK_GBS,
+           "bytes_per_polymul": bytes_per}
+    common = {"traffic": traffic, "traffic_source": "profiles/k_polymul_dual_n16384_l8.md", "kernel": kernel,
+              "polymuls_per_launch": polys}
+    if kernel not in VALU_INSTS_PER_POLYMUL:
+        return dict({"bound": "hbm"}, **hbm, **common)
+    insts = VALU_INSTS_PER_POLYMUL[kernel] * polys
+    valu = insts / (ms / 1e3) / 1e9
+    return dict({"bound": "valu", "achieved": valu, "peak": VALU_PEAK_G_PER_S,
+                 "unit": "G VALU wave-instructions/s", "frac": valu / VALU_PEAK_G_PER_S,
+                 "valu_insts_per_launch": insts,
+                 "peak_basis": "%d CUs x %.1f GHz x 1 wave64 VALU instruction per CU-cycle" % (CUS, ENGINE_CLOCK_GHZ),
+                 "hbm": hbm}, **common)
 
 
 def make_plan(jobs_spec):
@@ -504,6 +537,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch, probes and tuner")
+    ap.add_argument("--only", default="",
+                    help="comma list of the extra sections to run (tune, probe, c3, c4, crt, latency, u32, "
+                         "samplers, rlwe, served); default all")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[4000.0, 8000.0, 12000.0],
                     help="offered loads (requests/s) of the served-form runs")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -530,7 +566,12 @@ def main():
         capi.call("ya_stream_synchronize", stream)
         s_.close()
         d_.close()
-    if not args.no_extra:
+    only = set(x for x in args.only.split(",") if x)
+
+    def want(section):
+        return not args.no_extra and (not only or section in only)
+
+    if want("tune"):
         log("autotune 2^10..2^15")
         rep = (ctypes.c_double * 36)()
         capi.call("ya_autotune", 10, 15, rep)
@@ -539,6 +580,7 @@ def main():
                                  [round(rep[6 * k + j], 4) for j in (0, 1, 3, 4, 5)] for k in range(6)],
                              "chosen_batched_le": [capi.load().ya_tune_get(2, lg) for lg in range(10, 16)],
                              "chosen_fused_le": [capi.load().ya_tune_get(1, lg) for lg in range(10, 16)]}
+    if want("probe"):
         probes = {}
         for kind, name in ((0, "mad_u64_u32_per_s"), (1, "mul_lo_u32_per_s"), (2, "ct_butterflies_per_s"),
                            (3, "ct32_butterflies_per_s")):
@@ -589,16 +631,20 @@ def main():
     ms_step = timed(ev, stream, batched, args.steps)
     wall = time.perf_counter() - t0
     value = B / (ms_step / 1e3)
-    c0 = dc.download(w)
+    # every output word of the three forms is compared (the buffer is cleared
+    # between them, so each form writes all of it)
+    c_batched = dc.download()
+    capi.call("ya_memset_zero", dc.ptr, B * w * 8, stream)
     grid_form()
     ms_grid = timed(ev, stream, grid_form, max(1, args.steps // 2))
-    c0_grid = dc.download(w)
+    same_grid = bool(np.array_equal(c_batched, dc.download()))
+    capi.call("ya_memset_zero", dc.ptr, B * w * 8, stream)
     plain()
     capi.call("ya_stream_synchronize", stream)
     log("timing plain:", args.plain_steps, "steps of", B, "launches")
     ms_plain = timed(ev, stream, plain, args.plain_steps)
     plain_value = B / (ms_plain / 1e3)
-    c0_plain = dc.download(w)
+    same_plain = bool(np.array_equal(c_batched, dc.download()))
 
     bytes_per = 3 * n * L * 8
     achieved = bytes_per * B / (ms_step / 1e3) / 1e9
@@ -618,14 +664,7 @@ def main():
         "batched_over_plain": value / plain_value,
         "grid_form": {"value": B / (ms_grid / 1e3), "ms_per_step": ms_grid,
                       "kernel": "k_polymul<14,4>, one workgroup per residue"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC_BYTES_PER_POLYMUL[step_kernel] * B
-                     if step_kernel in TRAFFIC_BYTES_PER_POLYMUL else None,
-                     "traffic_source": "profiles/k_polymul_dual_n16384_l8.md",
-                     "kernel": step_kernel,
-                     "bytes_per_polymul": bytes_per,
-                     "polymuls_per_launch": B},
+        "roofline": roofline(step_kernel, ms_step, B, achieved, bytes_per),
         "modmul_rate": {"modmuls_per_polymul": mm,
                         "achieved_T_per_s": mm * B / (ms_step / 1e3) / 1e12},
         "timed_wall_s": wall,
@@ -635,25 +674,32 @@ def main():
         pk = extra["valu_probe"]["ct_butterflies_per_s"]
         out["modmul_rate"]["butterfly_probe_T_per_s"] = pk / 1e12
         out["modmul_rate"]["frac_of_probe"] = mm * B / (ms_step / 1e3) / pk
-    if not args.no_extra:
+    if want("c3"):
         log("C3 uneven batch")
         out["c3_uneven_batch"] = c3_bench(stream, ev, max(2, args.steps // 4))
+    if want("c4"):
         log("C4 large-ring MAC")
         out["c4_large_ring_mac"] = c4_bench(stream, ev, 3)
+    if want("crt"):
         log("CRT lift")
         out["crt_lift"] = crt_bench(stream, ev)
+    if want("latency"):
         log("single poly-mult latency")
         out["single_polymul_latency"] = latency_bench(stream, ev, n, L)
+    if want("u32"):
         log("32-bit moduli build")
         out["u32_moduli"] = u32_bench(stream, ev)
         if "valu_probe" in extra:  # the 32-bit build against its own butterfly ceiling
             pk32 = extra["valu_probe"]["ct32_butterflies_per_s"]
             out["u32_moduli"]["butterfly_probe_T_per_s"] = pk32 / 1e12
             out["u32_moduli"]["frac_of_probe"] = out["u32_moduli"]["modmuls_T_per_s"] * 1e12 / pk32
+    if want("samplers"):
         log("device samplers")
         out["device_samplers"] = sampler_bench(stream, ev)
+    if want("rlwe"):
         log("RLWE flow")
         out["rlwe_flow"] = rlwe_bench(stream, ev)
+    if want("served"):
         sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
         out["served"] = sv
         for r in sv["runs"]:
@@ -665,12 +711,16 @@ def main():
         uneven = [(1 << lg, m) for lg in range(10, 16) for m in range(1, 13)]
         out["served_uneven"] = served_bench(sorted({args.serve_rates[0], args.serve_rates[-1]}), args.serve_seconds,
                                             uneven, pool=2, max_pending=256, batch_jobs=64)
-    same = bool(np.array_equal(c0, c0_plain) and np.array_equal(c0, c0_grid))
+    same = same_grid and same_plain
     out["batched_equals_plain"] = same
+    out["compared"] = {"batched_vs_grid_and_plain_words": B * w, "batched_equals_grid": same_grid,
+                       "batched_equals_plain": same_plain}
     if not args.no_cpu:
         log("cpu baseline (~%.0f s)" % args.cpu_seconds)
-        cb, exact = cpu_baseline(a[:w].copy(), b[:w].copy(), n, L, c0, args.cpu_seconds)
+        sample = sorted({0, B - 1} | {int(k) for k in np.linspace(0, B - 1, 8)})
+        cb, exact = cpu_baseline(a, b, n, L, c_batched, sample, args.cpu_seconds)
         out["cpu_baseline"] = cb
+        out["compared"]["polys_vs_cpu_reference"] = sample
         out["bit_exact_vs_cpu"] = exact and same
         out["gpu_over_cpu"] = value / cb["value"]
         if "crt_lift" in out:
