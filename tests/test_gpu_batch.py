@@ -1,6 +1,7 @@
 """Batched form: uneven jobs in one persistent launch (and as a hipGraph),
 bit-exact against the oracle and against the plain per-poly path."""
 import ctypes
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -79,31 +80,63 @@ def test_graph_replays_with_new_data():
     bt.close()
 
 
-def test_full_c3_batch_matches_plain_path():
-    """1024 jobs, degrees 2^10..2^15, 1..12 moduli: every job's result equals
-    the plain ya_poly_multiply result (itself pinned to the oracle)."""
+def test_full_c3_batch_vs_oracle_through_graph():
+    """The uneven batch's shape: 1024 jobs, degrees 2^10..2^15, 1..12 moduli,
+    uniform operands, packed back to back in three buffers, run as the plan's
+    hipGraph (captured, then replayed with new operands): every job of both
+    runs against the C oracle, and the first run against the plain
+    ya_poly_multiply path."""
     device()
     rng = np.random.default_rng(2026)
     njobs = 1024
     specs = [(1 << int(rng.integers(10, 16)), int(rng.integers(1, 13))) for _ in range(njobs)]
-    say("C3 full batch", sum(n * L for n, L in specs), "words per operand")
+    offs = np.cumsum([0] + [n * L for n, L in specs])
+    total = int(offs[-1])
+    say("C3 full batch through the graph", total, "words per operand")
+    A, B, C, D = (capi.DeviceBuffer(total) for _ in range(4))
     arr = (capi.PolymulJob * njobs)()
-    bufs = []
     for k, (n, L) in enumerate(specs):
-        a = capi.sample_uniform(n, L, 50000 + k)
-        b = capi.sample_uniform(n, L, 90000 + k)
-        da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
-        dc, dp = capi.DeviceBuffer(n * L), capi.DeviceBuffer(n * L)
-        bufs.append((da, db, dc, dp))
-        arr[k] = capi.PolymulJob(da.ptr, db.ptr, dc.ptr, n, L)
-    capi.call("ya_batch_polymul", ctypes.cast(arr, ctypes.c_void_p), njobs, None)
-    for (n, L), (da, db, dc, dp) in zip(specs, bufs):
-        capi.call("ya_poly_multiply", dp.ptr, da.ptr, db.ptr, n, L, None)
-    capi.call("ya_device_synchronize")
-    for k, ((n, L), (da, db, dc, dp)) in enumerate(zip(specs, bufs)):
-        np.testing.assert_array_equal(dc.download(), dp.download(), err_msg="job %d" % k)
-    for t in bufs:
-        for d in t:
+        o = 8 * int(offs[k])
+        arr[k] = capi.PolymulJob(A.ptr + o, B.ptr + o, C.ptr + o, n, L)
+    plan = ctypes.c_void_p()
+    capi.call("ya_batch_plan_create", ctypes.byref(plan), ctypes.cast(arr, ctypes.c_void_p), njobs)
+    stream = ctypes.c_void_p()
+    capi.call("ya_stream_create", ctypes.byref(stream))
+
+    def operands(seed):
+        a = np.concatenate([capi.sample_uniform(n, L, seed + 2 * k) for k, (n, L) in enumerate(specs)])
+        b = np.concatenate([capi.sample_uniform(n, L, seed + 2 * k + 1) for k, (n, L) in enumerate(specs)])
+        return a, b
+
+    def check(a, b, got, tag):
+        def one(k):
+            n, L = specs[k]
+            s = slice(int(offs[k]), int(offs[k + 1]))
+            return k if not np.array_equal(got[s], orc.polymul(a[s], b[s], n, L)) else -1
+        with ThreadPoolExecutor(max_workers=16) as ex:
+            bad = [k for k in ex.map(one, range(njobs)) if k >= 0]
+        assert not bad, "%s: jobs %s differ from the oracle" % (tag, bad[:10])
+
+    try:
+        for run, seed in enumerate((50000, 70000)):
+            a, b = operands(seed)
+            A.upload(a)
+            B.upload(b)
+            say("C3 graph run", run)
+            capi.call("ya_batch_plan_run_graph", plan, stream)
+            capi.call("ya_stream_synchronize", stream)
+            got = C.download()
+            check(a, b, got, "graph run %d" % run)
+            if run == 0:
+                for k, (n, L) in enumerate(specs):
+                    o = 8 * int(offs[k])
+                    capi.call("ya_poly_multiply", D.ptr + o, A.ptr + o, B.ptr + o, n, L, stream)
+                capi.call("ya_stream_synchronize", stream)
+                np.testing.assert_array_equal(D.download(), got)
+    finally:
+        capi.load().ya_batch_plan_destroy(plan)
+        capi.call("ya_stream_destroy", stream)
+        for d in (A, B, C, D):
             d.close()
 
 
