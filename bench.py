@@ -210,7 +210,11 @@ def c3_bench(stream, ev, steps, seed=2026):
 
 def c4_bench(stream, ev, steps):
     """N = 2^16, L = 24: forward NTT of 16 operands, sum of 8 pair products in
-    the NTT domain, one inverse NTT (17 * N * L * 8 algorithmic bytes)."""
+    the NTT domain, one inverse NTT (17 * N * L * 8 algorithmic bytes).  The
+    job is ya_poly_add_products_backward (the pair-product pass and the summing
+    inverse, both block sizes of the pair pass timed, the faster kept), and,
+    for comparison, the reference's sequence as separate launches (forward of
+    the 16 operands, the MAC, the inverse).  Each step starts from acc = 0."""
     n, L, pairs = 1 << 16, 24, 8
     w = n * L
     ops = np.concatenate([capi.sample_uniform(n, L, 4000 + k) for k in range(2 * pairs)])
@@ -220,7 +224,11 @@ def c4_bench(stream, ev, steps):
     pb = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j + 1) for j in range(pairs)])
     e2 = Events()
 
-    def step():
+    def fused():
+        capi.call("ya_memset_zero", dacc.ptr, w * 8, stream)
+        capi.call("ya_poly_add_products_backward", dacc.ptr, pa, pb, pairs, n, L, stream)
+
+    def separate():
         capi.call("ya_poly_forward_many", dall.ptr, n, L, 2 * pairs, stream)
         capi.call("ya_event_record", e2.a, stream)
         capi.call("ya_memset_zero", dacc.ptr, w * 8, stream)
@@ -228,17 +236,44 @@ def c4_bench(stream, ev, steps):
         capi.call("ya_event_record", e2.b, stream)
         capi.call("ya_poly_backward", dacc.ptr, n, L, stream)
 
-    step()
+    lg = n.bit_length() - 1
+    by_lb = {}
+    for lb in (14, 13):
+        capi.call("ya_tune_set", 4, lg, lb)
+        fused()
+        capi.call("ya_stream_synchronize", stream)
+        by_lb[lb] = timed(ev, stream, fused, steps)
+    best = min(by_lb, key=by_lb.get)
+    capi.call("ya_tune_set", 4, lg, best)
+    ms = by_lb[best]
+    separate()
     capi.call("ya_stream_synchronize", stream)
-    ms = timed(ev, stream, step, steps)
+    ms_sep = timed(ev, stream, separate, steps)
     ms_mac = e2.elapsed_ms()
     dall.close()
     dacc.close()
     alg = 17 * w * 8
-    return {"degree": n, "nmoduli": L, "pairs": pairs, "ms_per_job": ms,
+    big = {}
+    for lgn, Lb in ((16, 24), (19, 4)):  # one large-ring poly-multiplication (ya_poly_multiply)
+        nb = 1 << lgn
+        a1 = capi.DeviceBuffer.from_host(capi.sample_uniform(nb, Lb, 77))
+        b1 = capi.DeviceBuffer.from_host(capi.sample_gaussian(nb, Lb, 3.2, 78))
+        c1 = capi.DeviceBuffer(nb * Lb)
+        one = lambda: capi.call("ya_poly_multiply", c1.ptr, a1.ptr, b1.ptr, nb, Lb, stream)
+        one()
+        capi.call("ya_stream_synchronize", stream)
+        ms1 = timed(ev, stream, one, steps)
+        big["n%d_l%d" % (nb, Lb)] = {"ms": ms1, "algorithmic_bytes": 3 * nb * Lb * 8,
+                                     "hbm_GBps_algorithmic": 3 * nb * Lb * 8 / (ms1 / 1e3) / 1e9}
+        for d_ in (a1, b1, c1):
+            d_.close()
+    return {"degree": n, "nmoduli": L, "pairs": pairs, "ms_per_job": ms, "large_ring_polymul": big,
             "jobs_per_s": 1e3 / ms, "algorithmic_bytes": alg,
             "hbm_GBps_algorithmic": alg / (ms / 1e3) / 1e9,
-            "mac_ms": ms_mac, "mac_GBps": (2 * pairs + 2) * w * 8 / (ms_mac / 1e3) / 1e9}
+            "entry_point": "ya_poly_add_products_backward",
+            "ms_by_pair_block_log": {str(k): v for k, v in by_lb.items()}, "pair_block_log": best,
+            "separate_launches": {"ms_per_job": ms_sep, "mac_ms": ms_mac,
+                                  "mac_GBps": (2 * pairs + 2) * w * 8 / (ms_mac / 1e3) / 1e9}}
 
 
 def u32_bench(stream, ev, reps=5):

@@ -185,6 +185,18 @@ int ya_poly_negate_many(uint64_t *x, size_t degree, size_t nmoduli, size_t count
  * arrays of npairs DEVICE pointers to polys of the acc's shape. */
 int ya_poly_add_products(uint64_t *acc, const uint64_t *const *a, const uint64_t *const *b,
                          size_t npairs, size_t degree, size_t nmoduli, void *stream);
+/* The large-ring multiply-accumulate and its inverse in one call: for every
+ * pair a_j.forward(); b_j.forward(); acc.add_product_of(a_j, b_j); then
+ * acc.backward() (details/poly.hpp:119-132, 171-186), i.e. acc = backward(acc +
+ * sum_j forward(a_j) * forward(b_j)) with acc given in the NTT domain
+ * (canonical; zero for a plain sum of products) -- the same words as that
+ * sequence.  a and b are HOST arrays of npairs DEVICE pointers to polys of
+ * acc's shape; a_j and b_j are work space, their words on return unspecified
+ * (from degree 2^13 both transforms of a pair and their product run in one
+ * pass that writes only the product, into a_j, and the inverse's first pass
+ * sums the products into acc).  The 2 * npairs + 1 buffers must not overlap. */
+int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *const *b, size_t npairs,
+                                  size_t degree, size_t nmoduli, void *stream);
 /* The reference example's encryption step (example.cpp:41-44 and 46-48:
  * ct += msg; ct.forward_lazy(); ct.add_product_of(u, pk)) over `count`
  * polynomials in one pass: acc_k = forward_lazy(acc_k + add_k) + u_k * key,
@@ -292,7 +304,9 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
  * batched kernel, 0 = standalone NTT kernels; le = log2 of the register-pass
  * radix (3, 4, or 5 for kinds 0 and 1).  kind 3: the fused kernel's
  * dual-forward form (both operands' forward transforms in one pass sequence
- * with shared twiddles; radix 8 / 16 at 2^10 and up), value 0 or 1.
+ * with shared twiddles; radix 8 / 16 at 2^10 and up), value 0 or 1.  kind 4:
+ * the block size (log2: 14, or 13 up to degree 2^18) of the pair-product pass
+ * of ya_poly_add_products_backward.
  * ya_tune_get returns le (or the kind 3 flag), or -1. */
 int ya_tune_get(int kind, int logn);
 int ya_tune_set(int kind, int logn, int le);

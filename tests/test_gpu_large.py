@@ -152,3 +152,63 @@ def test_c4_large_ring_mac():
         fb = orc.poly_forward(ops[2 * j + 1], n, L)
         acc = orc.poly_add_product_of(acc, fa, fb, n, L)
     np.testing.assert_array_equal(got, orc.poly_backward(acc, n, L))
+
+
+def _mac_want(acc, a, b, n, L):
+    want = acc
+    for x, y in zip(a, b):
+        want = orc.poly_add_product_of(want, orc.poly_forward(x, n, L), orc.poly_forward(y, n, L), n, L)
+    return orc.poly_backward(want, n, L)
+
+
+@pytest.mark.parametrize("logn,L,npairs,lb", [(12, 2, 3, 14), (13, 3, 3, 14), (14, 2, 17, 14), (16, 5, 4, 14),
+                                              (16, 5, 4, 13), (18, 2, 2, 13), (19, 1, 2, 14)])
+def test_add_products_backward(logn, L, npairs, lb):
+    """ya_poly_add_products_backward = the reference's sequence a_j.forward();
+    b_j.forward(); acc.add_product_of(a_j, b_j) for every pair, then
+    acc.backward(), from a non-zero NTT-domain acc: below 2^13 as separate
+    launches, from 2^13 the pair-product pass (block 2^lb) and the summing
+    inverse, 17 pairs in two chunks."""
+    lib = device()
+    n = 1 << logn
+    rng = np.random.default_rng(logn * 100 + npairs)
+    acc = rand_bound(rng, n, L, 1, P)
+    a = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+    b = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+    a[0][:3] = [P[0] - 1, 0, P[0] - 1]
+    b[0][:3] = [P[0] - 1, P[0] - 1, 0]
+    old = lib.ya_tune_get(4, logn)
+    capi.call("ya_tune_set", 4, logn, lb)
+    try:
+        da = [capi.DeviceBuffer.from_host(v) for v in a]
+        db = [capi.DeviceBuffer.from_host(v) for v in b]
+        dacc = capi.DeviceBuffer.from_host(acc)
+        pa = (ctypes.c_void_p * npairs)(*[x.ptr for x in da])
+        pb = (ctypes.c_void_p * npairs)(*[x.ptr for x in db])
+        say("add_products_backward", logn, L, npairs, lb)
+        capi.call("ya_poly_add_products_backward", dacc.ptr, pa, pb, npairs, n, L, None)
+        np.testing.assert_array_equal(dacc.download(), _mac_want(acc, a, b, n, L))
+    finally:
+        capi.call("ya_tune_set", 4, logn, old if old in (13, 14) else 14)
+
+
+def test_add_products_backward_c4_and_rejects_overlap():
+    """The large-ring job (N = 2^16, L = 24, 8 pairs, acc = 0) in one call
+    against the oracle's sequence; overlapping operands are refused."""
+    lib = device()
+    n, L, pairs = 1 << 16, 24, 8
+    w = n * L
+    ops = [capi.sample_uniform(n, L, 6000 + k) for k in range(2 * pairs)]
+    dall = capi.DeviceBuffer.from_host(np.concatenate(ops))
+    dacc = capi.DeviceBuffer(w)
+    capi.call("ya_memset_zero", dacc.ptr, w * 8, None)
+    pa = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j) for j in range(pairs)])
+    pb = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j + 1) for j in range(pairs)])
+    say("add_products_backward C4")
+    capi.call("ya_poly_add_products_backward", dacc.ptr, pa, pb, pairs, n, L, None)
+    got = dacc.download()
+    np.testing.assert_array_equal(got, _mac_want(np.zeros(w, dtype=np.uint64), ops[0::2], ops[1::2], n, L))
+    pb[3] = pa[2]
+    assert lib.ya_poly_add_products_backward(dacc.ptr, pa, pb, pairs, n, L, None) == 1
+    pb[3] = dacc.ptr
+    assert lib.ya_poly_add_products_backward(dacc.ptr, pa, pb, pairs, n, L, None) == 1

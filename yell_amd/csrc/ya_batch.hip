@@ -624,14 +624,19 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
 }
 
 extern "C" int ya_tune_get(int kind, int logn) {
-  if (kind < 0 || kind > 3 || logn < 0 || logn > kMaxLogN) return -1;
+  if (kind < 0 || kind > 4 || logn < 0 || logn > kMaxLogN) return -1;
   if (kind == 3) return tuned_shape(logn, 1).dual;
+  if (kind == 4) return tuned_mac_block_log(logn);
   return tuned_shape(logn, kind).le;
 }
 
 extern "C" int ya_tune_set(int kind, int logn, int le) {
   if (kind == 3 && logn >= 0 && logn <= kMaxLogN && (le == 0 || le == 1)) {
     set_tuned_dual(logn, le);
+    return YA_OK;
+  }
+  if (kind == 4 && logn >= 0 && logn <= kMaxLogN && (le == 14 || (le == 13 && logn <= 18))) {
+    set_tuned_mac_block_log(logn, le);
     return YA_OK;
   }
   if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN || le < 3 || le > 5 || (kind == 2 && le == 5))

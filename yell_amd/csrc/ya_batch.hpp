@@ -81,6 +81,9 @@ constexpr int kSplitMinK = 2, kSplitMaxK = 5, kSplitMinS = 9;
 // the degree has no split shape.
 int split_choose_k(int logn, size_t nres, int cus);
 hipError_t launch_polymul_split(const SplitArgs &s, int logn, size_t nres, int k, hipStream_t st);
+// A ring above 2^14 multiplies as a split launch too: blocks of 2^13 (two
+// 512-thread workgroups per CU) up to 2^18, 2^14 at 2^19; returns its k.
+int large_ring_split_k(int logn);
 
 hipError_t launch_polymul_table(const BatchRes *res, size_t nres, int logn, NttShape shape,
                                 hipStream_t st);

@@ -180,6 +180,16 @@ int current_device(DeviceState **out) {
     e = hipMemset(st->d_desc, 0, nslots * sizeof(ModDesc));
     if (e != hipSuccess) return hip_fail(e, "hipMemset(desc)");
     st->slots.reset(new Slot[nslots]);
+    // The stream-ordered allocations of the split and large-ring paths come
+    // from the device's default pool; keeping freed blocks in the pool (no
+    // release threshold) makes the next call's hipMallocAsync a reuse, not a
+    // new mapping.
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
     e = hipGetSymbolAddress((void **)&st->scratch, HIP_SYMBOL(scratch));
     if (e != hipSuccess || !st->scratch)
       return fail(YA_ERR_NO_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
@@ -553,6 +563,10 @@ int ya_graph_destroy(void *graph) {
   return YA_OK;
 }
 
+static bool overlaps(const uint64_t *a, size_t na, const uint64_t *b, size_t nb) {
+  return a && b && a < b + nb && b < a + na;
+}
+
 // ---- NTT ------------------------------------------------------------------
 // Paired forward transforms in the many-polynomial forms (on by default;
 // ya_pair_forward(0) runs one residue per workgroup, for comparison).
@@ -651,6 +665,79 @@ int ya_poly_add_products(uint64_t *acc, const uint64_t *const *a, const uint64_t
     }
   }
   return YA_OK;
+}
+
+// acc = backward(acc + sum_j forward(a_j) * forward(b_j)); a_j, b_j are work space.
+int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *const *b, size_t npairs,
+                                  size_t degree, size_t nmoduli, void *stream) {
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (!acc || (npairs && (!a || !b))) return fail(YA_ERR_INVALID, "null buffer");
+  const size_t pw = degree * nmoduli;
+  for (size_t j = 0; j < npairs; ++j) {
+    if (!a[j] || !b[j]) return fail(YA_ERR_INVALID, "null operand %zu", j);
+    if (overlaps(a[j], pw, acc, pw) || overlaps(b[j], pw, acc, pw) || overlaps(a[j], pw, b[j], pw))
+      return fail(YA_ERR_INVALID, "operands of pair %zu overlap each other or acc", j);
+    for (size_t k = 0; k < j; ++k)
+      if (overlaps(a[j], pw, a[k], pw) || overlaps(a[j], pw, b[k], pw) || overlaps(b[j], pw, a[k], pw) ||
+          overlaps(b[j], pw, b[k], pw))
+        return fail(YA_ERR_INVALID, "operands of pairs %zu and %zu overlap", k, j);
+  }
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  const ModDesc *d = desc_ptr(st, logn, 0);
+  const hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (logn < kMinPairLog || nmoduli > 65535) {
+    // small rings: the reference's sequence as separate launches
+    const NttShape sh = tuned_shape(logn, 0);
+    for (size_t j = 0; e == hipSuccess && j < npairs; ++j) {
+      e = launch_forward(a[j], d, logn, nmoduli, nmoduli, false, sh, s);
+      if (e == hipSuccess) e = launch_forward(b[j], d, logn, nmoduli, nmoduli, false, sh, s);
+    }
+    for (size_t j0 = 0; e == hipSuccess && j0 < npairs; j0 += kMacPairs) {
+      MacPairs mp{};
+      const int k = (int)std::min<size_t>(kMacPairs, npairs - j0);
+      for (int j = 0; j < k; ++j) {
+        mp.a[j] = a[j0 + j];
+        mp.b[j] = b[j0 + j];
+      }
+      for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
+        const size_t nr = std::min<size_t>(65535, nmoduli - r0);
+        MacPairs sub = mp;
+        for (int j = 0; j < k; ++j) {
+          sub.a[j] += r0 * degree;
+          sub.b[j] += r0 * degree;
+        }
+        e = launch_mac(acc + r0 * degree, sub, k, d + r0, degree, nr, s);
+      }
+    }
+    if (e == hipSuccess) e = launch_backward(acc, d, logn, nmoduli, nmoduli, sh, s);
+    return e == hipSuccess ? YA_OK : hip_fail(e, "add_products_backward (separate launches)");
+  }
+  // chunks of kMacPairs pairs: each chunk's products go to its a_j; all but
+  // the last chunk are added into acc, the last one is summed by the inverse
+  size_t j0 = 0;
+  do {
+    const int k = (int)std::min<size_t>(kMacPairs, npairs - j0);
+    PairPtrs pp{};
+    for (int j = 0; j < k; ++j) {
+      pp.a[j] = a[j0 + j];
+      pp.b[j] = b[j0 + j];
+    }
+    if (k > 0) e = launch_pair_products(pp, k, d, logn, nmoduli, tuned_mac_block_log(logn), s);
+    j0 += k;
+    if (e != hipSuccess) break;
+    if (j0 < npairs) {
+      for (int j = 0; e == hipSuccess && j < k; ++j)
+        e = launch_pointwise(PW_ADD, acc, acc, pp.a[j], d, degree, nmoduli, s);
+    } else {
+      e = launch_sum_backward(acc, pp, k, d, logn, nmoduli, s);
+    }
+  } while (e == hipSuccess && j0 < npairs);
+  return e == hipSuccess ? YA_OK : hip_fail(e, "add_products_backward");
 }
 
 static int seam_host(uint64_t *x, size_t degree, const uint64_t *w, const uint64_t *ws, uint64_t p,
@@ -778,9 +865,6 @@ static hipError_t pw_range(PwOp op, uint64_t *out, const uint64_t *a, const uint
   return hipSuccess;
 }
 
-static bool overlaps(const uint64_t *a, size_t na, const uint64_t *b, size_t nb) {
-  return a && b && a < b + nb && b < a + na;
-}
 
 int ya_poly_forward_lazy_add_product_many(uint64_t *acc, const uint64_t *add, const uint64_t *u,
                                           const uint64_t *key, size_t degree, size_t nmoduli, size_t count,
@@ -916,22 +1000,15 @@ static int polymul_impl(uint64_t *c, const uint64_t *a, const uint64_t *b, size_
       e = launch_polymul(c, a, b, d, logn, nres, nmoduli, tuned_shape(logn, 1), s);
     }
   } else {
-    // Large rings: forward(a) into c, forward(b) into the scratch,
-    // c = c * scratch, backward(c); polynomial by polynomial.
+    // Large rings: the split launch -- the top forward layers of a (into c)
+    // and b (into the scratch) per column, the rest of both forwards, the
+    // product and the inverse's block layers per block into c, the top
+    // inverse layers per column -- one polynomial's residues at a time
     const size_t pw = degree * nmoduli;
-    const NttShape sh = tuned_shape(logn, 0);
-    for (size_t k = 0; e == hipSuccess && k < count; ++k) {
-      uint64_t *ck = c + k * pw;
-      e = hipMemcpyAsync(ck, a + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
-      if (e == hipSuccess) e = hipMemcpyAsync(tmp, b + k * pw, pw * 8, hipMemcpyDeviceToDevice, s);
-      if (e == hipSuccess) e = launch_forward(ck, d, logn, nmoduli, nmoduli, true, sh, s);
-      if (e == hipSuccess) e = launch_forward(tmp, d, logn, nmoduli, nmoduli, true, sh, s);
-      for (size_t r0 = 0; e == hipSuccess && r0 < nmoduli; r0 += 65535) {
-        const size_t nr = std::min<size_t>(65535, nmoduli - r0);
-        e = launch_pointwise(PW_MUL, ck + r0 * degree, ck + r0 * degree, tmp + r0 * degree, d + r0,
-                             degree, nr, s);
-      }
-      if (e == hipSuccess) e = launch_backward(ck, d, logn, nmoduli, nmoduli, sh, s);
+    const int k = large_ring_split_k(logn);
+    for (size_t q = 0; e == hipSuccess && q < count; ++q) {
+      const SplitArgs sa{nullptr, a + q * pw, b + q * pw, c + q * pw, d, (uint32_t)nmoduli, tmp};
+      e = launch_polymul_split(sa, logn, nmoduli, k, s);
     }
   }
   if (tmp && !noalloc) {

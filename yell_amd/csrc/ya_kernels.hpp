@@ -81,6 +81,26 @@ struct MacPairs {
 hipError_t launch_mac(uint64_t *acc, const MacPairs &pairs, int npairs, const ModDesc *descs,
                       size_t n, size_t nres, hipStream_t st);
 
+// The large-ring multiply-accumulate: acc.add_product_of(forward(a_j),
+// forward(b_j)) for j < npairs, then acc.backward() (details/poly.hpp:119-132,
+// 171-186), the words of that sequence.  Two launches groups:
+// launch_pair_products: forward(a_j), forward(b_j) and their product, one
+// workgroup per (pair, residue, block) with both operands' passes sharing
+// twiddles; the product (canonical, natural order) goes to a_j, and b_j is
+// work space (above 2^kMaxBlockLog the top layers of a_j and b_j run first,
+// in place).  launch_sum_backward: acc + sum_j a_j (addmod: the same residues
+// as the muladd chain on canonical words), then the inverse.  Degrees
+// 2^kMinPairLog .. 2^kMaxLogN; at most kMacPairs pairs per launch; the 2 *
+// npairs + 1 buffers must not overlap.
+struct PairPtrs {
+  uint64_t *a[kMacPairs];
+  uint64_t *b[kMacPairs];
+};
+hipError_t launch_pair_products(const PairPtrs &pp, int npairs, const ModDesc *descs, int logn, size_t nmod,
+                                int lb, hipStream_t st);
+hipError_t launch_sum_backward(uint64_t *acc, const PairPtrs &pp, int npairs, const ModDesc *descs, int logn,
+                               size_t nmod, hipStream_t st);
+
 enum PwOp { PW_MUL = 0, PW_ADD = 1, PW_SUB = 2, PW_NEGATE = 3, PW_MULADD = 4, PW_CORRECT = 5 };
 
 // Element-wise op over `nres` residues of `n` words; residue r uses descs[r].p / pn.

@@ -492,4 +492,179 @@ hipError_t launch_inverse_lazy(uint64_t *data, const ModDesc *descs, int logn, s
   return backward_impl(data, descs, logn, nres, nres, true, NttShape{}, st);
 }
 
+// ---- The large-ring multiply-accumulate (launch_pair_products,
+// launch_sum_backward).
+
+// Top forward layers (index bits [B, B+K)) of polynomial blockIdx.y of the
+// pair table (a_{y/2} or b_{y/2}), in place, one thread per column.
+template <int B, int K>
+__global__ void __launch_bounds__(kTopThreads)
+k_fwd_top_pairs(PairPtrs pp, const ModDesc *__restrict__ descs) {
+  constexpr int NLOG = B + K;
+  constexpr uint32_t cols_per_res_log = B - 8;
+  const uint32_t res = blockIdx.x >> cols_per_res_log;
+  const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << 8) | threadIdx.x;
+  const uint32_t q = blockIdx.y;
+  const ModDesc d = descs[res];
+  uint64_t *base = ((q & 1u) ? pp.b[q >> 1] : pp.a[q >> 1]) + ((size_t)res << NLOG);
+  uint64_t x[1 << K];
+  gmem_load<B, K>(base, x, t);
+  fwd_layers<B, K, NLOG, B>(x, d.fwd, NLOG, 0u, d.p);
+  gmem_store<B, K>(base, x, t);
+}
+
+// Block blk of residue r of pair j: the remaining forward layers of a_j and
+// b_j as one dual pass sequence (shared twiddles), their Barrett product
+// (canonical from the lazy words), stored to a_j in natural order.  Residue
+// (prime) major over the XCDs, then pair, then block.
+// (Four waves per SIMD: at 2^13, two 512-thread workgroups share a CU.)
+template <int LB, int LE>
+__global__ void __launch_bounds__(1 << (LB - LE), 4)
+k_pair_product(PairPtrs pp, const ModDesc *__restrict__ descs, uint32_t npairs, uint32_t nlog) {
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t nblk_log = nlog - LB;
+  const uint32_t v = xcd_order(blockIdx.x, gridDim.x);
+  const uint32_t per_prime = npairs << nblk_log;
+  const uint32_t r = v / per_prime, rest = v % per_prime;
+  const uint32_t j = rest >> nblk_log, blk = rest & ((1u << nblk_log) - 1u);
+  const ModDesc d = descs[r];
+  const size_t off = ((size_t)r << nlog) + ((size_t)blk << LB);
+  uint64_t *px = uniform_ptr(pp.a[j] + off);
+  const uint64_t *py = uniform_ptr(pp.b[j] + off);
+  const uint32_t t = threadIdx.x;
+  uint64_t x[1 << LE], y[1 << LE];
+  constexpr int W0 = FirstWindow<LB, LE>::value;
+  gmem_load<W0, LE>(px, x, t);
+  gmem_load<W0, LE>(py, y, t);
+  fwd_block_dual<LB, LE>(x, y, lds, d.fwd, nlog, blk << LB, d.p, t);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(x[e], y[e], d.p, d.pn);
+  store_coalesced<LB, LE>(px, x, lds, t);
+}
+
+// Block blk of residue r: acc + sum_j a_j with coalesced reads (thread t:
+// words t | e << W0), one LDS exchange into the inverse's first layout, the
+// inverse layers of the block (TOP_MERGED when the block is the residue).
+// acc is read before the first barrier and written after the last, by this
+// workgroup only.
+template <int LB, int LE, int TOP>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_sum_inv(uint64_t *__restrict__ acc, PairPtrs pp, uint32_t npairs, const ModDesc *__restrict__ descs,
+          uint32_t nlog) {
+  typedef const __attribute__((address_space(1))) uint64_t gw;
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  const uint32_t nblk_log = nlog - LB;
+  const uint32_t r = blockIdx.x >> nblk_log, blk = blockIdx.x & ((1u << nblk_log) - 1u);
+  const ModDesc d = descs[r];
+  const size_t off = ((size_t)r << nlog) + ((size_t)blk << LB);
+  const uint32_t t = threadIdx.x;
+  constexpr int W0 = FirstWindow<LB, LE>::value;
+  const uint32_t b0 = win_base<W0, LE>(t);
+  uint64_t x[1 << LE];
+  gw *ar = (gw *)uniform_ptr(acc + off);
+#pragma unroll
+  for (int e = 0; e < (1 << LE); ++e) x[e] = ar[b0 | ((uint32_t)e << W0)];
+  // U products' loads in flight at a time (4 with 256-thread workgroups, 2
+  // within the 128 registers of the larger ones), then one at a time
+  constexpr int U = LB <= 12 ? 4 : 2;
+  uint32_t j = 0;
+  for (; j + U <= npairs; j += U) {
+    uint64_t v[U][1 << LE];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      gw *pu = (gw *)uniform_ptr(pp.a[j + u] + off);
+#pragma unroll
+      for (int e = 0; e < (1 << LE); ++e) v[u][e] = pu[b0 | ((uint32_t)e << W0)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < (1 << LE); ++e) x[e] = addmod(x[e], v[u][e], d.p);
+  }
+  for (; j < npairs; ++j) {
+    gw *pj = (gw *)uniform_ptr(pp.a[j] + off);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) x[e] = addmod(x[e], pj[b0 | ((uint32_t)e << W0)], d.p);
+  }
+  lds_store<W0, LE>(lds, x, t);
+  __syncthreads();
+  lds_load<0, LE>(lds, x, t);
+  inv_block<LB, LE, TOP>(x, acc + off, lds, d.inv, nlog, blk << LB, d, t);
+}
+
+template <int B>
+static hipError_t fwd_top_pairs_b(const PairPtrs &pp, int npairs, const ModDesc *descs, int k, size_t nmod,
+                                  hipStream_t st) {
+  const dim3 grid((unsigned)(nmod << (B - 8)), (unsigned)(2 * npairs));
+  switch (k) {
+    case 1: hipLaunchKernelGGL((k_fwd_top_pairs<B, 1>), grid, dim3(kTopThreads), 0, st, pp, descs); break;
+    case 2: hipLaunchKernelGGL((k_fwd_top_pairs<B, 2>), grid, dim3(kTopThreads), 0, st, pp, descs); break;
+    case 3: hipLaunchKernelGGL((k_fwd_top_pairs<B, 3>), grid, dim3(kTopThreads), 0, st, pp, descs); break;
+    case 4: hipLaunchKernelGGL((k_fwd_top_pairs<B, 4>), grid, dim3(kTopThreads), 0, st, pp, descs); break;
+    case 5: hipLaunchKernelGGL((k_fwd_top_pairs<B, 5>), grid, dim3(kTopThreads), 0, st, pp, descs); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int LB>
+static hipError_t pair_product_lb(const PairPtrs &pp, int npairs, const ModDesc *descs, int logn, size_t nmod,
+                                  hipStream_t st) {
+  const dim3 grid((unsigned)((nmod * npairs) << (logn - LB)));
+  hipLaunchKernelGGL((k_pair_product<LB, 4>), grid, dim3(1u << (LB - 4)), 0, st, pp, descs, (uint32_t)npairs,
+                     (uint32_t)logn);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_products(const PairPtrs &pp, int npairs, const ModDesc *descs, int logn, size_t nmod,
+                                int lb, hipStream_t st) {
+  if (logn < kMinPairLog || logn > kMaxLogN || npairs <= 0 || npairs > kMacPairs || nmod == 0)
+    return hipErrorInvalidValue;
+  if (((nmod * npairs) << (logn - kMinPairLog)) > 0x7fffffffu) return hipErrorInvalidValue;
+  if (logn <= lb) lb = logn;
+  if ((lb != 13 && lb != 14) || logn - lb > 5) return hipErrorInvalidValue;
+  if (logn > lb) {
+    const hipError_t e = lb == 13 ? fwd_top_pairs_b<13>(pp, npairs, descs, logn - lb, nmod, st)
+                                  : fwd_top_pairs_b<14>(pp, npairs, descs, logn - lb, nmod, st);
+    if (e != hipSuccess) return e;
+  }
+  return lb == 13 ? pair_product_lb<13>(pp, npairs, descs, logn, nmod, st)
+                  : pair_product_lb<14>(pp, npairs, descs, logn, nmod, st);
+}
+
+template <int LB, int TOP, int LE = 4>
+static hipError_t sum_inv_lb(uint64_t *acc, const PairPtrs &pp, int npairs, const ModDesc *descs, int logn,
+                             size_t nmod, hipStream_t st) {
+  const dim3 grid((unsigned)(nmod << (logn - LB)));
+  hipLaunchKernelGGL((k_sum_inv<LB, LE, TOP>), grid, dim3(1u << (LB - LE)), 0, st, acc, pp, (uint32_t)npairs,
+                     descs, (uint32_t)logn);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_backward(uint64_t *acc, const PairPtrs &pp, int npairs, const ModDesc *descs, int logn,
+                               size_t nmod, hipStream_t st) {
+  if (logn < kMinPairLog || logn > kMaxLogN || npairs < 0 || npairs > kMacPairs || nmod == 0 ||
+      (nmod << (logn - 12)) > 0x7fffffffu)
+    return hipErrorInvalidValue;
+  // blocks of 2^14, or 2^13 / 2^12 to give a launch of few residues more
+  // workgroups (the inverse's column pass then takes the rest)
+  const int lb = logn <= kMaxBlockLog ? logn : large_block_log(logn, nmod);
+  hipError_t e;
+  if (logn == lb) {
+    e = lb == 13 ? sum_inv_lb<13, TOP_MERGED>(acc, pp, npairs, descs, logn, nmod, st)
+                 : sum_inv_lb<14, TOP_MERGED>(acc, pp, npairs, descs, logn, nmod, st);
+    return e;
+  }
+  // radix-8 passes below 2^14: twice the waves of radix-16 ones, for a pass
+  // that is short of waves (the sum's loads and the inverse's dependency
+  // chains need them)
+  switch (lb) {
+    case 12: e = sum_inv_lb<12, TOP_GS, 3>(acc, pp, npairs, descs, logn, nmod, st); break;
+    case 13: e = sum_inv_lb<13, TOP_GS, 3>(acc, pp, npairs, descs, logn, nmod, st); break;
+    default: e = sum_inv_lb<14, TOP_GS>(acc, pp, npairs, descs, logn, nmod, st); break;
+  }
+  if (e != hipSuccess) return e;
+  return launch_inv_top(acc, descs, logn, lb, nmod, nmod, false, st);
+}
+
 }  // namespace ya

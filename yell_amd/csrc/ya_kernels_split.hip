@@ -81,8 +81,10 @@ k_split_inv_cols(SplitArgs s) {
 
 // Block j of 2^S words of residue v: c_j holds layers [S, S+K) of forward(a),
 // t_j those of forward(b).
+// (Four waves per SIMD: 512-thread workgroups of 2^13 blocks share a CU in
+// pairs.)
 template <int S, int LE>
-__global__ void __launch_bounds__(1 << (S - LE))
+__global__ void __launch_bounds__(1 << (S - LE), 4)
 k_split_blocks(SplitArgs s, uint32_t nlog) {
   __shared__ uint64_t lds[LdsWords<S>::value];
   const uint32_t kb = nlog - S;
@@ -135,6 +137,11 @@ int split_choose_k(int logn, size_t nres, int cus) {
   return k;
 }
 
+int large_ring_split_k(int logn) {
+  if (logn <= kMaxBlockLog || logn > kMaxLogN) return 0;
+  return logn <= 18 ? logn - 13 : logn - 14;
+}
+
 hipError_t launch_polymul_split(const SplitArgs &s, int logn, size_t nres, int k, hipStream_t st) {
   if (nres == 0 || nres > ((size_t)1 << 20)) return hipErrorInvalidValue;
   const int S = logn - k;
@@ -145,6 +152,9 @@ hipError_t launch_polymul_split(const SplitArgs &s, int logn, size_t nres, int k
     YA_SPLIT(10, 2) YA_SPLIT(10, 3) YA_SPLIT(10, 4)
     YA_SPLIT(11, 2) YA_SPLIT(11, 3)
     YA_SPLIT(12, 2)
+    // rings above 2^14 (large_ring_split_k)
+    YA_SPLIT(13, 2) YA_SPLIT(13, 3) YA_SPLIT(13, 4) YA_SPLIT(13, 5)
+    YA_SPLIT(14, 5)
     default: return hipErrorInvalidValue;
   }
 #undef YA_SPLIT

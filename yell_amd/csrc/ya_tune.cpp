@@ -2,6 +2,7 @@
 // autotuner (ya_autotune in ya_batch.hip) times the candidate shapes of a
 // degree class on the device and stores the winner here.
 #include <atomic>
+#include <mutex>
 
 #include "ya_internal.hpp"
 
@@ -9,15 +10,29 @@ namespace ya {
 
 static std::atomic<int> g_le[3][kMaxLogN + 1];
 static std::atomic<int> g_dual[kMaxLogN + 1];  // fused kernel: dual-forward form
-static std::atomic<bool> g_init{false};
+static std::atomic<int> g_mac_lb[kMaxLogN + 1];  // large-ring MAC: pair-product block log
+static std::once_flag g_init;
 
+// Defaults, set once; a thread that arrives while another sets them waits.
 static void init_defaults() {
-  bool expected = false;
-  if (!g_init.compare_exchange_strong(expected, true)) return;
-  for (int k = 0; k < 3; ++k)
-    for (int l = 0; l <= kMaxLogN; ++l) g_le[k][l].store(4);
-  // dual-forward fused form: measured faster at every class it is built for
-  for (int l = 0; l <= kMaxLogN; ++l) g_dual[l].store(l >= 10 ? 1 : 0);
+  std::call_once(g_init, [] {
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l <= kMaxLogN; ++l) g_le[k][l].store(4);
+    // dual-forward fused form: measured faster at every class it is built for
+    for (int l = 0; l <= kMaxLogN; ++l) g_dual[l].store(l >= 10 ? 1 : 0);
+    for (int l = 0; l <= kMaxLogN; ++l) g_mac_lb[l].store(14);
+  });
+}
+
+int tuned_mac_block_log(int logn) {
+  init_defaults();
+  const int lb = logn >= 0 && logn <= kMaxLogN ? g_mac_lb[logn].load() : 14;
+  return logn <= lb ? logn : lb;
+}
+
+void set_tuned_mac_block_log(int logn, int lb) {
+  init_defaults();
+  if (logn >= 0 && logn <= kMaxLogN) g_mac_lb[logn].store(lb);
 }
 
 NttShape tuned_shape(int logn, int kind) {
