@@ -144,38 +144,57 @@ __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__
   }
 }
 
+// The lane number (0..63), computed where it is written: the asm is volatile
+// so the value is not hoisted out of the item loop and kept in a register.
+__device__ __forceinline__ uint32_t lane_id_opaque() {
+  uint32_t lane;
+  __asm__ volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+  return lane;
+}
+
+// Thread 0 claims the next work item: its own XCD segment first, then the
+// others in turn; the segment cursor lives in LDS (no per-thread register is
+// carried across items).  0xffffffff when every segment is drained.
+__device__ __forceinline__ uint32_t claim_item(const BatchSegs &segs, BatchQueue *q, uint32_t &seg,
+                                               uint32_t &tried) {
+  while (tried < kSegs) {
+    const uint32_t n = segs.begin[seg + 1] - segs.begin[seg];
+    if (n != 0) {
+      const uint32_t j = atomicAdd(&q->next[seg], 1u);
+      if (j < n) return segs.begin[seg] + j;
+    }
+    seg = (seg + 1) % kSegs;
+    ++tried;
+  }
+  return 0xffffffffu;
+}
+
 __global__ void __launch_bounds__(kBatchThreads)
 k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
                 const BatchRes *__restrict__ res, BatchQueue *q, uint64_t *__restrict__ scratch) {
   __shared__ uint64_t lds[LdsWords<14>::value];
-  __shared__ uint32_t s_item;
-  const uint32_t home = blockIdx.x % kSegs;
-  uint32_t seg = home, tried = 0;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      uint32_t k = 0xffffffffu;
-      while (tried < kSegs) {  // own segment first, then the others in turn
-        const uint32_t n = segs.begin[seg + 1] - segs.begin[seg];
-        if (n != 0) {
-          const uint32_t j = atomicAdd(&q->next[seg], 1u);
-          if (j < n) {
-            k = segs.begin[seg] + j;
-            break;
-          }
-        }
-        seg = (seg + 1) % kSegs;
-        ++tried;
-      }
-      s_item = k;
-    }
-    __syncthreads();
-    const uint32_t k = s_item;
-    __syncthreads();
+  // claim-ahead: item i is read from s_item[i & 1] while thread 0 claims
+  // item i + 1 into the other slot, so the queue atomic's round trip overlaps
+  // the work and one barrier per item remains
+  __shared__ uint32_t s_item[2], s_seg, s_tried;
+  if (threadIdx.x == 0) {
+    s_seg = blockIdx.x % kSegs;
+    s_tried = 0;
+    s_item[0] = claim_item(segs, q, s_seg, s_tried);
+  }
+  __syncthreads();
+  // The thread index is rebuilt per item from the wave's base (a scalar) and
+  // the lane number, so no vector register holds it across the items (one
+  // that did was spilled to scratch, the bodies needing all 128).
+  const uint32_t wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+  for (uint32_t i = 0;; ++i) {
+    const uint32_t k = s_item[i & 1u];
     if (k == 0xffffffffu) break;
-    const BatchItem it = items[k];
     // per item, so that nothing derived from the thread index is hoisted out
     // of the loop (and kept live across every item)
-    const uint32_t t = phase_v(threadIdx.x);
+    const uint32_t t = wave_base | lane_id_opaque();
+    if (t == 0) s_item[(i + 1) & 1u] = claim_item(segs, q, s_seg, s_tried);
+    const BatchItem it = items[k];
     switch (it.logn * 8 + it.le) {
       case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, t); break;
       case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, t); break;
@@ -195,10 +214,12 @@ k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
       case 15 * 8 + 4: batch_15(it, res, lds, scratch + ((size_t)blockIdx.x << 14), t); break;
       default: break;  // rejected by the host
     }
-    __syncthreads();  // LDS of this item is dead before the next item's first store
+    // the item's LDS is dead before the next item's first store, and the
+    // next claim is visible
+    __syncthreads();
   }
   // The last workgroup to leave resets the queue for the next launch.
-  if (threadIdx.x == 0) {
+  if ((wave_base | lane_id_opaque()) == 0) {
     __threadfence();
     if (atomicAdd(&q->done, 1u) == gridDim.x - 1) {
       for (int i = 0; i < kSegs; ++i) q->next[i] = 0;
