@@ -531,7 +531,8 @@ def rlwe_cpu_reference(rl, seconds=3.0):
                       "flow (N=%d, L=%d) through the reference's sources" % (reps, reps_t, th, n, L)}
 
 
-def served_bench(rates, duration, shapes, pool=8, max_pending=1024, batch_jobs=256):
+def served_bench(rates, duration, shapes, pool=8, max_pending=1024, batch_jobs=256, modes=("batched", "plain"),
+                 inflight=3, verify=1):
     """Poisson stream of host-operand requests (shape drawn uniformly from
     `shapes`, `pool` operand pairs per shape): continuous batching vs one
     request per launch, same seeded stream."""
@@ -539,8 +540,9 @@ def served_bench(rates, duration, shapes, pool=8, max_pending=1024, batch_jobs=2
            "duration_s": duration, "runs": []}
     degrees = (ctypes.c_uint32 * len(shapes))(*[n for n, _ in shapes])
     mods = (ctypes.c_uint32 * len(shapes))(*[L for _, L in shapes])
-    for mode, cfgkw in (("batched", dict(max_batch_jobs=batch_jobs, max_inflight=3)),
-                        ("plain", dict(max_batch_jobs=1, max_inflight=1))):
+    configs = {"batched": dict(max_batch_jobs=batch_jobs, max_inflight=inflight),
+               "plain": dict(max_batch_jobs=1, max_inflight=1)}
+    for mode, cfgkw in ((m, configs[m]) for m in modes):
         cfg = capi.ServerConfig(hbm_budget_bytes=2 << 30, max_pending=max_pending, max_moduli=16, **cfgkw)
         srv = ctypes.c_void_p()
         capi.call("ya_server_create", ctypes.byref(srv), ctypes.byref(cfg))
@@ -551,7 +553,7 @@ def served_bench(rates, duration, shapes, pool=8, max_pending=1024, batch_jobs=2
         capi.call("ya_server_run_poisson", srv, ctypes.byref(warm), ctypes.byref(capi.ServeReport()))
         for rate in rates:
             spec = capi.PoissonSpec(rate_per_s=rate, duration_s=duration, nshapes=len(shapes),
-                                    degrees=degrees, nmoduli=mods, pool=pool, verify=1, seed=99)
+                                    degrees=degrees, nmoduli=mods, pool=pool, verify=verify, seed=99)
             rep = capi.ServeReport()
             log("served", mode, "rate", rate)
             capi.call("ya_server_run_poisson", srv, ctypes.byref(spec), ctypes.byref(rep))
@@ -559,6 +561,34 @@ def served_bench(rates, duration, shapes, pool=8, max_pending=1024, batch_jobs=2
             d.update({"mode": mode, "offered_rate_per_s": rate})
             out["runs"].append(d)
         capi.call("ya_server_destroy", srv)
+    return out
+
+
+KNEE_P99_US = 5000.0
+
+
+def served_sweep(rates, duration, n, L, batch_jobs=256, inflight=3, verify=1):
+    """Batched served stream (C2 shape, every result verified) at increasing
+    offered loads until it saturates: the knee is the highest offered rate
+    whose completed throughput keeps within 3% of it with no rejection, no
+    failure and p99 latency within KNEE_P99_US."""
+    out = served_bench(rates, duration, [(n, L)], modes=("batched",), batch_jobs=batch_jobs, inflight=inflight,
+                       verify=verify)
+    out["knee_rule"] = "throughput >= 0.97 x offered, no rejection or failure, p99 <= %.0f us" % KNEE_P99_US
+    knee = None
+    for r in out["runs"]:
+        # against the arrivals the Poisson draw actually made in the window
+        ok = (r["rejected"] == 0 and r["failed"] == 0 and r["mismatches"] == 0 and
+              r["throughput_per_s"] >= 0.97 * r["offered"] / duration and r["p99_us"] <= KNEE_P99_US)
+        r["sustained"] = bool(ok)
+        if ok and (knee is None or r["offered_rate_per_s"] > knee):
+            knee = r["offered_rate_per_s"]
+    out["knee_rate_per_s"] = knee
+    out["max_throughput_per_s"] = max(r["throughput_per_s"] for r in out["runs"])
+    at = [r for r in out["runs"] if r["offered_rate_per_s"] == knee]
+    if at:
+        out["p50_us_at_knee"], out["p99_us_at_knee"] = at[0]["p50_us"], at[0]["p99_us"]
+        out["jobs_per_batch_at_knee"] = at[0]["mean_batch_jobs"]
     return out
 
 
@@ -574,10 +604,14 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch, probes and tuner")
     ap.add_argument("--only", default="",
                     help="comma list of the extra sections to run (tune, probe, c3, c4, crt, latency, u32, "
-                         "samplers, rlwe, served); default all")
+                         "samplers, rlwe, sweep, served); default all")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[4000.0, 8000.0, 12000.0],
                     help="offered loads (requests/s) of the served-form runs")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--sweep-rates", type=float, nargs="+",
+                    default=[4000.0, 8000.0, 12000.0, 16000.0, 20000.0, 24000.0, 28000.0],
+                    help="offered loads of the batched served sweep to the saturation knee")
+    ap.add_argument("--sweep-seconds", type=float, default=1.0)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -734,6 +768,12 @@ def main():
     if want("rlwe"):
         log("RLWE flow")
         out["rlwe_flow"] = rlwe_bench(stream, ev)
+    if want("sweep"):
+        log("served sweep")
+        sw = served_sweep(args.sweep_rates, args.sweep_seconds, n, L)
+        out["served_sweep"] = sw
+        out["served_knee_rate_per_s"] = sw["knee_rate_per_s"]
+        out["served_p99_us_at_knee"] = sw.get("p99_us_at_knee")
     if want("served"):
         sv = served_bench(args.serve_rates, args.serve_seconds, [(n, L)])
         out["served"] = sv

@@ -106,3 +106,43 @@ def test_poisson_stream_verified():
     assert d["completed"] + d["rejected"] == d["offered"]
     assert d["p99_us"] >= d["p50_us"] > 0
     capi.call("ya_server_destroy", s)
+
+
+class Pageable:
+    """An ordinary numpy buffer: not in the library's pinned registry, so the
+    server moves it with DMA copies instead of its transfer kernels."""
+
+    def __init__(self, nwords):
+        self.a = np.zeros(nwords + 1, dtype=np.uint64)[1:]   # 8-byte aligned, usually not 16
+        self.p = self.a.ctypes.data
+
+
+def test_pinned_and_pageable_requests_in_one_batch():
+    """A batch mixing requests in registered pinned memory (moved by the
+    transfer kernels), in ordinary pageable memory (DMA copies), and with a
+    pinned buffer offset by 8 bytes (not 16-byte aligned: DMA): every result
+    equals the oracle's."""
+    s = _server(hbm_budget_bytes=256 << 20, max_batch_jobs=32, max_inflight=1)
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(24):
+        n, L = 1 << int(rng.integers(10, 15)), int(rng.integers(1, 9))
+        kind = k % 3
+        if kind == 0:
+            a, b, c = Pinned(n * L), Pinned(n * L), Pinned(n * L)
+        elif kind == 1:
+            a, b, c = Pageable(n * L), Pageable(n * L), Pageable(n * L)
+        else:
+            a, b, c = Pinned(n * L), Pinned(n * L), Pinned(n * L + 1)
+            c.p += 8
+            c.a = np.ctypeslib.as_array((ctypes.c_uint64 * (n * L)).from_address(c.p))
+        a.a[:] = capi.sample_uniform(n, L, 900 + 2 * k)
+        b.a[:] = capi.sample_gaussian(n, L, 3.2, 901 + 2 * k)
+        reqs.append((n, L, a, b, c))
+    say("server pinned + pageable mix", len(reqs))
+    tickets = [_submit(s, a, b, c, n, L) for n, L, a, b, c in reqs]
+    for t in tickets:
+        capi.call("ya_server_wait", s, t, 60000, None)
+    for n, L, a, b, c in reqs:
+        np.testing.assert_array_equal(c.a, orc.polymul(a.a, b.a, n, L))
+    capi.call("ya_server_destroy", s)

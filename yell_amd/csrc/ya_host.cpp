@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -319,6 +320,54 @@ int check_shape(size_t degree, size_t cm0, size_t nmoduli, int *logn) {
   return YA_OK;
 }
 
+// Pinned host allocations of this library (ya_host_alloc_pinned and the
+// server's own buffers): base -> {bytes, device address of the base}.  A
+// kernel may read or write host memory only inside one of these, so the
+// served form's copy kernels take a request's buffers only after
+// pinned_device_range found the whole range here.
+struct PinnedBlock {
+  size_t bytes;
+  uintptr_t dev;
+};
+static std::mutex g_pin_mu;
+static std::map<uintptr_t, PinnedBlock> g_pinned;
+
+int pinned_alloc(void **ptr, size_t bytes) {
+  *ptr = nullptr;
+  hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  void *dev = nullptr;
+  if (hipHostGetDevicePointer(&dev, *ptr, 0) == hipSuccess && dev) {
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    g_pinned[(uintptr_t)*ptr] = PinnedBlock{bytes, (uintptr_t)dev};
+  } else {
+    (void)hipGetLastError();  // usable for copies, not registered for kernels
+  }
+  return YA_OK;
+}
+
+int pinned_free(void *ptr) {
+  if (!ptr) return YA_OK;
+  {
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    g_pinned.erase((uintptr_t)ptr);
+  }
+  const hipError_t e = hipHostFree(ptr);
+  return e == hipSuccess ? YA_OK : hip_fail(e, "hipHostFree");
+}
+
+void *pinned_device_range(const void *p, size_t bytes) {
+  if (!p) return nullptr;
+  const uintptr_t a = (uintptr_t)p;
+  std::lock_guard<std::mutex> g(g_pin_mu);
+  auto it = g_pinned.upper_bound(a);
+  if (it == g_pinned.begin()) return nullptr;
+  --it;
+  const uintptr_t base = it->first;
+  if (a < base || bytes > it->second.bytes || a - base > it->second.bytes - bytes) return nullptr;
+  return (void *)(it->second.dev + (a - base));
+}
+
 }  // namespace ya
 
 using namespace ya;
@@ -472,14 +521,9 @@ int ya_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream) {
 }
 int ya_host_alloc_pinned(void **ptr, size_t bytes) {
   if (!ptr) return fail(YA_ERR_INVALID, "null ptr");
-  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
-  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
-  return YA_OK;
+  return pinned_alloc(ptr, bytes);
 }
-int ya_host_free_pinned(void *ptr) {
-  if (ptr) YA_HIP(hipHostFree(ptr), "hipHostFree");
-  return YA_OK;
-}
+int ya_host_free_pinned(void *ptr) { return pinned_free(ptr); }
 int ya_pointer_is_device(const void *ptr, int *is_device) {
   if (!is_device) return fail(YA_ERR_INVALID, "null output");
   *is_device = 0;

@@ -48,6 +48,14 @@ void set_tuned_dual(int logn, int dual);
 int tuned_mac_block_log(int logn);
 void set_tuned_mac_block_log(int logn, int lb);
 
+// Pinned host memory registered for kernel access (ya_host_alloc_pinned
+// allocates through pinned_alloc).  pinned_device_range returns the device
+// address of [p, p + bytes) when the whole range lies inside one registered
+// allocation, else null.
+int pinned_alloc(void **ptr, size_t bytes);
+int pinned_free(void *ptr);
+void *pinned_device_range(const void *p, size_t bytes);
+
 // Largest Gaussian parameter of the samplers: ceil(13 sigma) stays below every
 // prime of both builds, so a signed sample maps to one residue without reduction.
 constexpr double kMaxGaussianSigma = 1e6;

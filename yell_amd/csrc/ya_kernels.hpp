@@ -101,6 +101,20 @@ hipError_t launch_pair_products(const PairPtrs &pp, int npairs, const ModDesc *d
 hipError_t launch_sum_backward(uint64_t *acc, const PairPtrs &pp, int npairs, const ModDesc *descs, int logn,
                                size_t nmod, hipStream_t st);
 
+// Copies between pinned host memory and device memory by a kernel (the
+// served form's operand upload and result download: one launch for a whole
+// batch instead of one DMA copy per buffer).  Each descriptor moves `words`
+// 64-bit words, an even number, from src to dst, both 16-byte aligned device
+// addresses (a pinned host buffer's device address from pinned_device_range).
+// The descriptor array itself must be readable by the device (pinned host or
+// device memory).  At most 65535 descriptors; max_words the largest `words`.
+struct XferDesc {
+  const uint64_t *src;
+  uint64_t *dst;
+  uint64_t words;
+};
+hipError_t launch_xfer(const XferDesc *descs, size_t n, size_t max_words, hipStream_t st);
+
 enum PwOp { PW_MUL = 0, PW_ADD = 1, PW_SUB = 2, PW_NEGATE = 3, PW_MULADD = 4, PW_CORRECT = 5 };
 
 // Element-wise op over `nres` residues of `n` words; residue r uses descs[r].p / pn.

@@ -88,6 +88,41 @@ hipError_t launch_pointwise_many(PwOp op, uint64_t *out, const uint64_t *a, cons
   return hipGetLastError();
 }
 
+// One 128 KiB chunk of descriptor blockIdx.y per workgroup; each thread
+// moves 16-byte vectors, four loads in flight before their stores (reads
+// over PCIe have microseconds of latency).
+constexpr uint32_t kXferThreads = 256, kXferChunkLog = 14;
+typedef uint64_t xfer_v2 __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(kXferThreads) k_xfer(const XferDesc *__restrict__ descs) {
+  const XferDesc x = descs[blockIdx.y];
+  const uint64_t w0 = (uint64_t)blockIdx.x << kXferChunkLog;
+  if (w0 >= x.words) return;
+  const uint64_t w1 = min(x.words, w0 + (1ull << kXferChunkLog));
+  const xfer_v2 *src = (const xfer_v2 *)x.src;
+  xfer_v2 *dst = (xfer_v2 *)x.dst;
+  const uint64_t v0 = w0 >> 1, v1 = w1 >> 1;  // 16-byte vector indices
+  constexpr uint32_t U = 4;
+  uint64_t v = v0 + threadIdx.x;
+  for (; v + (U - 1) * kXferThreads < v1; v += U * kXferThreads) {
+    xfer_v2 t[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) t[u] = src[v + u * kXferThreads];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) dst[v + u * kXferThreads] = t[u];
+  }
+  for (; v < v1; v += kXferThreads) dst[v] = src[v];
+}
+
+hipError_t launch_xfer(const XferDesc *descs, size_t n, size_t max_words, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (n > 65535 || max_words == 0) return hipErrorInvalidValue;
+  const size_t chunks = (max_words + (1ull << kXferChunkLog) - 1) >> kXferChunkLog;
+  if (chunks > 0x7fffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_xfer, dim3((unsigned)chunks, (unsigned)n), dim3(kXferThreads), 0, st, descs);
+  return hipGetLastError();
+}
+
 hipError_t launch_pointwise(PwOp op, uint64_t *out, const uint64_t *a, const uint64_t *b,
                             const ModDesc *descs, size_t n, size_t nres, hipStream_t st) {
   return launch_pointwise_many(op, out, a, b, descs, n, nres, 1, 0, st);

@@ -92,11 +92,17 @@ struct Slot {
   hipEvent_t t0 = nullptr, t1 = nullptr;  // batch start / kernel end (timing)
   clk::time_point host_dispatch;          // when the dispatcher took the batch
   double launch_us = 0;                   // host time spent issuing the batch
-  BatchItem *d_items = nullptr, *h_items = nullptr;
-  BatchRes *d_res = nullptr, *h_res = nullptr;
+  BatchItem *h_items = nullptr;  // pinned host tables of the batch ...
+  BatchRes *h_res = nullptr;
+  const BatchItem *dh_items = nullptr;  // ... and their device addresses
+  const BatchRes *dh_res = nullptr;
   BatchQueue *d_q = nullptr;
   uint64_t *d_scratch = nullptr;
   size_t cap_items = 0, cap_res = 0;
+  // copy descriptors of the transfer kernels (pinned host memory, read by
+  // the kernels through their device addresses): uploads, downloads
+  XferDesc *h_xin = nullptr, *h_xout = nullptr;
+  const XferDesc *d_xin = nullptr, *d_xout = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
   bool busy = false;
 };
@@ -163,12 +169,12 @@ int server_free(ya_server *s) {
     if (sl.t0) (void)hipEventDestroy(sl.t0);
     if (sl.t1) (void)hipEventDestroy(sl.t1);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    (void)hipFree(sl.d_items);
-    (void)hipFree(sl.d_res);
     (void)hipFree(sl.d_q);
     (void)hipFree(sl.d_scratch);
-    if (sl.h_items) (void)hipHostFree(sl.h_items);
-    if (sl.h_res) (void)hipHostFree(sl.h_res);
+    (void)pinned_free(sl.h_items);
+    (void)pinned_free(sl.h_res);
+    (void)pinned_free(sl.h_xin);
+    (void)pinned_free(sl.h_xout);
   }
   (void)hipFree(s->arena_base);
   delete s;
@@ -215,8 +221,35 @@ int launch_slot(ya_server *s, int k) {
     split_k = one_class ? split_factor(s->st, lg, res.size()) : 0;
   }
   const size_t nres = direct ? jobs[0].nmoduli : res.size();
+  // Uploads: a request whose a, b and c lie in pinned memory of this library
+  // (16-byte aligned) is moved by the transfer kernel -- one launch for the
+  // batch, no per-copy set-up -- the others by DMA copies.
+  std::vector<uint8_t> by_kernel(jobs.size(), 0);
+  size_t nin = 0, nout = 0, max_in = 0, max_out = 0;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const Request &r = *sl.reqs[j];
+    const size_t bytes = r.words * 8;
+    const auto aligned = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
+    const void *da = pinned_device_range(r.a, bytes), *db = pinned_device_range(r.b, bytes);
+    void *dc = pinned_device_range(r.c, bytes);
+    if (!da || !db || !dc || !aligned(da) || !aligned(db) || !aligned(dc) || (r.words & 1)) continue;
+    if (nin + 2 > 65535 || nout + 1 > 65535) continue;  // one launch's descriptor limit
+    by_kernel[j] = 1;
+    if ((const uint64_t *)db == (const uint64_t *)da + r.words) {  // adjacent: b follows a in the arena too
+      sl.h_xin[nin++] = XferDesc{(const uint64_t *)da, (uint64_t *)jobs[j].a, 2 * r.words};
+      max_in = std::max(max_in, 2 * r.words);
+    } else {
+      sl.h_xin[nin++] = XferDesc{(const uint64_t *)da, (uint64_t *)jobs[j].a, r.words};
+      sl.h_xin[nin++] = XferDesc{(const uint64_t *)db, (uint64_t *)jobs[j].b, r.words};
+      max_in = std::max(max_in, r.words);
+    }
+    sl.h_xout[nout++] = XferDesc{jobs[j].c, (uint64_t *)dc, r.words};
+    max_out = std::max(max_out, r.words);
+  }
   if (e == hipSuccess) e = hipEventRecord(sl.t0, sl.stream);
+  if (e == hipSuccess) e = launch_xfer(sl.d_xin, nin, max_in, sl.stream);
   for (size_t j = 0; j < jobs.size() && e == hipSuccess; ++j) {
+    if (by_kernel[j]) continue;
     const Request &r = *sl.reqs[j];
     if (r.b == r.a + r.words) {  // a and b adjacent on the host too: one copy (b follows a in the arena)
       e = hipMemcpyAsync((void *)jobs[j].a, r.a, 2 * r.words * 8, hipMemcpyHostToDevice, sl.stream);
@@ -226,13 +259,13 @@ int launch_slot(ya_server *s, int k) {
         e = hipMemcpyAsync((void *)jobs[j].b, r.b, r.words * 8, hipMemcpyHostToDevice, sl.stream);
     }
   }
+  // The batch tables are written to the slot's pinned host arrays, which the
+  // kernels read through their device addresses (a few KiB, read once per
+  // item): no DMA copy, whose fixed set-up costs as much as a small batch's
+  // transfer.
   if (!direct) {
     std::memcpy(sl.h_items, items.data(), items.size() * sizeof(BatchItem));
     std::memcpy(sl.h_res, res.data(), res.size() * sizeof(BatchRes));
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(sl.d_items, sl.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, sl.stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(sl.d_res, sl.h_res, res.size() * sizeof(BatchRes), hipMemcpyHostToDevice, sl.stream);
   }
   if (e == hipSuccess) {
     if (split_k) {
@@ -240,15 +273,17 @@ int launch_slot(ya_server *s, int k) {
       // go in place into the arena copy of b, which the server owns
       const SplitArgs sa = direct ? SplitArgs{nullptr, jobs[0].a, jobs[0].b, jobs[0].c, desc_ptr(s->st, lg, 0),
                                               (uint32_t)jobs[0].nmoduli, nullptr}
-                                  : SplitArgs{sl.d_res, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
+                                  : SplitArgs{sl.dh_res, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
       e = launch_polymul_split(sa, lg, nres, split_k, sl.stream);
     } else {
       const unsigned grid = (unsigned)std::min<size_t>(s->cus, items.size());
-      e = launch_batch(sl.d_items, segs, sl.d_res, sl.d_q, sl.d_scratch, grid, sl.stream);
+      e = launch_batch(sl.dh_items, segs, sl.dh_res, sl.d_q, sl.d_scratch, grid, sl.stream);
     }
   }
   if (e == hipSuccess) e = hipEventRecord(sl.t1, sl.stream);
+  if (e == hipSuccess) e = launch_xfer(sl.d_xout, nout, max_out, sl.stream);
   for (size_t j = 0; j < jobs.size() && e == hipSuccess; ++j) {
+    if (by_kernel[j]) continue;
     const Request &r = *sl.reqs[j];
     e = hipMemcpyAsync(r.c, jobs[j].c, r.words * 8, hipMemcpyDeviceToHost, sl.stream);
   }
@@ -454,12 +489,26 @@ int ya_server_create(ya_server **out, const ya_server_config *cfg_in) {
     if (e == hipSuccess) e = hipEventCreate(&sl.done);
     if (e == hipSuccess) e = hipEventCreate(&sl.t0);
     if (e == hipSuccess) e = hipEventCreate(&sl.t1);
-    if (e == hipSuccess) e = hipMalloc(&sl.d_items, sl.cap_items * sizeof(BatchItem));
-    if (e == hipSuccess) e = hipMalloc(&sl.d_res, sl.cap_res * sizeof(BatchRes));
     if (e == hipSuccess) e = hipMalloc(&sl.d_q, sizeof(BatchQueue));
     if (e == hipSuccess) e = hipMemcpy(sl.d_q, &q0, sizeof q0, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_items, sl.cap_items * sizeof(BatchItem), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_res, sl.cap_res * sizeof(BatchRes), hipHostMallocDefault);
+    const size_t bi = sl.cap_items * sizeof(BatchItem), br = sl.cap_res * sizeof(BatchRes);
+    if (e == hipSuccess &&
+        (pinned_alloc((void **)&sl.h_items, bi) != YA_OK || pinned_alloc((void **)&sl.h_res, br) != YA_OK))
+      e = hipErrorOutOfMemory;
+    if (e == hipSuccess) {
+      sl.dh_items = (const BatchItem *)pinned_device_range(sl.h_items, bi);
+      sl.dh_res = (const BatchRes *)pinned_device_range(sl.h_res, br);
+      if (!sl.dh_items || !sl.dh_res) e = hipErrorInvalidValue;
+    }
+    const size_t xin = 2 * (size_t)cfg.max_batch_jobs * sizeof(XferDesc), xout = xin / 2;
+    if (e == hipSuccess && (pinned_alloc((void **)&sl.h_xin, xin) != YA_OK ||
+                            pinned_alloc((void **)&sl.h_xout, xout) != YA_OK))
+      e = hipErrorOutOfMemory;
+    if (e == hipSuccess) {
+      sl.d_xin = (const XferDesc *)pinned_device_range(sl.h_xin, xin);
+      sl.d_xout = (const XferDesc *)pinned_device_range(sl.h_xout, xout);
+      if (!sl.d_xin || !sl.d_xout) e = hipErrorInvalidValue;
+    }
     if (e != hipSuccess) {
       server_free(s);
       return fail(YA_ERR_ALLOC, "server slot: %s", hipGetErrorString(e));
@@ -605,14 +654,14 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   int rc = YA_OK;
   auto cleanup = [&]() {
     for (auto &sh : shapes)
-      if (sh.ops) (void)hipHostFree(sh.ops);
+      if (sh.ops) (void)pinned_free(sh.ops);
   };
   for (uint32_t k = 0; k < spec->nshapes && rc == YA_OK; ++k) {
     Shape &sh = shapes[k];
     sh.n = spec->degrees[k];
     sh.L = spec->nmoduli[k];
     sh.words = (size_t)sh.n * sh.L;
-    if (hipHostMalloc((void **)&sh.ops, pool * 2 * sh.words * 8, hipHostMallocDefault) != hipSuccess) {
+    if (pinned_alloc((void **)&sh.ops, pool * 2 * sh.words * 8) != YA_OK) {
       rc = fail(YA_ERR_ALLOC, "pinned operand pool");
       break;
     }
@@ -661,7 +710,7 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   }
   ring = std::min(ring, arr.size() + 1);
   uint64_t *results = nullptr;
-  if (hipHostMalloc((void **)&results, ring * max_words * 8, hipHostMallocDefault) != hipSuccess) {
+  if (pinned_alloc((void **)&results, ring * max_words * 8) != YA_OK) {
     cleanup();
     return fail(YA_ERR_ALLOC, "pinned result ring (%zu bytes)", ring * max_words * 8);
   }
@@ -675,10 +724,15 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   std::atomic<uint64_t> mismatches{0}, failures{0};
   uint64_t backpressure = 0;
   // kWaiters threads take results (waiter w: requests w, w + kWaiters, ...),
-  // verify them (a 1 MiB compare per C2 request: one thread alone would limit
-  // the stream near 8 K req/s) and retire their ring entries; the ring
-  // advances over the contiguous prefix of retired requests.
-  constexpr size_t kWaiters = 4;
+  // verify them and retire their ring entries; the ring advances over the
+  // contiguous prefix of retired requests.  Verification reads two 1 MiB
+  // buffers per C2 request: with 4 threads the verifiers fell behind near
+  // 16 K req/s, the ring filled and arrivals were held back and released in
+  // bursts (client_stalls), which the server saw as queueing -- p99 of 10-40
+  // ms that a run without verification did not have (1.2 ms at 16 K req/s).
+  // Half of the host's cores, 4 to 12, keep the checking off the path.
+  const size_t kWaiters =
+      std::max<size_t>(4, std::min<size_t>(12, std::thread::hardware_concurrency() / 2));
   std::vector<std::thread> waiters;
   for (size_t w = 0; w < kWaiters; ++w) {
     waiters.emplace_back([&, w]() {
@@ -759,7 +813,7 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   rep->p99_batch_launch_us = st.p99_batch_launch_us;
   rep->p50_batch_host_us = st.p50_batch_host_us;
   rep->p99_batch_host_us = st.p99_batch_host_us;
-  (void)hipHostFree(results);
+  (void)pinned_free(results);
   cleanup();
   return YA_OK;
 }
