@@ -128,6 +128,18 @@ def test_polymul(logn):
     np.testing.assert_array_equal(polymul(a, b, n, L), orc.polymul(a, b, n, L))
 
 
+@pytest.mark.parametrize("logn,L", [(10, 12), (12, 12), (14, 12), (15, 12), (13, 9)])
+def test_polymul_many_moduli(logn, L):
+    """The plain path with up to 12 moduli (the uneven batch's range) against
+    the oracle, residue by residue."""
+    n = 1 << logn
+    rng = np.random.default_rng(700 + logn + L)
+    a = rand_bound(rng, n, L, 1, P)
+    b = rand_bound(rng, n, L, 1, P)
+    say("polymul many moduli", logn, L)
+    np.testing.assert_array_equal(polymul(a, b, n, L), orc.polymul(a, b, n, L))
+
+
 def test_polymul_c2_shape_uniform_times_gaussian():
     n, L = 1 << 14, 8
     a = capi.sample_uniform(n, L, 2026)
