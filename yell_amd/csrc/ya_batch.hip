@@ -32,7 +32,12 @@ template <int LB, int LE>
 __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes *__restrict__ res,
                                             uint64_t *lds, uint32_t t) {
   constexpr uint32_t TS = 1u << (LB - LE);  // threads per residue
-  const uint32_t sub = t / TS, tl = t % TS;
+  // a sub-group of whole waves has a wave-uniform index: take it from lane
+  // 0's thread index (a scalar), so the LDS slice address is scalar too
+  const uint32_t sub = TS >= (uint32_t)kBatchThreads ? 0u
+                       : TS >= 64u        ? __builtin_amdgcn_readfirstlane(t) / TS
+                                          : t / TS;
+  const uint32_t tl = t % TS;
   const bool active = sub < it.count;
   BatchRes r;
   ModDesc d;
@@ -54,12 +59,12 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
     constexpr int W0 = FirstWindow<LB, LE>::value;
     gmem_load<W0, LE>(r.a, x, tl);
     gmem_load<W0, LE>(r.b, y, tl);
-    fwd_block_dual<LB, LE>(x, y, my_lds, d.fwd, LB, 0u, d.p, tl);
+    fwd_block_dual<LB, LE>(x, y, my_lds, (const TwC *)d.fwd, LB, 0u, d.p, tl);
   } else {
-    fwd_block<LB, LE>(x, r.a, my_lds, d.fwd, LB, 0u, d.p, tl);
+    fwd_block<LB, LE>(x, r.a, my_lds, (const TwC *)d.fwd, LB, 0u, d.p, tl);
     if (active) gmem_store<0, LE>(r.c, x, tl);
     const uint32_t t2 = phase_v(tl);
-    fwd_block<LB, LE>(x, r.b, my_lds, d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+    fwd_block<LB, LE>(x, r.b, my_lds, (const TwC *)d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
     if (active) {
       gmem_load<0, LE>(r.c, y, t2);
     } else {  // no residue: any defined words will do, nothing is stored
@@ -70,7 +75,7 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(tl);
-  inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, d.inv, LB, 0u, d, t3);
+  inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, (const TwC *)d.inv, LB, 0u, d, t3);
 }
 
 // Bit 14 of one operand of a 2^15 poly-multiplication in registers: thread t

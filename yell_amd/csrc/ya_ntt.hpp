@@ -79,6 +79,39 @@ __device__ __forceinline__ Tw32 load_tw(const Tw32 *p) {
   return Tw32{v.x, v.y};
 }
 
+// A twiddle table that the persistent batch kernel reads: its wave-uniform
+// loads go through the constant address space (scalar loads).  In that kernel
+// the compiler cannot prove on its own that the table is not written before
+// a load (the previous work item's stores to c come first in the loop), so
+// through the global address space every lane would load the same pair on
+// the vector memory path.  The fused and block kernels keep the plain Tw
+// tables: there the compiler already picks scalar loads where they pay, and
+// more of them measured slower (they share the lgkm counter with the LDS
+// exchanges).
+struct TwC : Tw {};
+template <int WLO>
+__device__ __forceinline__ Tw load_tw_w(const Tw *p) {
+  return load_tw(p);
+}
+template <int WLO>
+__device__ __forceinline__ Tw32 load_tw_w(const Tw32 *p) {
+  return load_tw(p);
+}
+template <int WLO>
+__device__ __forceinline__ TwC load_tw_w(const TwC *p) {
+  TwC r;
+  if constexpr (WLO >= 6) {
+    const u64x2 v = *(const __attribute__((address_space(4))) u64x2 *)p;
+    r.w = v.x;
+    r.ws = v.y;
+  } else {
+    const u64x2 v = *(const __attribute__((address_space(1))) u64x2 *)p;
+    r.w = v.x;
+    r.ws = v.y;
+  }
+  return r;
+}
+
 // Twiddle base for a pass whose window starts at WLO: the bits of gbase are
 // thread bits >= WLO, so for WLO >= 6 they are equal across a 64-lane wave and
 // the value is made explicitly wave-uniform (scalar twiddle loads).
@@ -199,7 +232,7 @@ __device__ __forceinline__ void fwd_layers(W (&x)[1 << LE], const TW *__restrict
     const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
 #pragma unroll
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
-      const TW t = load_tw(tw + toff + g);
+      const TW t = load_tw_w<WLO>(tw + toff + g);
 #pragma unroll
       for (int q = 0; q < (1 << j); ++q) {
         const int e0 = (g << (j + 1)) | q;
@@ -222,7 +255,7 @@ __device__ __forceinline__ void fwd_layers2(W (&x)[1 << LE], W (&y)[1 << LE], co
     const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
 #pragma unroll
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
-      const TW t = load_tw(tw + toff + g);
+      const TW t = load_tw_w<WLO>(tw + toff + g);
 #pragma unroll
       for (int q = 0; q < (1 << j); ++q) {
         const int e0 = (g << (j + 1)) | q;
@@ -244,7 +277,7 @@ __device__ __forceinline__ void inv_layers(W (&x)[1 << LE], const TW *__restrict
     const uint32_t toff = ((1u << nlog) >> (b + 1)) + (uniform_if_wave<WLO>(gbase) >> (b + 1));
 #pragma unroll
     for (int g = 0; g < (1 << (LE - 1 - j)); ++g) {
-      const TW t = load_tw(tw + toff + g);
+      const TW t = load_tw_w<WLO>(tw + toff + g);
 #pragma unroll
       for (int q = 0; q < (1 << j); ++q) {
         const int e0 = (g << (j + 1)) | q;
