@@ -28,7 +28,7 @@ __device__ __forceinline__ uint64_t mul32x32(uint32_t a, uint32_t b) {
   return (uint64_t)a * (uint64_t)b;
 }
 
-// ---- Three short assembly sequences.  The compiler neither exposes the
+// ---- Short assembly sequences.  The compiler neither exposes the
 // carry-out of v_mad_u64_u32 nor chains a product into a full 64-bit addend
 // (it zero-extends 32-bit halves into fresh register pairs, one v_mov each),
 // and it rewrites x >= m ? x - m : x into a compare, two masked selects of m
@@ -107,14 +107,60 @@ __device__ __forceinline__ void mul128(uint64_t a, uint64_t b, uint64_t &hi, uin
   lo = with_hi(p00, lo32(m2));
 }
 
-// x*w - q*p (mod 2^64) as x*w + q*np with np = 2^64 - p: the two low
-// products in one multiply-add chain, the four cross products into the high
-// word.
+// a * b + c with only the low 32 bits of the result wanted: one
+// v_mad_u64_u32 whose addend is the previous step's register pair (its high
+// half is don't-care), so a sum of 32-bit products is a chain of multiply-adds
+// with no separate add.
+__device__ __forceinline__ uint64_t mad_lo_acc(uint32_t a, uint32_t b, uint64_t c) {
+  uint64_t d, mask;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(mask) : "v"(a), "s"(b), "v"(c));
+  return d;
+}
+
+// a * b + c (mod 2^64) into a register pair of its own (c stays live).
+__device__ __forceinline__ uint64_t mad64_to(uint32_t a, uint32_t b, uint64_t c) {
+  uint64_t d, mask;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(mask) : "v"(a), "s"(b), "v"(c));
+  return d;
+}
+
+// 2*x + m (mod 2^64) as one v_lshl_add_u64 (the optimizer otherwise forms
+// (x + m/2) << 1 in two instructions).
+__device__ __forceinline__ uint64_t dbl_add(uint64_t x, uint64_t m) {
+  uint64_t d;
+  asm("v_lshl_add_u64 %0, %1, 1, %2" : "=v"(d) : "v"(x), "s"(m));
+  return d;
+}
+
+// acc + (x*w - q*p) (mod 2^64) with np = 2^64 - p: the cross products
+// x0*w1 + x1*w0 + q0*n1 + q1*n0 (they only reach the high word) summed by a
+// multiply-add chain, the low products acc + x0*w0 + q0*n0 by another, the
+// cross sum added to the high word in between.  Seven VALU instructions where
+// the low product, four 32-bit multiplies, two three-way adds and the add of
+// acc took nine.
+__device__ __forceinline__ uint64_t add_mul_sub_lo64(uint64_t acc, uint64_t x, uint64_t w, uint64_t q,
+                                                     uint64_t np) {
+  const uint32_t x0 = lo32(x), x1 = hi32(x), w0 = lo32(w), w1 = hi32(w);
+  const uint32_t q0 = lo32(q), q1 = hi32(q), n0 = lo32(np), n1 = hi32(np);
+  uint64_t c = mul32x32(x0, w1);
+  c = mad_lo_acc(x1, w0, c);
+  c = mad_lo_acc(q0, n1, c);
+  c = mad_lo_acc(q1, n0, c);
+  const uint64_t a = mad64_to(x0, w0, acc);
+  return mad64_wrap(q0, n0, with_hi(a, hi32(a) + lo32(c)));
+}
+
+// x*w - q*p (mod 2^64), the same chains without an addend (the low product
+// starts the second).
 __device__ __forceinline__ uint64_t mul_sub_lo64(uint64_t x, uint64_t w, uint64_t q, uint64_t np) {
   const uint32_t x0 = lo32(x), x1 = hi32(x), w0 = lo32(w), w1 = hi32(w);
   const uint32_t q0 = lo32(q), q1 = hi32(q), n0 = lo32(np), n1 = hi32(np);
-  const uint64_t r = mad64_wrap(q0, n0, mul32x32(x0, w0));
-  return with_hi(r, hi32(r) + x0 * w1 + x1 * w0 + q0 * n1 + q1 * n0);
+  uint64_t c = mul32x32(x0, w1);
+  c = mad_lo_acc(x1, w0, c);
+  c = mad_lo_acc(q0, n1, c);
+  c = mad_lo_acc(q1, n0, c);
+  const uint64_t a = mul32x32(x0, w0);
+  return mad64_wrap(q0, n0, with_hi(a, hi32(a) + lo32(c)));
 }
 
 // acc + q*np (mod 2^64), acc the multiply-add's 64-bit addend.
@@ -179,12 +225,14 @@ __device__ __forceinline__ uint64_t mulmod_shoup_lazy(uint64_t x, const Tw &t, c
   return mul_sub_lo64(x, t.w, mulhi64(x, t.ws), k.np);
 }
 
-// lib/ntt.cpp:69-83
+// lib/ntt.cpp:69-83: x0 = u0 + v, x1 = u0 + 2p - v with v the lazy Shoup
+// product.  u0 + v comes out of the product's multiply-add chain (u0 as its
+// addend), and x1 = 2*u0 + 2p - x0 (mod 2^64), the same word.
 __device__ __forceinline__ void ct_butterfly(uint64_t &x0, uint64_t &x1, const Tw &t, const PrimeK &k) {
   const uint64_t u0 = sub_if_ge(x0, k.p2, k.np2);
-  const uint64_t v = mulmod_shoup_lazy(x1, t, k);
-  x0 = u0 + v;
-  x1 = u0 + k.p2 - v;
+  const uint64_t s = add_mul_sub_lo64(u0, x1, t.w, mulhi64(x1, t.ws), k.np);
+  x0 = s;
+  x1 = dbl_add(u0, k.p2) - s;
 }
 
 // lib/ntt.cpp:47-63

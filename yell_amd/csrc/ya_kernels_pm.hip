@@ -56,7 +56,9 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
   const uint32_t v = xcd_order(blockIdx.x, g);
   const uint32_t count = g / nmod;  // polynomials in the launch
   const uint32_t cm = v / count, poly = v % count;
-  const uint32_t res = poly * nmod + cm;
+  // the residue index is wave-uniform: keep it (and the offsets derived from
+  // it) in scalar registers, not in a vector pair live across the kernel
+  const uint32_t res = __builtin_amdgcn_readfirstlane(poly * nmod + cm);
   const ModDesc d = descs[cm];
   const size_t off = (size_t)res << LB;
   polymul_residue<LB, LE, DUAL>(uniform_ptr(c + off), uniform_ptr(a + off), uniform_ptr(b + off), d, lds,
