@@ -211,10 +211,12 @@ def c3_bench(stream, ev, steps, seed=2026):
 def c4_bench(stream, ev, steps):
     """N = 2^16, L = 24: forward NTT of 16 operands, sum of 8 pair products in
     the NTT domain, one inverse NTT (17 * N * L * 8 algorithmic bytes).  The
-    job is ya_poly_add_products_backward (the pair-product pass and the summing
-    inverse, both block sizes of the pair pass timed, the faster kept), and,
-    for comparison, the reference's sequence as separate launches (forward of
-    the 16 operands, the MAC, the inverse).  Each step starts from acc = 0."""
+    job is ya_poly_sum_products_backward (a fresh accumulator: the pair-product
+    pass and the summing inverse, which writes the result without reading it;
+    both block sizes of the pair pass timed, the faster kept); beside it
+    ya_poly_add_products_backward after a zero fill of acc (the same words,
+    checked) and the reference's sequence as separate launches (forward of the
+    16 operands, the MAC, the inverse)."""
     n, L, pairs = 1 << 16, 24, 8
     w = n * L
     ops = np.concatenate([capi.sample_uniform(n, L, 4000 + k) for k in range(2 * pairs)])
@@ -228,6 +230,9 @@ def c4_bench(stream, ev, steps):
         capi.call("ya_memset_zero", dacc.ptr, w * 8, stream)
         capi.call("ya_poly_add_products_backward", dacc.ptr, pa, pb, pairs, n, L, stream)
 
+    def fresh():
+        capi.call("ya_poly_sum_products_backward", dacc.ptr, pa, pb, pairs, n, L, stream)
+
     def separate():
         capi.call("ya_poly_forward_many", dall.ptr, n, L, 2 * pairs, stream)
         capi.call("ya_event_record", e2.a, stream)
@@ -240,12 +245,25 @@ def c4_bench(stream, ev, steps):
     by_lb = {}
     for lb in (14, 13):
         capi.call("ya_tune_set", 4, lg, lb)
-        fused()
+        fresh()
         capi.call("ya_stream_synchronize", stream)
-        by_lb[lb] = timed(ev, stream, fused, steps)
+        by_lb[lb] = timed(ev, stream, fresh, steps)
     best = min(by_lb, key=by_lb.get)
     capi.call("ya_tune_set", 4, lg, best)
     ms = by_lb[best]
+    fused()
+    capi.call("ya_stream_synchronize", stream)
+    ms_acc = timed(ev, stream, fused, steps)
+    # the two forms give the same words (each call uses a_j, b_j as work
+    # space, so the operands are uploaded again before each)
+    dall.upload(ops)
+    fresh()
+    capi.call("ya_stream_synchronize", stream)
+    got_fresh = dacc.download()
+    dall.upload(ops)
+    fused()
+    capi.call("ya_stream_synchronize", stream)
+    same = bool(np.array_equal(got_fresh, dacc.download()))
     separate()
     capi.call("ya_stream_synchronize", stream)
     ms_sep = timed(ev, stream, separate, steps)
@@ -270,8 +288,10 @@ def c4_bench(stream, ev, steps):
     return {"degree": n, "nmoduli": L, "pairs": pairs, "ms_per_job": ms, "large_ring_polymul": big,
             "jobs_per_s": 1e3 / ms, "algorithmic_bytes": alg,
             "hbm_GBps_algorithmic": alg / (ms / 1e3) / 1e9,
-            "entry_point": "ya_poly_add_products_backward",
+            "entry_point": "ya_poly_sum_products_backward",
             "ms_by_pair_block_log": {str(k): v for k, v in by_lb.items()}, "pair_block_log": best,
+            "accumulate_form": {"entry_point": "ya_poly_add_products_backward (after a zero fill of acc)",
+                                "ms_per_job": ms_acc, "same_words_as_fresh": same},
             "separate_launches": {"ms_per_job": ms_sep, "mac_ms": ms_mac,
                                   "mac_GBps": (2 * pairs + 2) * w * 8 / (ms_mac / 1e3) / 1e9}}
 

@@ -197,6 +197,14 @@ int ya_poly_add_products(uint64_t *acc, const uint64_t *const *a, const uint64_t
  * sums the products into acc).  The 2 * npairs + 1 buffers must not overlap. */
 int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *const *b, size_t npairs,
                                   size_t degree, size_t nmoduli, void *stream);
+/* The same with a fresh accumulator: out = backward(sum_j forward(a_j) *
+ * forward(b_j)), npairs >= 1 -- the reference's sequence on a new (zeroed)
+ * poly<N> acc(L) (details/poly.hpp:31-41 zero-fills it; details/poly.hpp:119-132,
+ * 171-186), the same words as ya_poly_add_products_backward on a zeroed acc,
+ * without the zero fill and without reading out.  a and b as above; out must
+ * not overlap them. */
+int ya_poly_sum_products_backward(uint64_t *out, uint64_t *const *a, uint64_t *const *b, size_t npairs,
+                                  size_t degree, size_t nmoduli, void *stream);
 /* The reference example's encryption step (example.cpp:41-44 and 46-48:
  * ct += msg; ct.forward_lazy(); ct.add_product_of(u, pk)) over `count`
  * polynomials in one pass: acc_k = forward_lazy(acc_k + add_k) + u_k * key,

@@ -212,3 +212,36 @@ def test_add_products_backward_c4_and_rejects_overlap():
     assert lib.ya_poly_add_products_backward(dacc.ptr, pa, pb, pairs, n, L, None) == 1
     pb[3] = dacc.ptr
     assert lib.ya_poly_add_products_backward(dacc.ptr, pa, pb, pairs, n, L, None) == 1
+
+
+@pytest.mark.parametrize("logn,L,npairs,lb", [(12, 2, 3, 14), (13, 3, 1, 14), (14, 2, 17, 14), (16, 5, 4, 13),
+                                              (16, 24, 8, 13), (16, 3, 33, 13), (19, 1, 2, 14)])
+def test_sum_products_backward(logn, L, npairs, lb):
+    """ya_poly_sum_products_backward = the reference's sequence on a fresh
+    (zero-filled) accumulator: below 2^13 the first pair's product written by
+    the pointwise multiply, then the MAC; from 2^13 the summing inverse that
+    never reads the output (which holds garbage words beforehand), 17 and 33
+    pairs in two and three chunks (the first chunk's first product copied)."""
+    lib = device()
+    n = 1 << logn
+    rng = np.random.default_rng(logn * 1000 + npairs * 10 + L)
+    a = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+    b = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+    a[0][:3] = [P[0] - 1, 0, P[0] - 1]
+    b[0][:3] = [P[0] - 1, P[0] - 1, 0]
+    old = lib.ya_tune_get(4, logn)
+    capi.call("ya_tune_set", 4, logn, lb)
+    try:
+        da = [capi.DeviceBuffer.from_host(v) for v in a]
+        db = [capi.DeviceBuffer.from_host(v) for v in b]
+        dout = capi.DeviceBuffer.from_host(rand_bound(rng, n, L, 1, P) | np.uint64(1 << 63))
+        pa = (ctypes.c_void_p * npairs)(*[x.ptr for x in da])
+        pb = (ctypes.c_void_p * npairs)(*[x.ptr for x in db])
+        say("sum_products_backward", logn, L, npairs, lb)
+        capi.call("ya_poly_sum_products_backward", dout.ptr, pa, pb, npairs, n, L, None)
+        np.testing.assert_array_equal(dout.download(), _mac_want(np.zeros(n * L, dtype=np.uint64), a, b, n, L))
+        assert lib.ya_poly_sum_products_backward(dout.ptr, pa, pb, 0, n, L, None) == 1
+        pb[0] = dout.ptr
+        assert lib.ya_poly_sum_products_backward(dout.ptr, pa, pb, npairs, n, L, None) == 1
+    finally:
+        capi.call("ya_tune_set", 4, logn, old if old in (13, 14) else 14)

@@ -73,6 +73,7 @@ _SIGS = {
     "ya_poly_backward_many": (i32, [vp, sz, sz, sz, vp]),
     "ya_poly_add_products": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_poly_add_products_backward": (i32, [vp, vp, vp, sz, sz, sz, vp]),
+    "ya_poly_sum_products_backward": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_poly_multiply_many": (i32, [vp, vp, vp, sz, sz, sz, vp]),
     "ya_poly_multiply_noalloc": (i32, [vp, vp, vp, sz, sz, vp]),
     "ya_sample_uniform": (i32, [vp, sz, sz, u64]),

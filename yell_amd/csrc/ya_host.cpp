@@ -711,13 +711,16 @@ int ya_poly_add_products(uint64_t *acc, const uint64_t *const *a, const uint64_t
   return YA_OK;
 }
 
-// acc = backward(acc + sum_j forward(a_j) * forward(b_j)); a_j, b_j are work space.
-int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *const *b, size_t npairs,
-                                  size_t degree, size_t nmoduli, void *stream) {
+// acc = backward(acc + sum_j forward(a_j) * forward(b_j)) (accumulate), or
+// backward(sum_j forward(a_j) * forward(b_j)) without reading acc; a_j, b_j
+// are work space.
+static int products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *const *b, size_t npairs,
+                             size_t degree, size_t nmoduli, bool accumulate, void *stream) {
   int logn;
   int rc = check_shape(degree, 0, nmoduli, &logn);
   if (rc) return rc;
   if (!acc || (npairs && (!a || !b))) return fail(YA_ERR_INVALID, "null buffer");
+  if (!accumulate && npairs == 0) return fail(YA_ERR_INVALID, "no pairs to sum");
   const size_t pw = degree * nmoduli;
   for (size_t j = 0; j < npairs; ++j) {
     if (!a[j] || !b[j]) return fail(YA_ERR_INVALID, "null operand %zu", j);
@@ -735,13 +738,19 @@ int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *c
   const hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSuccess;
   if (logn < kMinPairLog || nmoduli > 65535) {
-    // small rings: the reference's sequence as separate launches
+    // small rings: the reference's sequence as separate launches (without an
+    // accumulator to read, the first pair's product is written, not added)
     const NttShape sh = tuned_shape(logn, 0);
     for (size_t j = 0; e == hipSuccess && j < npairs; ++j) {
       e = launch_forward(a[j], d, logn, nmoduli, nmoduli, false, sh, s);
       if (e == hipSuccess) e = launch_forward(b[j], d, logn, nmoduli, nmoduli, false, sh, s);
     }
-    for (size_t j0 = 0; e == hipSuccess && j0 < npairs; j0 += kMacPairs) {
+    size_t first = 0;
+    if (e == hipSuccess && !accumulate) {
+      e = launch_pointwise(PW_MUL, acc, a[0], b[0], d, degree, nmoduli, s);
+      first = 1;
+    }
+    for (size_t j0 = first; e == hipSuccess && j0 < npairs; j0 += kMacPairs) {
       MacPairs mp{};
       const int k = (int)std::min<size_t>(kMacPairs, npairs - j0);
       for (int j = 0; j < k; ++j) {
@@ -759,11 +768,13 @@ int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *c
       }
     }
     if (e == hipSuccess) e = launch_backward(acc, d, logn, nmoduli, nmoduli, sh, s);
-    return e == hipSuccess ? YA_OK : hip_fail(e, "add_products_backward (separate launches)");
+    return e == hipSuccess ? YA_OK : hip_fail(e, "products_backward (separate launches)");
   }
   // chunks of kMacPairs pairs: each chunk's products go to its a_j; all but
-  // the last chunk are added into acc, the last one is summed by the inverse
+  // the last chunk are added into acc (copied, for the first product when acc
+  // is not read), the last one is summed by the inverse
   size_t j0 = 0;
+  bool acc_live = accumulate;  // does acc hold words to add to?
   do {
     const int k = (int)std::min<size_t>(kMacPairs, npairs - j0);
     PairPtrs pp{};
@@ -775,13 +786,29 @@ int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *c
     j0 += k;
     if (e != hipSuccess) break;
     if (j0 < npairs) {
-      for (int j = 0; e == hipSuccess && j < k; ++j)
-        e = launch_pointwise(PW_ADD, acc, acc, pp.a[j], d, degree, nmoduli, s);
+      for (int j = 0; e == hipSuccess && j < k; ++j) {
+        if (!acc_live) {
+          e = hipMemcpyAsync(acc, pp.a[j], pw * 8, hipMemcpyDeviceToDevice, s);
+          acc_live = true;
+        } else {
+          e = launch_pointwise(PW_ADD, acc, acc, pp.a[j], d, degree, nmoduli, s);
+        }
+      }
     } else {
-      e = launch_sum_backward(acc, pp, k, d, logn, nmoduli, s);
+      e = launch_sum_backward(acc, pp, k, d, logn, nmoduli, acc_live, s);
     }
   } while (e == hipSuccess && j0 < npairs);
-  return e == hipSuccess ? YA_OK : hip_fail(e, "add_products_backward");
+  return e == hipSuccess ? YA_OK : hip_fail(e, "products_backward");
+}
+
+int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *const *b, size_t npairs,
+                                  size_t degree, size_t nmoduli, void *stream) {
+  return products_backward(acc, a, b, npairs, degree, nmoduli, true, stream);
+}
+
+int ya_poly_sum_products_backward(uint64_t *out, uint64_t *const *a, uint64_t *const *b, size_t npairs,
+                                  size_t degree, size_t nmoduli, void *stream) {
+  return products_backward(out, a, b, npairs, degree, nmoduli, false, stream);
 }
 
 static int seam_host(uint64_t *x, size_t degree, const uint64_t *w, const uint64_t *ws, uint64_t p,

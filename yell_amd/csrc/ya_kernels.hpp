@@ -89,7 +89,8 @@ hipError_t launch_mac(uint64_t *acc, const MacPairs &pairs, int npairs, const Mo
 // twiddles; the product (canonical, natural order) goes to a_j, and b_j is
 // work space (above 2^kMaxBlockLog the top layers of a_j and b_j run first,
 // in place).  launch_sum_backward: acc + sum_j a_j (addmod: the same residues
-// as the muladd chain on canonical words), then the inverse.  Degrees
+// as the muladd chain on canonical words), then the inverse; with accumulate
+// false acc is only written (sum_j a_j, npairs >= 1).  Degrees
 // 2^kMinPairLog .. 2^kMaxLogN; at most kMacPairs pairs per launch; the 2 *
 // npairs + 1 buffers must not overlap.
 struct PairPtrs {
@@ -99,7 +100,7 @@ struct PairPtrs {
 hipError_t launch_pair_products(const PairPtrs &pp, int npairs, const ModDesc *descs, int logn, size_t nmod,
                                 int lb, hipStream_t st);
 hipError_t launch_sum_backward(uint64_t *acc, const PairPtrs &pp, int npairs, const ModDesc *descs, int logn,
-                               size_t nmod, hipStream_t st);
+                               size_t nmod, bool accumulate, hipStream_t st);
 
 // Copies between pinned host memory and device memory by a kernel (the
 // served form's operand upload and result download: one launch for a whole

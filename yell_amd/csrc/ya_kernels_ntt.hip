@@ -222,16 +222,21 @@ static hipError_t launch_ntt4(uint64_t *data, const ModDesc *descs, size_t nres,
 
 // Column kernels for N = 2^(B+K): one thread per column c in [0, 2^B) holding
 // the 2^K words c + e * 2^B (the layers on index bits >= B; the blocks of
-// 2^B words take the rest).
+// 2^B words take the rest).  Workgroups of 2^TW threads: 256, or 64 for a
+// launch of few columns (C4's inverse: 24 residues of 2^12 columns are 384
+// workgroups of 256 threads, 1.5 per CU, or 1536 of 64, 6 per CU).
 constexpr int kTopThreads = 256;
+constexpr int kTopThreadsLog = 8, kTopThreadsSmallLog = 6;
+// workgroups of 256 threads below which the column kernels use 64
+constexpr size_t kTopSmallBelow = 512;
 
-template <int B, int K>
-__global__ void __launch_bounds__(kTopThreads)
+template <int B, int K, int TW = kTopThreadsLog>
+__global__ void __launch_bounds__(1 << TW)
 k_fwd_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod) {
   constexpr int NLOG = B + K;
-  constexpr uint32_t cols_per_res_log = B - 8;  // 2^B / 256 workgroups per residue
+  constexpr uint32_t cols_per_res_log = B - TW;  // 2^B / 2^TW workgroups per residue
   const uint32_t res = blockIdx.x >> cols_per_res_log;
-  const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << 8) | threadIdx.x;
+  const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << TW) | threadIdx.x;
   const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << NLOG);
   uint64_t x[1 << K];
@@ -240,13 +245,13 @@ k_fwd_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32
   gmem_store<B, K>(base, x, t);
 }
 
-template <int B, int K, bool MERGED>
-__global__ void __launch_bounds__(kTopThreads)
+template <int B, int K, bool MERGED, int TW = kTopThreadsLog>
+__global__ void __launch_bounds__(1 << TW)
 k_inv_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32_t nmod) {
   constexpr int NLOG = B + K;
-  constexpr uint32_t cols_per_res_log = B - 8;
+  constexpr uint32_t cols_per_res_log = B - TW;
   const uint32_t res = blockIdx.x >> cols_per_res_log;
-  const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << 8) | threadIdx.x;
+  const uint32_t t = ((blockIdx.x & ((1u << cols_per_res_log) - 1u)) << TW) | threadIdx.x;
   const ModDesc d = descs[res % nmod];
   uint64_t *base = data + ((size_t)res << NLOG);
   uint64_t x[1 << K];
@@ -267,19 +272,27 @@ static int large_block_log(int logn, size_t nres) {
   return lb;
 }
 
-template <int B>
-static hipError_t fwd_top_b(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
-                            hipStream_t st) {
-  const dim3 grid((unsigned)(nres << (B - 8)));
+template <int B, int TW>
+static hipError_t fwd_top_bt(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
+                             hipStream_t st) {
+  const dim3 grid((unsigned)(nres << (B - TW))), block(1u << TW);
   switch (k) {
-    case 1: hipLaunchKernelGGL((k_fwd_top<B, 1>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 2: hipLaunchKernelGGL((k_fwd_top<B, 2>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 3: hipLaunchKernelGGL((k_fwd_top<B, 3>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 4: hipLaunchKernelGGL((k_fwd_top<B, 4>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 5: hipLaunchKernelGGL((k_fwd_top<B, 5>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 1: hipLaunchKernelGGL((k_fwd_top<B, 1, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 2: hipLaunchKernelGGL((k_fwd_top<B, 2, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 3: hipLaunchKernelGGL((k_fwd_top<B, 3, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 4: hipLaunchKernelGGL((k_fwd_top<B, 4, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 5: hipLaunchKernelGGL((k_fwd_top<B, 5, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int B>
+static hipError_t fwd_top_b(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
+                            hipStream_t st) {
+  if ((nres << (B - kTopThreadsLog)) < kTopSmallBelow)
+    return fwd_top_bt<B, kTopThreadsSmallLog>(data, descs, k, nres, nmod, st);
+  return fwd_top_bt<B, kTopThreadsLog>(data, descs, k, nres, nmod, st);
 }
 
 // The top K = logn - lb forward layers of nres residues (columns of 2^lb).
@@ -294,19 +307,27 @@ static hipError_t launch_fwd_top(uint64_t *data, const ModDesc *descs, int logn,
   }
 }
 
-template <int B, bool MERGED>
-static hipError_t inv_top_b(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
-                            hipStream_t st) {
-  const dim3 grid((unsigned)(nres << (B - 8)));
+template <int B, bool MERGED, int TW>
+static hipError_t inv_top_bt(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
+                             hipStream_t st) {
+  const dim3 grid((unsigned)(nres << (B - TW))), block(1u << TW);
   switch (k) {
-    case 1: hipLaunchKernelGGL((k_inv_top<B, 1, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 2: hipLaunchKernelGGL((k_inv_top<B, 2, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 3: hipLaunchKernelGGL((k_inv_top<B, 3, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 4: hipLaunchKernelGGL((k_inv_top<B, 4, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
-    case 5: hipLaunchKernelGGL((k_inv_top<B, 5, MERGED>), grid, dim3(kTopThreads), 0, st, data, descs, (uint32_t)nmod); break;
+    case 1: hipLaunchKernelGGL((k_inv_top<B, 1, MERGED, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 2: hipLaunchKernelGGL((k_inv_top<B, 2, MERGED, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 3: hipLaunchKernelGGL((k_inv_top<B, 3, MERGED, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 4: hipLaunchKernelGGL((k_inv_top<B, 4, MERGED, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
+    case 5: hipLaunchKernelGGL((k_inv_top<B, 5, MERGED, TW>), grid, block, 0, st, data, descs, (uint32_t)nmod); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int B, bool MERGED>
+static hipError_t inv_top_b(uint64_t *data, const ModDesc *descs, int k, size_t nres, size_t nmod,
+                            hipStream_t st) {
+  if ((nres << (B - kTopThreadsLog)) < kTopSmallBelow)
+    return inv_top_bt<B, MERGED, kTopThreadsSmallLog>(data, descs, k, nres, nmod, st);
+  return inv_top_bt<B, MERGED, kTopThreadsLog>(data, descs, k, nres, nmod, st);
 }
 
 // The top K = logn - lb inverse layers (the last merged with N^-1 unless lazy).
@@ -546,8 +567,9 @@ k_pair_product(PairPtrs pp, const ModDesc *__restrict__ descs, uint32_t npairs, 
 // words t | e << W0), one LDS exchange into the inverse's first layout, the
 // inverse layers of the block (TOP_MERGED when the block is the residue).
 // acc is read before the first barrier and written after the last, by this
-// workgroup only.
-template <int LB, int LE, int TOP>
+// workgroup only.  ACC false: acc is not read (the sum starts at a_0, the
+// words of 0 + a_0 for a canonical a_0) and only written.
+template <int LB, int LE, int TOP, bool ACC>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_sum_inv(uint64_t *__restrict__ acc, PairPtrs pp, uint32_t npairs, const ModDesc *__restrict__ descs,
           uint32_t nlog) {
@@ -561,13 +583,20 @@ k_sum_inv(uint64_t *__restrict__ acc, PairPtrs pp, uint32_t npairs, const ModDes
   constexpr int W0 = FirstWindow<LB, LE>::value;
   const uint32_t b0 = win_base<W0, LE>(t);
   uint64_t x[1 << LE];
-  gw *ar = (gw *)uniform_ptr(acc + off);
+  uint32_t j = 0;
+  if constexpr (ACC) {
+    gw *ar = (gw *)uniform_ptr(acc + off);
 #pragma unroll
-  for (int e = 0; e < (1 << LE); ++e) x[e] = ar[b0 | ((uint32_t)e << W0)];
+    for (int e = 0; e < (1 << LE); ++e) x[e] = ar[b0 | ((uint32_t)e << W0)];
+  } else {
+    gw *a0 = (gw *)uniform_ptr(pp.a[0] + off);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) x[e] = a0[b0 | ((uint32_t)e << W0)];
+    j = 1;
+  }
   // U products' loads in flight at a time (4 with 256-thread workgroups, 2
   // within the 128 registers of the larger ones), then one at a time
   constexpr int U = LB <= 12 ? 4 : 2;
-  uint32_t j = 0;
   for (; j + U <= npairs; j += U) {
     uint64_t v[U][1 << LE];
 #pragma unroll
@@ -634,34 +663,38 @@ hipError_t launch_pair_products(const PairPtrs &pp, int npairs, const ModDesc *d
 
 template <int LB, int TOP, int LE = 4>
 static hipError_t sum_inv_lb(uint64_t *acc, const PairPtrs &pp, int npairs, const ModDesc *descs, int logn,
-                             size_t nmod, hipStream_t st) {
+                             size_t nmod, bool accumulate, hipStream_t st) {
   const dim3 grid((unsigned)(nmod << (logn - LB)));
-  hipLaunchKernelGGL((k_sum_inv<LB, LE, TOP>), grid, dim3(1u << (LB - LE)), 0, st, acc, pp, (uint32_t)npairs,
-                     descs, (uint32_t)logn);
+  if (accumulate)
+    hipLaunchKernelGGL((k_sum_inv<LB, LE, TOP, true>), grid, dim3(1u << (LB - LE)), 0, st, acc, pp,
+                       (uint32_t)npairs, descs, (uint32_t)logn);
+  else
+    hipLaunchKernelGGL((k_sum_inv<LB, LE, TOP, false>), grid, dim3(1u << (LB - LE)), 0, st, acc, pp,
+                       (uint32_t)npairs, descs, (uint32_t)logn);
   return hipGetLastError();
 }
 
 hipError_t launch_sum_backward(uint64_t *acc, const PairPtrs &pp, int npairs, const ModDesc *descs, int logn,
-                               size_t nmod, hipStream_t st) {
+                               size_t nmod, bool accumulate, hipStream_t st) {
   if (logn < kMinPairLog || logn > kMaxLogN || npairs < 0 || npairs > kMacPairs || nmod == 0 ||
-      (nmod << (logn - 12)) > 0x7fffffffu)
+      (nmod << (logn - 12)) > 0x7fffffffu || (!accumulate && npairs < 1))
     return hipErrorInvalidValue;
   // blocks of 2^14, or 2^13 / 2^12 to give a launch of few residues more
   // workgroups (the inverse's column pass then takes the rest)
   const int lb = logn <= kMaxBlockLog ? logn : large_block_log(logn, nmod);
   hipError_t e;
   if (logn == lb) {
-    e = lb == 13 ? sum_inv_lb<13, TOP_MERGED>(acc, pp, npairs, descs, logn, nmod, st)
-                 : sum_inv_lb<14, TOP_MERGED>(acc, pp, npairs, descs, logn, nmod, st);
+    e = lb == 13 ? sum_inv_lb<13, TOP_MERGED>(acc, pp, npairs, descs, logn, nmod, accumulate, st)
+                 : sum_inv_lb<14, TOP_MERGED>(acc, pp, npairs, descs, logn, nmod, accumulate, st);
     return e;
   }
   // radix-8 passes below 2^14: twice the waves of radix-16 ones, for a pass
   // that is short of waves (the sum's loads and the inverse's dependency
   // chains need them)
   switch (lb) {
-    case 12: e = sum_inv_lb<12, TOP_GS, 3>(acc, pp, npairs, descs, logn, nmod, st); break;
-    case 13: e = sum_inv_lb<13, TOP_GS, 3>(acc, pp, npairs, descs, logn, nmod, st); break;
-    default: e = sum_inv_lb<14, TOP_GS>(acc, pp, npairs, descs, logn, nmod, st); break;
+    case 12: e = sum_inv_lb<12, TOP_GS, 3>(acc, pp, npairs, descs, logn, nmod, accumulate, st); break;
+    case 13: e = sum_inv_lb<13, TOP_GS, 3>(acc, pp, npairs, descs, logn, nmod, accumulate, st); break;
+    default: e = sum_inv_lb<14, TOP_GS>(acc, pp, npairs, descs, logn, nmod, accumulate, st); break;
   }
   if (e != hipSuccess) return e;
   return launch_inv_top(acc, descs, logn, lb, nmod, nmod, false, st);
