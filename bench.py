@@ -662,13 +662,16 @@ def main():
 
     if want("tune"):
         log("autotune 2^10..2^15")
-        rep = (ctypes.c_double * 36)()
+        S = 7  # YA_TUNE_REPORT_STRIDE
+        rep = (ctypes.c_double * (6 * S))()
         capi.call("ya_autotune", 10, 15, rep)
+        lib_ = capi.load()
         extra["autotune"] = {"classes": list(range(10, 16)),
-                             "ms_batched_r8_r16_fused_r8_r16_r32": [
-                                 [round(rep[6 * k + j], 4) for j in (0, 1, 3, 4, 5)] for k in range(6)],
-                             "chosen_batched_le": [capi.load().ya_tune_get(2, lg) for lg in range(10, 16)],
-                             "chosen_fused_le": [capi.load().ya_tune_get(1, lg) for lg in range(10, 16)]}
+                             "ms_batched_r8_r16_fused_r8_r16_r32_fused_two_sequences_r8_r16": [
+                                 [round(rep[S * k + j], 4) for j in range(S)] for k in range(6)],
+                             "chosen_batched_le": [lib_.ya_tune_get(2, lg) for lg in range(10, 16)],
+                             "chosen_fused_le": [lib_.ya_tune_get(1, lg) for lg in range(10, 16)],
+                             "chosen_fused_dual": [lib_.ya_tune_get(3, lg) for lg in range(10, 16)]}
     if want("probe"):
         probes = {}
         for kind, name in ((0, "mad_u64_u32_per_s"), (1, "mul_lo_u32_per_s"), (2, "ct_butterflies_per_s"),

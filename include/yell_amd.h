@@ -300,13 +300,20 @@ int ya_batch_plan_destroy(ya_batch_plan *plan);
 int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
 
 /* Autotuner (no counterpart in the reference).  For every degree class
- * 2^lo .. 2^hi (within 2^7 .. 2^15) times the instantiated pass shapes
- * (radix-8 or radix-16 register passes; the radix also fixes how many
- * residues share a 1024-thread workgroup tile) for the batched kernel and
- * the fused single-launch kernel on the device, and keeps the faster shape
- * for later plans / launches.  report (optional) receives 4 doubles per
- * class: batched ms at radix 8, radix 16, fused ms at radix 8, radix 16
- * (-1 where not applicable).  Results are identical for every shape. */
+ * 2^lo .. 2^hi (within 2^7 .. 2^15) times, on the device, the instantiated
+ * shapes of the batched (persistent) kernel -- radix-8 or radix-16 register
+ * passes, i.e. 3 or 4 NTT layers fused per pass; the radix also fixes the
+ * tile: how many residues share a 1024-thread workgroup -- and of the fused
+ * single-launch kernel -- radix 8, 16 or 32 (one residue per workgroup of
+ * 2^(logn - log2 radix) threads), and from 2^10 at radix 8 / 16 both forward
+ * transforms as one pass sequence with shared twiddles (dual) or as two --
+ * and keeps the fastest of each for later plans / launches (table kinds 2,
+ * 1 and 3; kind 0 follows kind 1).  report (optional) receives
+ * YA_TUNE_REPORT_STRIDE doubles per class: batched ms at radix 8, 16; fused
+ * ms at radix 8, 16 (dual where built), 32; fused ms at radix 8, 16 as two
+ * sequences (-1 where not applicable).  Results are identical for every
+ * shape. */
+enum { YA_TUNE_REPORT_STRIDE = 7 };
 int ya_autotune(int logn_lo, int logn_hi, double *report);
 /* Read / set one entry of the shape table: kind 1 = fused kernel, 2 =
  * batched kernel, 0 = standalone NTT kernels; le = log2 of the register-pass

@@ -175,11 +175,20 @@ def test_every_pass_shape_gives_the_same_words():
 
 
 def test_autotune_runs_and_fills_table():
+    """Every timed shape of 2^12 and 2^13 reports a time: batched radix 8 / 16,
+    fused radix 8 / 16 / 32 and the fused two-sequence form at radix 8 / 16;
+    the table then holds shapes that exist."""
     lib = device()
-    rep = (ctypes.c_double * 12)()
+    S = 7  # YA_TUNE_REPORT_STRIDE
+    rep = (ctypes.c_double * (2 * S))()
     capi.call("ya_autotune", 12, 13, rep)
-    assert all(rep[6 * c + k] > 0 for c in range(2) for k in (0, 1, 3, 4, 5))
-    assert lib.ya_tune_get(2, 12) in (3, 4) and lib.ya_tune_get(1, 13) in (3, 4)
+    assert all(rep[S * c + k] > 0 for c in range(2) for k in range(S))
+    assert lib.ya_tune_get(2, 12) in (3, 4) and lib.ya_tune_get(1, 13) in (3, 4, 5)
+    assert lib.ya_tune_get(3, 12) in (0, 1) and lib.ya_tune_get(3, 13) in (0, 1)
+    for lg in (12, 13):  # leave the defaults for the tests that follow
+        for kind in (0, 1, 2):
+            capi.call("ya_tune_set", kind, lg, 4)
+        capi.call("ya_tune_set", 3, lg, 1)
 
 
 @pytest.mark.parametrize("form", [1, 2])

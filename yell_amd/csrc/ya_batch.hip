@@ -602,8 +602,9 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
   }
   uint64_t *a = buf, *b = buf + words, *c = buf + 2 * words;
   for (int lg = logn_lo; lg <= logn_hi && rc == YA_OK; ++lg) {
-    double *rep = report ? report + 6 * (lg - logn_lo) : nullptr;
-    if (rep) rep[0] = rep[1] = rep[2] = rep[3] = rep[4] = rep[5] = -1.0;
+    double *rep = report ? report + YA_TUNE_REPORT_STRIDE * (lg - logn_lo) : nullptr;
+    if (rep)
+      for (int k = 0; k < YA_TUNE_REPORT_STRIDE; ++k) rep[k] = -1.0;
     if ((rc = ensure_tables(st, lg, 0, L))) break;
     const size_t n = (size_t)1 << lg;
     const size_t njobs = words / (n * L);
@@ -612,7 +613,7 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
       jobs[j] = ya_polymul_job{a + j * n * L, b + j * n * L, c + j * n * L, (uint32_t)n, (uint32_t)L};
     const int lemin = (lg >= 14) ? 4 : 3, lemax = (lg >= 12 && lg <= 14) ? 5 : 4;
     float best[2] = {1e30f, 1e30f};
-    int arg[2] = {4, 4};
+    int arg[2] = {4, 4}, arg_dual = tuned_shape(lg, 1).dual;
     for (int le = lemin; le <= lemax && rc == YA_OK; ++le) {
       float ms;
       if (le <= 4) {  // the batched kernel's 1024-thread tiles take radix 8 or 16
@@ -629,20 +630,26 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
         if (rep) rep[le - 3] = ms;
         if (ms < best[0]) best[0] = ms, arg[0] = le;
       }
-      if (lg <= 14 && rc == YA_OK) {
+      if (lg > 14) continue;
+      // the fused kernel: both forward transforms as one pass sequence with
+      // shared twiddles (dual, built for radix 8 / 16 from 2^10) and as two
+      // sequences, each timed where it exists
+      const bool has_dual = lg >= 10 && le <= 4;
+      for (int dual = has_dual ? 1 : 0; dual >= 0 && rc == YA_OK; --dual) {
         NttShape sh{le};
-        sh.dual = le <= 4 ? tuned_shape(lg, 1).dual : 0;  // the form that runs
+        sh.dual = dual;
         rc = time_ms(s, 3, &ms, [&]() {
           hipError_t ee = launch_polymul(c, a, b, desc_ptr(st, lg, 0), lg, njobs * L, L, sh, s);
           return ee == hipSuccess ? YA_OK : hip_fail(ee, "tune fused launch");
         });
-        if (rep) rep[3 + le - 3] = ms;
-        if (ms < best[1]) best[1] = ms, arg[1] = le;
+        if (rep) rep[(has_dual && dual == 0) ? 5 + le - 3 : 2 + le - 3] = ms;
+        if (ms < best[1]) best[1] = ms, arg[1] = le, arg_dual = dual;
       }
     }
     set_tuned_shape(lg, 2, NttShape{arg[0]});
     set_tuned_shape(lg, 1, NttShape{arg[1]});
     set_tuned_shape(lg, 0, NttShape{arg[1]});
+    if (lg >= 10 && lg <= 14 && arg[1] <= 4) set_tuned_dual(lg, arg_dual);
   }
   (void)hipStreamDestroy(s);
   (void)hipFree(buf);
