@@ -27,16 +27,60 @@
 
 namespace ya {
 
-// Sub-grouped fused poly-multiplication of up to 2^(10+LE-LB) residues.
-template <int LB, int LE>
-__device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes *__restrict__ res,
-                                            uint64_t *lds, uint32_t t) {
+// Barriers of the sub-groups.  A sub-group of one wave or less needs no
+// barrier between its passes: a wave's LDS accesses are performed in issue
+// order, so the wave waits for its own LDS operations and the compiler is
+// kept from moving LDS accesses across the point.
+struct WaveSync {
+  __device__ __forceinline__ void operator()() const {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
+// A sub-group of 2..8 waves: a sense-reversing barrier in LDS, so that the
+// sub-groups of a workgroup run their passes independently (the workgroup's
+// barrier would hold them all in lockstep).  Lane 0 of each wave counts its
+// wave in; the last wave of the sub-group resets the count and flips the
+// flag, the others wait for the flip.  The wait is bounded (kSubSpinLimit
+// polls of about a hundred cycles each): a barrier that never completed would
+// leave wrong words, which the parity tests see, rather than a hung kernel.
+constexpr uint32_t kSubSpinLimit = 1u << 20;
+struct SubSync {
+  uint32_t *cnt, *flag;  // this sub-group's arrival count and completion flag (LDS)
+  uint32_t last;         // waves in the sub-group - 1
+  uint32_t sense;        // flag value while this wave's current barrier is open
+  bool lane0;
+  __device__ __forceinline__ void operator()() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // this wave's LDS accesses are done
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    uint32_t old = 0;
+    if (lane0) old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    old = __builtin_amdgcn_readfirstlane(old);
+    if (old == last) {
+      if (lane0) {
+        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(flag, sense ^ 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    } else {
+      for (uint32_t spin = 0;
+           spin < kSubSpinLimit && __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == sense;
+           ++spin)
+        __builtin_amdgcn_s_sleep(1);
+    }
+    sense ^= 1u;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+};
+
+// The fused poly-multiplication of one sub-group's residue (sub-group `sub`
+// of 2^(LB-LE) threads), its exchanges separated by `sync`.
+template <int LB, int LE, class S>
+__device__ __forceinline__ void batch_small_body(const BatchItem &it, const BatchRes *__restrict__ res,
+                                                 uint64_t *lds, uint32_t sub, uint32_t t, S &&sync) {
   constexpr uint32_t TS = 1u << (LB - LE);  // threads per residue
-  // a sub-group of whole waves has a wave-uniform index: take it from lane
-  // 0's thread index (a scalar), so the LDS slice address is scalar too
-  const uint32_t sub = TS >= (uint32_t)kBatchThreads ? 0u
-                       : TS >= 64u        ? __builtin_amdgcn_readfirstlane(t) / TS
-                                          : t / TS;
   const uint32_t tl = t % TS;
   const bool active = sub < it.count;
   BatchRes r;
@@ -59,12 +103,12 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
     constexpr int W0 = FirstWindow<LB, LE>::value;
     gmem_load<W0, LE>(r.a, x, tl);
     gmem_load<W0, LE>(r.b, y, tl);
-    fwd_block_dual<LB, LE>(x, y, my_lds, (const TwC *)d.fwd, LB, 0u, d.p, tl);
+    fwd_block_dual<LB, LE>(x, y, my_lds, (const TwC *)d.fwd, LB, 0u, d.p, tl, sync);
   } else {
-    fwd_block<LB, LE>(x, r.a, my_lds, (const TwC *)d.fwd, LB, 0u, d.p, tl);
+    fwd_block<LB, LE>(x, r.a, my_lds, (const TwC *)d.fwd, LB, 0u, d.p, tl, sync);
     if (active) gmem_store<0, LE>(r.c, x, tl);
     const uint32_t t2 = phase_v(tl);
-    fwd_block<LB, LE>(x, r.b, my_lds, (const TwC *)d.fwd + phase_zero_s(), LB, 0u, d.p, t2);
+    fwd_block<LB, LE>(x, r.b, my_lds, (const TwC *)d.fwd + phase_zero_s(), LB, 0u, d.p, t2, sync);
     if (active) {
       gmem_load<0, LE>(r.c, y, t2);
     } else {  // no residue: any defined words will do, nothing is stored
@@ -75,8 +119,37 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(tl);
-  inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, (const TwC *)d.inv, LB, 0u, d, t3);
+  inv_block<LB, LE, TOP_MERGED>(x, active ? r.c : nullptr, my_lds, (const TwC *)d.inv, LB, 0u, d, t3, sync);
 }
+
+// Sub-grouped fused poly-multiplication of up to 2^(10+LE-LB) residues.
+// s_cnt / s_flag: the workgroup's sub-group barrier words (8 each, zeroed at
+// kernel start; every barrier leaves its count at zero).
+template <int LB, int LE>
+__device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes *__restrict__ res,
+                                            uint64_t *lds, uint32_t *s_cnt, uint32_t *s_flag, uint32_t t) {
+  constexpr uint32_t TS = 1u << (LB - LE);  // threads per residue
+  // a sub-group of whole waves has a wave-uniform index: take it from lane
+  // 0's thread index (a scalar), so the LDS slice address is scalar too
+  const uint32_t sub = TS >= (uint32_t)kBatchThreads ? 0u
+                       : TS >= 64u        ? __builtin_amdgcn_readfirstlane(t) / TS
+                                          : t / TS;
+  if constexpr (TS >= (uint32_t)kBatchThreads) {
+    batch_small_body<LB, LE>(it, res, lds, sub, t, WgSync());
+  } else if constexpr (TS <= 64u) {
+    batch_small_body<LB, LE>(it, res, lds, sub, t, WaveSync());
+  } else {
+    static_assert(TS / 64u <= 8u, "sub-group barrier words");
+    SubSync sy;
+    sy.cnt = s_cnt + sub;
+    sy.flag = s_flag + sub;
+    sy.last = TS / 64u - 1u;
+    sy.sense = __builtin_amdgcn_readfirstlane(__hip_atomic_load(s_flag + sub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    sy.lane0 = (t & 63u) == 0u;
+    batch_small_body<LB, LE>(it, res, lds, sub, t, sy);
+  }
+}
+
 
 // Bit 14 of one operand of a 2^15 poly-multiplication in registers: thread t
 // loads its first-window words t | e << 10 of the lower half and their
@@ -182,6 +255,11 @@ k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
   // item i + 1 into the other slot, so the queue atomic's round trip overlaps
   // the work and one barrier per item remains
   __shared__ uint32_t s_item[2], s_seg, s_tried;
+  __shared__ uint32_t s_cnt[8], s_flag[8];  // sub-group barriers (SubSync)
+  if (threadIdx.x < 8) {
+    s_cnt[threadIdx.x] = 0;
+    s_flag[threadIdx.x] = 0;
+  }
   if (threadIdx.x == 0) {
     s_seg = blockIdx.x % kSegs;
     s_tried = 0;
@@ -201,21 +279,21 @@ k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
     if (t == 0) s_item[(i + 1) & 1u] = claim_item(segs, q, s_seg, s_tried);
     const BatchItem it = items[k];
     switch (it.logn * 8 + it.le) {
-      case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, t); break;
-      case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, t); break;
-      case 8 * 8 + 3: batch_small<8, 3>(it, res, lds, t); break;
-      case 8 * 8 + 4: batch_small<8, 4>(it, res, lds, t); break;
-      case 9 * 8 + 3: batch_small<9, 3>(it, res, lds, t); break;
-      case 9 * 8 + 4: batch_small<9, 4>(it, res, lds, t); break;
-      case 10 * 8 + 3: batch_small<10, 3>(it, res, lds, t); break;
-      case 10 * 8 + 4: batch_small<10, 4>(it, res, lds, t); break;
-      case 11 * 8 + 3: batch_small<11, 3>(it, res, lds, t); break;
-      case 11 * 8 + 4: batch_small<11, 4>(it, res, lds, t); break;
-      case 12 * 8 + 3: batch_small<12, 3>(it, res, lds, t); break;
-      case 12 * 8 + 4: batch_small<12, 4>(it, res, lds, t); break;
-      case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, t); break;
-      case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, t); break;
-      case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, t); break;
+      case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, s_cnt, s_flag, t); break;
+      case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 8 * 8 + 3: batch_small<8, 3>(it, res, lds, s_cnt, s_flag, t); break;
+      case 8 * 8 + 4: batch_small<8, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 9 * 8 + 3: batch_small<9, 3>(it, res, lds, s_cnt, s_flag, t); break;
+      case 9 * 8 + 4: batch_small<9, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 10 * 8 + 3: batch_small<10, 3>(it, res, lds, s_cnt, s_flag, t); break;
+      case 10 * 8 + 4: batch_small<10, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 11 * 8 + 3: batch_small<11, 3>(it, res, lds, s_cnt, s_flag, t); break;
+      case 11 * 8 + 4: batch_small<11, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 12 * 8 + 3: batch_small<12, 3>(it, res, lds, s_cnt, s_flag, t); break;
+      case 12 * 8 + 4: batch_small<12, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, s_cnt, s_flag, t); break;
+      case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, s_cnt, s_flag, t); break;
       case 15 * 8 + 4: batch_15(it, res, lds, scratch + ((size_t)blockIdx.x << 14), t); break;
       default: break;  // rejected by the host
     }

@@ -47,6 +47,14 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g) {
 }
 
 // ---------------------------------------------------------------------------
+// Barriers between the passes of a block transform.  WgSync is the
+// workgroup's barrier; a transform whose threads are a part of a larger
+// workgroup (the persistent batch kernel's sub-groups) passes a barrier of
+// its own threads instead (ya_batch.hip).
+struct WgSync {
+  __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+
 // Window start of the first forward pass of a 2^LB block.
 template <int LB, int LE>
 struct FirstWindow {
@@ -57,10 +65,10 @@ struct FirstWindow {
 // global `src` (block base), or already in x (PRELOADED: loaded with
 // gmem_load<FirstWindow<LB,LE>::value, LE>).  On return x holds the words of
 // window [0, LE) for thread t (i = e | t << LE), in [0, 4p).
-template <int LB, int LE, bool PRELOADED = false, class W, class TW>
+template <int LB, int LE, bool PRELOADED = false, class W, class TW, class S = WgSync>
 __device__ __forceinline__ void fwd_block(W (&x)[1 << LE], const typename Ident<W>::type *__restrict__ src,
                                           W *lds, const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
-                                          typename Ident<W>::type p, uint32_t t) {
+                                          typename Ident<W>::type p, uint32_t t, S &&sync = S()) {
   constexpr int NP = (LB + LE - 1) / LE;
   static_for<0, NP>([&](auto PC) {
     constexpr int P = decltype(PC)::value;
@@ -75,9 +83,9 @@ __device__ __forceinline__ void fwd_block(W (&x)[1 << LE], const typename Ident<
     const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
     fwd_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, p);
     if constexpr (P + 1 < NP) {
-      __syncthreads();
+      sync();
       lds_store<WLO, LE>(lds, x, t);
-      __syncthreads();
+      sync();
     }
   });
 }
@@ -86,10 +94,10 @@ __device__ __forceinline__ void fwd_block(W (&x)[1 << LE], const typename Ident<
 // with gmem_load<FirstWindow<LB,LE>::value, LE>): per pass the layers of both
 // share their twiddles; the exchanges go through LDS one block after the
 // other.  On return x and y hold window [0, LE), as fwd_block leaves them.
-template <int LB, int LE, class W, class TW>
+template <int LB, int LE, class W, class TW, class S = WgSync>
 __device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE], W *lds,
                                                const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
-                                               typename Ident<W>::type p, uint32_t t) {
+                                               typename Ident<W>::type p, uint32_t t, S &&sync = S()) {
   constexpr int NP = (LB + LE - 1) / LE;
   static_for<0, NP>([&](auto PC) {
     constexpr int P = decltype(PC)::value;
@@ -101,13 +109,13 @@ __device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE],
     if constexpr (P + 1 < NP) {
       constexpr int HI2 = LB - (P + 1) * LE;
       constexpr int WLO2 = HI2 - LE > 0 ? HI2 - LE : 0;
-      __syncthreads();
+      sync();
       lds_store<WLO, LE>(lds, x, t);
-      __syncthreads();
+      sync();
       lds_load<WLO2, LE>(lds, x, t);
-      __syncthreads();
+      sync();
       lds_store<WLO, LE>(lds, y, t);
-      __syncthreads();
+      sync();
       lds_load<WLO2, LE>(lds, y, t);
     }
   });
@@ -122,10 +130,10 @@ __device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE],
 //               lib/ntt.cpp:192), outputs in [0, 2p).
 enum { TOP_GS = 0, TOP_MERGED = 1, TOP_SKIP = 2 };
 
-template <int LB, int LE, int TOP, class W, class TW, class D>
+template <int LB, int LE, int TOP, class W, class TW, class D, class S = WgSync>
 __device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::type *__restrict__ dst,
                                           W *lds, const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
-                                          const D &d, uint32_t t) {
+                                          const D &d, uint32_t t, S &&sync = S()) {
   constexpr int NP = (LB + LE - 1) / LE;
   static_for<0, NP>([&](auto QC) {
     constexpr int Q = decltype(QC)::value;
@@ -144,9 +152,9 @@ __device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::ty
       inv_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, d.p);
     }
     if constexpr (!LASTQ) {
-      __syncthreads();
+      sync();
       lds_store<WLO, LE>(lds, x, t);
-      __syncthreads();
+      sync();
     } else if (dst != nullptr) {  // null: a sub-group with no residue of its own
       gmem_store<WLO, LE>(dst, x, t);
     }
