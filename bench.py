@@ -706,18 +706,22 @@ def main():
             off = k * w * 8
             capi.call("ya_poly_multiply", dc.ptr + off, da.ptr + off, db.ptr + off, n, L, stream)
 
+    def settle_on(fn, seconds=0.2):
+        # after host-side work (input generation, a large download) the GPU
+        # has idled and its clocks ramp back up over ~0.1 s of work; keep
+        # stepping (untimed) until `seconds` of steps have run
+        k = 0
+        t_settle = time.perf_counter()
+        while time.perf_counter() - t_settle < seconds:
+            fn()
+            capi.call("ya_stream_synchronize", stream)
+            k += 1
+        return k
+
     for _ in range(args.warmup):
         batched()
     capi.call("ya_stream_synchronize", stream)
-    # settle: after the host-side input generation the GPU has idled for
-    # seconds and its clocks ramp back up over ~0.1 s of work; keep stepping
-    # (untimed) until 0.2 s of batched steps have run
-    settle = 0
-    t_settle = time.perf_counter()
-    while time.perf_counter() - t_settle < 0.2:
-        batched()
-        capi.call("ya_stream_synchronize", stream)
-        settle += 1
+    settle = settle_on(batched)
     log("timing batched:", args.steps, "steps")
     t0 = time.perf_counter()
     ms_step = timed(ev, stream, batched, args.steps)
@@ -728,8 +732,10 @@ def main():
     c_batched = dc.download()
     capi.call("ya_memset_zero", dc.ptr, B * w * 8, stream)
     grid_form()
-    ms_grid = timed(ev, stream, grid_form, max(1, args.steps // 2))
+    capi.call("ya_stream_synchronize", stream)
     same_grid = bool(np.array_equal(c_batched, dc.download()))
+    settle_on(grid_form, 0.1)
+    ms_grid = timed(ev, stream, grid_form, max(1, args.steps // 2))
     capi.call("ya_memset_zero", dc.ptr, B * w * 8, stream)
     plain()
     capi.call("ya_stream_synchronize", stream)
