@@ -40,6 +40,11 @@ class Batch:
     def results(self):
         return [dc.download() for _, _, dc in self.bufs]
 
+    def faults(self):
+        n = ctypes.c_uint()
+        capi.call("ya_batch_plan_faults", self.plan, ctypes.byref(n))
+        return n.value
+
     def close(self):
         capi.load().ya_batch_plan_destroy(self.plan)
 
@@ -57,6 +62,7 @@ def test_uneven_batch_vs_oracle():
         capi.call("ya_device_synchronize")
         for (n, L, a, b), got in zip(host, bt.results()):
             np.testing.assert_array_equal(got, orc.polymul(a, b, n, L))
+        assert bt.faults() == 0
     bt.close()
 
 
@@ -153,10 +159,13 @@ def test_rejects_unsupported_degree():
 
 
 def test_every_pass_shape_gives_the_same_words():
-    """The autotuner may pick radix-8 or radix-16 passes: both give the oracle's words."""
+    """The autotuner may pick radix-8 or radix-16 passes: both give the oracle's
+    words, in every sub-group form of the persistent kernel (one wave or less:
+    no barrier; 2..8 waves: the sub-group's own LDS barrier; 16: the
+    workgroup's), and no sub-group barrier gives up waiting."""
     lib = device()
     rng = np.random.default_rng(5)
-    host = _jobs(rng, 10, [9, 10, 11, 12, 13], 5)
+    host = _jobs(rng, 14, [7, 8, 9, 10, 11, 12, 13], 5)
     saved = {lg: lib.ya_tune_get(2, lg) for lg in range(7, 14)}
     try:
         for le in (3, 4):
@@ -168,6 +177,7 @@ def test_every_pass_shape_gives_the_same_words():
             capi.call("ya_device_synchronize")
             for (n, L, a, b), got in zip(host, bt.results()):
                 np.testing.assert_array_equal(got, orc.polymul(a, b, n, L))
+            assert bt.faults() == 0
             bt.close()
     finally:
         for lg, le in saved.items():

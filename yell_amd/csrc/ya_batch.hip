@@ -45,11 +45,13 @@ struct WaveSync {
 // barrier would hold them all in lockstep).  Lane 0 of each wave counts its
 // wave in; the last wave of the sub-group resets the count and flips the
 // flag, the others wait for the flip.  The wait is bounded (kSubSpinLimit
-// polls of about a hundred cycles each): a barrier that never completed would
-// leave wrong words, which the parity tests see, rather than a hung kernel.
+// polls of about a hundred cycles each): a barrier that never completed
+// counts a fault in the queue (ya_batch_plan_faults) instead of hanging the
+// kernel.
 constexpr uint32_t kSubSpinLimit = 1u << 20;
 struct SubSync {
   uint32_t *cnt, *flag;  // this sub-group's arrival count and completion flag (LDS)
+  uint32_t *faults;      // the queue's fault count (global)
   uint32_t last;         // waves in the sub-group - 1
   uint32_t sense;        // flag value while this wave's current barrier is open
   bool lane0;
@@ -65,10 +67,14 @@ struct SubSync {
         __hip_atomic_store(flag, sense ^ 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     } else {
-      for (uint32_t spin = 0;
-           spin < kSubSpinLimit && __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == sense;
-           ++spin)
+      uint32_t spin = 0;
+      while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == sense) {
+        if (++spin == kSubSpinLimit) {
+          if (lane0) atomicAdd(faults, 1u);
+          break;
+        }
         __builtin_amdgcn_s_sleep(1);
+      }
     }
     sense ^= 1u;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -127,7 +133,8 @@ __device__ __forceinline__ void batch_small_body(const BatchItem &it, const Batc
 // kernel start; every barrier leaves its count at zero).
 template <int LB, int LE>
 __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes *__restrict__ res,
-                                            uint64_t *lds, uint32_t *s_cnt, uint32_t *s_flag, uint32_t t) {
+                                            uint64_t *lds, uint32_t *s_cnt, uint32_t *s_flag, uint32_t *faults,
+                                            uint32_t t) {
   constexpr uint32_t TS = 1u << (LB - LE);  // threads per residue
   // a sub-group of whole waves has a wave-uniform index: take it from lane
   // 0's thread index (a scalar), so the LDS slice address is scalar too
@@ -143,6 +150,7 @@ __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes 
     SubSync sy;
     sy.cnt = s_cnt + sub;
     sy.flag = s_flag + sub;
+    sy.faults = faults;
     sy.last = TS / 64u - 1u;
     sy.sense = __builtin_amdgcn_readfirstlane(__hip_atomic_load(s_flag + sub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
     sy.lane0 = (t & 63u) == 0u;
@@ -279,21 +287,21 @@ k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
     if (t == 0) s_item[(i + 1) & 1u] = claim_item(segs, q, s_seg, s_tried);
     const BatchItem it = items[k];
     switch (it.logn * 8 + it.le) {
-      case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, s_cnt, s_flag, t); break;
-      case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, s_cnt, s_flag, t); break;
-      case 8 * 8 + 3: batch_small<8, 3>(it, res, lds, s_cnt, s_flag, t); break;
-      case 8 * 8 + 4: batch_small<8, 4>(it, res, lds, s_cnt, s_flag, t); break;
-      case 9 * 8 + 3: batch_small<9, 3>(it, res, lds, s_cnt, s_flag, t); break;
-      case 9 * 8 + 4: batch_small<9, 4>(it, res, lds, s_cnt, s_flag, t); break;
-      case 10 * 8 + 3: batch_small<10, 3>(it, res, lds, s_cnt, s_flag, t); break;
-      case 10 * 8 + 4: batch_small<10, 4>(it, res, lds, s_cnt, s_flag, t); break;
-      case 11 * 8 + 3: batch_small<11, 3>(it, res, lds, s_cnt, s_flag, t); break;
-      case 11 * 8 + 4: batch_small<11, 4>(it, res, lds, s_cnt, s_flag, t); break;
-      case 12 * 8 + 3: batch_small<12, 3>(it, res, lds, s_cnt, s_flag, t); break;
-      case 12 * 8 + 4: batch_small<12, 4>(it, res, lds, s_cnt, s_flag, t); break;
-      case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, s_cnt, s_flag, t); break;
-      case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, s_cnt, s_flag, t); break;
-      case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, s_cnt, s_flag, t); break;
+      case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 8 * 8 + 3: batch_small<8, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 8 * 8 + 4: batch_small<8, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 9 * 8 + 3: batch_small<9, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 9 * 8 + 4: batch_small<9, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 10 * 8 + 3: batch_small<10, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 10 * 8 + 4: batch_small<10, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 11 * 8 + 3: batch_small<11, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 11 * 8 + 4: batch_small<11, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 12 * 8 + 3: batch_small<12, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 12 * 8 + 4: batch_small<12, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
       case 15 * 8 + 4: batch_15(it, res, lds, scratch + ((size_t)blockIdx.x << 14), t); break;
       default: break;  // rejected by the host
     }
@@ -590,6 +598,18 @@ int ya_batch_plan_info(const ya_batch_plan *pl, size_t *items, size_t *residues,
                   : pl->form == YA_BATCH_SPLIT ? pl->nres << pl->split_k
                                                : pl->grid;
   if (form) *form = pl->form;
+  return YA_OK;
+}
+
+int ya_batch_plan_faults(ya_batch_plan *pl, unsigned *count) {
+  if (!pl || !count) return fail(YA_ERR_INVALID, "null argument");
+  *count = 0;
+  if (!pl->d_q) return YA_OK;
+  uint32_t n = 0;
+  hipError_t e = hipMemcpy(&n, &pl->d_q->faults, sizeof n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && n) e = hipMemset(&pl->d_q->faults, 0, sizeof n);
+  if (e != hipSuccess) return hip_fail(e, "batch fault count");
+  *count = n;
   return YA_OK;
 }
 
