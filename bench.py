@@ -298,8 +298,8 @@ def c4_bench(stream, ev, steps):
 
 def u32_bench(stream, ev, reps=5):
     """32-bit moduli build (ya32_poly_multiply_many): N = 2^14 with 16 primes
-    of 30 bits (a ~480-bit modulus, close to the 8 x 62-bit C2 modulus), 256
-    polys per launch (4096 residues, as the C2 step)."""
+    of 30 bits (a ~480-bit modulus, close to the 8 x 62-bit modulus of the headline), 256
+    polys per launch (4096 residues, as the headline step)."""
     n, L, count = 1 << 14, 16, 256
     w = n * L
     rng = np.random.default_rng(5)
@@ -351,7 +351,7 @@ def u32_cpu_reference(u32, seconds=3.0):
 
 
 def latency_bench(stream, ev, n, L, reps=20):
-    """Device time of ONE poly-multiplication (a single C2 request) as a split
+    """Device time of ONE poly-multiplication (a single N = 2^14, L = 8 request) as a split
     launch (each residue over many workgroups) and as the one-workgroup-per-
     residue fused kernel; median over reps."""
     a = capi.DeviceBuffer.from_host(capi.sample_uniform(n, L, 31))
@@ -402,7 +402,7 @@ def crt_bench(stream, ev, reps=10):
 
 
 def crt_cpu_reference(crt):
-    """The reference's poly2mpz (GMP, one thread) on the same C2 polynomial,
+    """The reference's poly2mpz (GMP, one thread) on the same N = 2^14, L = 8 polynomial,
     and bit-exactness of the GPU lift against it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle  # checker only
@@ -420,7 +420,7 @@ def crt_cpu_reference(crt):
 
 
 def sampler_bench(stream, ev, reps=20):
-    """Device samplers drawing a C2 operand pair straight into HBM (a uniform,
+    """Device samplers drawing an N = 2^14, L = 8 operand pair straight into HBM (a uniform,
     b Gaussian sigma = 3.2; N = 2^14, L = 8) against the host samplers plus
     the upload they replace; ZO and hwt at the RLWE flow's shape."""
     n, L = 1 << 14, 8
@@ -588,7 +588,7 @@ KNEE_P99_US = 5000.0
 
 
 def served_sweep(rates, duration, n, L, batch_jobs=256, inflight=3, verify=1):
-    """Batched served stream (C2 shape, every result verified) at increasing
+    """Batched served stream (N = 2^14, L = 8, every result verified) at increasing
     offered loads until it saturates: the knee is the highest offered rate
     whose completed throughput keeps within 3% of it with no rejection, no
     failure and p99 latency within KNEE_P99_US."""
@@ -621,7 +621,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch, probes and tuner")
+    ap.add_argument("--no-extra", action="store_true", help="skip the uneven batch, probes and tuner")
     ap.add_argument("--only", default="",
                     help="comma list of the extra sections to run (tune, probe, c3, c4, crt, latency, u32, "
                          "samplers, rlwe, sweep, served); default all")
@@ -773,10 +773,10 @@ def main():
         out["modmul_rate"]["butterfly_probe_T_per_s"] = pk / 1e12
         out["modmul_rate"]["frac_of_probe"] = mm * B / (ms_step / 1e3) / pk
     if want("c3"):
-        log("C3 uneven batch")
+        log("uneven batch: 1024 jobs")
         out["c3_uneven_batch"] = c3_bench(stream, ev, max(2, args.steps // 4))
     if want("c4"):
-        log("C4 large-ring MAC")
+        log("large-ring MAC: N = 2^16, L = 24, 8 pairs")
         out["c4_large_ring_mac"] = c4_bench(stream, ev, 3)
     if want("crt"):
         log("CRT lift")

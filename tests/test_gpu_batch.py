@@ -98,7 +98,7 @@ def test_full_c3_batch_vs_oracle_through_graph():
     specs = [(1 << int(rng.integers(10, 16)), int(rng.integers(1, 13))) for _ in range(njobs)]
     offs = np.cumsum([0] + [n * L for n, L in specs])
     total = int(offs[-1])
-    say("C3 full batch through the graph", total, "words per operand")
+    say("uneven 1024-job batch through the graph", total, "words per operand")
     A, B, C, D = (capi.DeviceBuffer(total) for _ in range(4))
     arr = (capi.PolymulJob * njobs)()
     for k, (n, L) in enumerate(specs):
@@ -128,7 +128,7 @@ def test_full_c3_batch_vs_oracle_through_graph():
             a, b = operands(seed)
             A.upload(a)
             B.upload(b)
-            say("C3 graph run", run)
+            say("uneven batch graph run", run)
             capi.call("ya_batch_plan_run_graph", plan, stream)
             capi.call("ya_stream_synchronize", stream)
             got = C.download()

@@ -71,7 +71,7 @@ def test_lift_register_path(n, L):
 def test_lift_c4_shape():
     """N = 2^16, L = 24: every 61st coefficient against the restatement."""
     n, L = 1 << 16, 24
-    say("crt C4 shape", n, L)
+    say("crt 2^16 x 24 shape", n, L)
     rng = np.random.default_rng(24)
     res = np.concatenate([rng.integers(0, p, size=n, dtype=np.uint64) for p in primes(L)])
     _check_against_oracle(res, n, L, sample=np.arange(0, n, 61))

@@ -138,14 +138,14 @@ def test_c4_large_ring_mac():
     w = n * L
     dacc = capi.DeviceBuffer(w)
     capi.call("ya_memset_zero", dacc.ptr, w * 8, None)
-    say("C4 forward x16")
+    say("2^16 x 24: forward x16")
     capi.call("ya_poly_forward_many", dall.ptr, n, L, 2 * pairs, None)
     pa = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j) for j in range(pairs)])
     pb = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j + 1) for j in range(pairs)])
     capi.call("ya_poly_add_products", dacc.ptr, pa, pb, pairs, n, L, None)
     capi.call("ya_poly_backward", dacc.ptr, n, L, None)
     got = dacc.download()
-    say("C4 oracle")
+    say("2^16 x 24: oracle")
     acc = np.zeros(w, dtype=np.uint64)
     for j in range(pairs):
         fa = orc.poly_forward(ops[2 * j], n, L)
@@ -204,7 +204,7 @@ def test_add_products_backward_c4_and_rejects_overlap():
     capi.call("ya_memset_zero", dacc.ptr, w * 8, None)
     pa = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j) for j in range(pairs)])
     pb = (ctypes.c_void_p * pairs)(*[dall.ptr + 8 * w * (2 * j + 1) for j in range(pairs)])
-    say("add_products_backward C4")
+    say("add_products_backward 2^16 x 24, 8 pairs")
     capi.call("ya_poly_add_products_backward", dacc.ptr, pa, pb, pairs, n, L, None)
     got = dacc.download()
     np.testing.assert_array_equal(got, _mac_want(np.zeros(w, dtype=np.uint64), ops[0::2], ops[1::2], n, L))

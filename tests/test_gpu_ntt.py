@@ -144,7 +144,7 @@ def test_polymul_c2_shape_uniform_times_gaussian():
     n, L = 1 << 14, 8
     a = capi.sample_uniform(n, L, 2026)
     b = capi.sample_gaussian(n, L, 3.2, 2027)
-    say("polymul C2", n, L)
+    say("polymul 2^14 x 8", n, L)
     np.testing.assert_array_equal(polymul(a, b, n, L), orc.polymul(a, b, n, L))
 
 
@@ -152,7 +152,7 @@ def test_polymul_c1_schoolbook():
     n = 4096
     a = capi.sample_uniform(n, 1, 11)
     b = capi.sample_uniform(n, 1, 12)
-    say("polymul C1 schoolbook")
+    say("polymul 4096 x 1 schoolbook")
     np.testing.assert_array_equal(polymul(a, b, n, 1), orc.schoolbook(a, b, n, 0))
 
 

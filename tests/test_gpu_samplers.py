@@ -1,6 +1,6 @@
 """Device samplers (ya_sample_*_device, ya32_sample_*_device): the same words
 as the host samplers for the same seed, bit-exact, at small and full sizes
-(C2: N = 2^14, L = 8; C4: N = 2^16, L = 24), hwt at weight 1 and at the
+(N = 2^14, L = 8 and N = 2^16, L = 24), hwt at weight 1 and at the
 degree, the Gaussian amplifier against the oracle."""
 import numpy as np
 import pytest

@@ -64,7 +64,7 @@ def test_split_extreme_words():
 
 
 def test_auto_policy_small_and_large():
-    """Automatic policy: a single C2 poly is split, 64 of them are not; both
+    """Automatic policy: a single 2^14 x 8 poly is split, 64 of them are not; both
     give the oracle's words."""
     n, L = 1 << 14, 8
     a = capi.sample_uniform(n, L, 5)

@@ -28,7 +28,7 @@ __device__ __forceinline__ void static_for(F &&f) {
 // wave-uniform value.  When the compiler sees the address as kernel argument
 // plus an offset from blockIdx it rebuilds the per-word addresses in vector
 // registers, and the 64-bit multiply chains then lose their zero-extended
-// register pairs to extra v_mov copies (the fused C2 kernel: 11,845 VALU
+// register pairs to extra v_mov copies (the fused 2^14 kernel: 11,845 VALU
 // instructions per wave instead of 10,534).
 template <class T>
 __device__ __forceinline__ T *uniform_ptr(T *p) {

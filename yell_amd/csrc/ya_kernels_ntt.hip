@@ -223,7 +223,7 @@ static hipError_t launch_ntt4(uint64_t *data, const ModDesc *descs, size_t nres,
 // Column kernels for N = 2^(B+K): one thread per column c in [0, 2^B) holding
 // the 2^K words c + e * 2^B (the layers on index bits >= B; the blocks of
 // 2^B words take the rest).  Workgroups of 2^TW threads: 256, or 64 for a
-// launch of few columns (C4's inverse: 24 residues of 2^12 columns are 384
+// launch of few columns (the inverse of 24 residues of 2^16 in 2^12 columns: 384
 // workgroups of 256 threads, 1.5 per CU, or 1536 of 64, 6 per CU).
 constexpr int kTopThreads = 256;
 constexpr int kTopThreadsLog = 8, kTopThreadsSmallLog = 6;
@@ -263,7 +263,7 @@ k_inv_top(uint64_t *__restrict__ data, const ModDesc *__restrict__ descs, uint32
 
 // Block size of a ring above 2^kMaxBlockLog: 2^14 words, or 2^13 / 2^12 when
 // the launch would otherwise have fewer workgroups than the device has CUs
-// (a few residues: C4's single inverse is 24 residues of 2^16), keeping the
+// (a few residues: one inverse of 24 residues of 2^16), keeping the
 // column pass to at most 4 layers (5 at 2^19: 32 words per thread).
 constexpr size_t kFillWorkgroups = 256;
 static int large_block_log(int logn, size_t nres) {

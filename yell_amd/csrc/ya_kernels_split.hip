@@ -2,7 +2,7 @@
 // many workgroups.
 //
 // The fused kernel gives each residue one workgroup; a launch of only a few
-// residues (one C2 request is 8) then occupies 8 of the 256 CUs and its
+// residues (one request of 2^14 words x 8 primes is 8) then occupies 8 of the 256 CUs and its
 // latency is that of one CU transforming 2^14 words three times.  Here a
 // residue of N = 2^(S+K) words is split at index bit S:
 //   1. column kernel: forward layers [S, S+K) of a (into c) and of b (into a

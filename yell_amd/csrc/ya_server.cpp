@@ -726,7 +726,7 @@ int ya_server_run_poisson(ya_server *s, const ya_poisson_spec *spec, ya_serve_re
   // kWaiters threads take results (waiter w: requests w, w + kWaiters, ...),
   // verify them and retire their ring entries; the ring advances over the
   // contiguous prefix of retired requests.  Verification reads two 1 MiB
-  // buffers per C2 request: with 4 threads the verifiers fell behind near
+  // buffers per request of 2^14 x 8 words: with 4 threads the verifiers fell behind near
   // 16 K req/s, the ring filled and arrivals were held back and released in
   // bursts (client_stalls), which the server saw as queueing -- p99 of 10-40
   // ms that a run without verification did not have (1.2 ms at 16 K req/s).
