@@ -165,7 +165,17 @@ def make_plan(jobs_spec):
     return plan
 
 
-def timed(ev, stream, fn, steps):
+def timed(ev, stream, fn, steps, settle_s=0.1):
+    """Mean ms of `steps` calls of fn, timed with events on `stream`, after
+    `settle_s` seconds of untimed calls: after host-side work (input
+    generation, uploads, downloads) the GPU has idled and its clocks take
+    ~0.1 s of work to ramp back up."""
+    t0 = time.perf_counter()
+    while True:
+        fn()
+        capi.call("ya_stream_synchronize", stream)
+        if time.perf_counter() - t0 >= settle_s:
+            break
     capi.call("ya_event_record", ev.a, stream)
     for _ in range(steps):
         fn()
@@ -205,6 +215,7 @@ def c3_bench(stream, ev, steps, seed=2026):
             "value": len(specs) / (ms / 1e3), "unit": "poly-mults/s", "ms_per_batch": ms,
             "plain_value": len(specs) / (ms_plain / 1e3), "ms_plain": ms_plain,
             "batched_over_plain": ms_plain / ms,
+            "modmuls_T_per_s": sum(modmuls_per_polymul(n, L) for n, L in specs) / (ms / 1e3) / 1e12,
             "hbm_GBps_algorithmic": 3 * 8 * total / (ms / 1e3) / 1e9}
 
 
@@ -309,13 +320,7 @@ def u32_bench(stream, ev, reps=5):
     da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
     dc = capi.DeviceBuffer(w * count, np.uint32)
     args = (dc.ptr, da.ptr, db.ptr, n, L, count, stream)
-    capi.call("ya32_poly_multiply_many", *args)
-    capi.call("ya_stream_synchronize", stream)
-    capi.call("ya_event_record", ev.a, stream)
-    for _ in range(reps):
-        capi.call("ya32_poly_multiply_many", *args)
-    capi.call("ya_event_record", ev.b, stream)
-    ms = ev.elapsed_ms() / reps
+    ms = timed(ev, stream, lambda: capi.call("ya32_poly_multiply_many", *args), reps)
     c0 = dc.download(w)
     mm = modmuls_per_polymul(n, L)
     bits = sum(p.bit_length() for p in primes)
@@ -360,6 +365,7 @@ def latency_bench(stream, ev, n, L, reps=20):
     out = {}
     for name, mode in (("split_auto", 0), ("fused", -1)):
         capi.call("ya_split_policy", mode)
+        timed(ev, stream, lambda: capi.call("ya_poly_multiply", c.ptr, a.ptr, b.ptr, n, L, stream), 1)
         ts = []
         for _ in range(reps + 2):
             capi.call("ya_stream_synchronize", stream)
@@ -383,13 +389,7 @@ def crt_bench(stream, ev, reps=10):
         res = capi.sample_uniform(n, L, 77 + L)
         din = capi.DeviceBuffer.from_host(res)
         dout = capi.DeviceBuffer(n * w.value)
-        capi.call("ya_poly_crt_lift", dout.ptr, din.ptr, n, L, stream)
-        capi.call("ya_stream_synchronize", stream)
-        capi.call("ya_event_record", ev.a, stream)
-        for _ in range(reps):
-            capi.call("ya_poly_crt_lift", dout.ptr, din.ptr, n, L, stream)
-        capi.call("ya_event_record", ev.b, stream)
-        ms = ev.elapsed_ms() / reps
+        ms = timed(ev, stream, lambda: capi.call("ya_poly_crt_lift", dout.ptr, din.ptr, n, L, stream), reps)
         alg = (n * L + n * w.value) * 8
         out["n%d_l%d" % (n, L)] = {"us_per_poly": ms * 1e3, "coeffs_per_s": n / (ms / 1e3),
                                    "limbs": w.value, "algorithmic_bytes": alg,

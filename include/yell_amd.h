@@ -317,8 +317,9 @@ int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream);
  * 1 and 3; kind 0 follows kind 1).  report (optional) receives
  * YA_TUNE_REPORT_STRIDE doubles per class: batched ms at radix 8, 16; fused
  * ms at radix 8, 16 (dual where built), 32; fused ms at radix 8, 16 as two
- * sequences (-1 where not applicable).  Results are identical for every
- * shape. */
+ * sequences (-1 where not applicable); each is the best of 5 launches taken
+ * round-robin over the class's shapes, after ~0.15 s of untimed launches
+ * that bring the GPU's clocks up.  Results are identical for every shape. */
 enum { YA_TUNE_REPORT_STRIDE = 7 };
 int ya_autotune(int logn_lo, int logn_hi, double *report);
 /* Read / set one entry of the shape table: kind 1 = fused kernel, 2 =

@@ -650,28 +650,36 @@ int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream) {
 // tile; LE = 4: radix-16 passes) on zero-filled synthetic operands sized to
 // keep every CU busy for several items, and keeps the faster one in the
 // launch-shape table used by later plans and fused launches.  Timing is
-// data-independent (the butterflies are branch-free).
+// data-independent (the butterflies are branch-free); each candidate's time
+// is its best of kTuneRounds single launches, taken round-robin over the
+// class's candidates after the clocks have settled.
 namespace {
 
-int time_ms(hipStream_t s, int reps, float *best, const std::function<int()> &fn) {
-  hipEvent_t a, b;
-  (void)hipEventCreate(&a);
-  (void)hipEventCreate(&b);
-  *best = 1e30f;
-  int rc = fn();  // warm-up
-  for (int r = 0; rc == YA_OK && r < reps; ++r) {
-    (void)hipEventRecord(a, s);
-    rc = fn();
-    (void)hipEventRecord(b, s);
-    if (hipEventSynchronize(b) != hipSuccess) rc = fail(YA_ERR_LAUNCH, "tune sync");
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, a, b);
-    if (ms < *best) *best = ms;
-  }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+// One timed call of fn on s (ms through events a, b).
+int time_once(hipStream_t s, hipEvent_t a, hipEvent_t b, float *ms, const std::function<int()> &fn) {
+  (void)hipEventRecord(a, s);
+  int rc = fn();
+  (void)hipEventRecord(b, s);
+  if (hipEventSynchronize(b) != hipSuccess) return fail(YA_ERR_LAUNCH, "tune sync");
+  *ms = 0;
+  (void)hipEventElapsedTime(ms, a, b);
   return rc;
 }
+
+// One candidate launch shape of a degree class and where its time goes in
+// the report.
+struct TuneCand {
+  int slot;         // report column
+  bool persistent;  // batch plan (persistent kernel) or fused grid launch
+  int le, dual;
+  ya_batch_plan *pl;
+  float best;
+};
+
+// The GPU's clocks are low after an idle period and take ~0.1 s of work to
+// ramp up; the first class's first candidate runs untimed for that long.
+constexpr double kTuneSettleSeconds = 0.15;
+constexpr int kTuneRounds = 5;
 
 }  // namespace
 
@@ -698,6 +706,9 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
     (void)hipFree(buf);
     return hip_fail(e, "tune set-up");
   }
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess)
+    rc = fail(YA_ERR_LAUNCH, "tune events");
   uint64_t *a = buf, *b = buf + words, *c = buf + 2 * words;
   for (int lg = logn_lo; lg <= logn_hi && rc == YA_OK; ++lg) {
     double *rep = report ? report + YA_TUNE_REPORT_STRIDE * (lg - logn_lo) : nullptr;
@@ -710,10 +721,8 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
     for (size_t j = 0; j < njobs; ++j)
       jobs[j] = ya_polymul_job{a + j * n * L, b + j * n * L, c + j * n * L, (uint32_t)n, (uint32_t)L};
     const int lemin = (lg >= 14) ? 4 : 3, lemax = (lg >= 12 && lg <= 14) ? 5 : 4;
-    float best[2] = {1e30f, 1e30f};
-    int arg[2] = {4, 4}, arg_dual = tuned_shape(lg, 1).dual;
+    std::vector<TuneCand> cands;
     for (int le = lemin; le <= lemax && rc == YA_OK; ++le) {
-      float ms;
       if (le <= 4) {  // the batched kernel's 1024-thread tiles take radix 8 or 16
         set_tuned_shape(lg, 2, NttShape{le});
         ya_batch_plan *pl = nullptr;
@@ -723,32 +732,60 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
         rc = ya_batch_plan_create(&pl, jobs.data(), jobs.size());
         g_form_override.store(saved_form);
         if (rc) break;
-        rc = time_ms(s, 3, &ms, [&]() { return ya_batch_plan_run(pl, s); });
-        plan_free(pl);
-        if (rep) rep[le - 3] = ms;
-        if (ms < best[0]) best[0] = ms, arg[0] = le;
+        cands.push_back(TuneCand{le - 3, true, le, 0, pl, 1e30f});
       }
       if (lg > 14) continue;
       // the fused kernel: both forward transforms as one pass sequence with
       // shared twiddles (dual, built for radix 8 / 16 from 2^10) and as two
       // sequences, each timed where it exists
       const bool has_dual = lg >= 10 && le <= 4;
-      for (int dual = has_dual ? 1 : 0; dual >= 0 && rc == YA_OK; --dual) {
-        NttShape sh{le};
-        sh.dual = dual;
-        rc = time_ms(s, 3, &ms, [&]() {
-          hipError_t ee = launch_polymul(c, a, b, desc_ptr(st, lg, 0), lg, njobs * L, L, sh, s);
-          return ee == hipSuccess ? YA_OK : hip_fail(ee, "tune fused launch");
-        });
-        if (rep) rep[(has_dual && dual == 0) ? 5 + le - 3 : 2 + le - 3] = ms;
-        if (ms < best[1]) best[1] = ms, arg[1] = le, arg_dual = dual;
-      }
+      for (int dual = has_dual ? 1 : 0; dual >= 0; --dual)
+        cands.push_back(TuneCand{(has_dual && dual == 0) ? 5 + le - 3 : 2 + le - 3, false, le, dual, nullptr, 1e30f});
     }
+    auto run = [&](const TuneCand &cd) -> int {
+      if (cd.persistent) return ya_batch_plan_run(cd.pl, s);
+      NttShape sh{cd.le};
+      sh.dual = cd.dual;
+      hipError_t ee = launch_polymul(c, a, b, desc_ptr(st, lg, 0), lg, njobs * L, L, sh, s);
+      return ee == hipSuccess ? YA_OK : hip_fail(ee, "tune fused launch");
+    };
+    if (rc == YA_OK && !cands.empty() && lg == logn_lo) {
+      const auto t0 = std::chrono::steady_clock::now();
+      do {
+        rc = run(cands[0]);
+        if (rc == YA_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(YA_ERR_LAUNCH, "tune settle");
+      } while (rc == YA_OK &&
+               std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < kTuneSettleSeconds);
+    }
+    // every candidate once untimed, then kTuneRounds rounds over all of
+    // them (round-robin, so a drift of the clocks reaches each alike)
+    for (auto &cd : cands)
+      if (rc == YA_OK) rc = run(cd);
+    for (int r = 0; r < kTuneRounds && rc == YA_OK; ++r)
+      for (auto &cd : cands) {
+        float ms = 0;
+        if (rc == YA_OK) rc = time_once(s, ev0, ev1, &ms, [&]() { return run(cd); });
+        if (ms < cd.best) cd.best = ms;
+      }
+    float best[2] = {1e30f, 1e30f};
+    int arg[2] = {4, 4}, arg_dual = tuned_shape(lg, 1).dual;
+    for (auto &cd : cands) {
+      if (rep && rc == YA_OK) rep[cd.slot] = cd.best;
+      const int kind = cd.persistent ? 0 : 1;
+      if (cd.best < best[kind]) {
+        best[kind] = cd.best, arg[kind] = cd.le;
+        if (!cd.persistent) arg_dual = cd.dual;
+      }
+      if (cd.pl) plan_free(cd.pl);
+    }
+    if (rc) break;
     set_tuned_shape(lg, 2, NttShape{arg[0]});
     set_tuned_shape(lg, 1, NttShape{arg[1]});
     set_tuned_shape(lg, 0, NttShape{arg[1]});
     if (lg >= 10 && lg <= 14 && arg[1] <= 4) set_tuned_dual(lg, arg_dual);
   }
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
   (void)hipStreamDestroy(s);
   (void)hipFree(buf);
   return rc;
