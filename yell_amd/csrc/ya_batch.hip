@@ -206,19 +206,19 @@ __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__
     top15_fwd_keep(r.a, x, r.c + (1u << 14), w, k, t);
     top15_fwd_keep(r.b, y, scr, w, k, t);
   }
-  fwd_block_dual<14, LE>(x, y, lds, d.fwd + phase_zero_s(), 15, 0u, d.p, t);
+  fwd_block_dual<14, LE>(x, y, lds, (const TwC *)d.fwd + phase_zero_s(), 15, 0u, d.p, t);
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t1 = phase_v(t);
-  inv_block<14, LE, TOP_GS>(x, r.c, lds, d.inv + phase_zero_s(), 15, 0u, d, t1);
+  inv_block<14, LE, TOP_GS>(x, r.c, lds, (const TwC *)d.inv + phase_zero_s(), 15, 0u, d, t1);
   const uint32_t t2 = phase_v(t);
   gmem_load<W0, LE>(r.c + (1u << 14), x, t2);
   gmem_load<W0, LE>(scr, y, t2);
-  fwd_block_dual<14, LE>(x, y, lds, d.fwd + phase_zero_s(), 15, 1u << 14, d.p, t2);
+  fwd_block_dual<14, LE>(x, y, lds, (const TwC *)d.fwd + phase_zero_s(), 15, 1u << 14, d.p, t2);
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);
-  inv_block<14, LE, TOP_GS>(x, nullptr, lds, d.inv + phase_zero_s(), 15, 1u << 14, d, t3);
+  inv_block<14, LE, TOP_GS>(x, nullptr, lds, (const TwC *)d.inv + phase_zero_s(), 15, 1u << 14, d, t3);
   // merged top layer (include/yell/details/ntt.hpp:151-159) on words i, i + 2^14
   const uint64_t p = d.p, p2 = p << 1;
 #pragma unroll

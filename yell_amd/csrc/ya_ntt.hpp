@@ -79,8 +79,9 @@ __device__ __forceinline__ Tw32 load_tw(const Tw32 *p) {
   return Tw32{v.x, v.y};
 }
 
-// A twiddle table that the persistent batch kernel reads: its wave-uniform
-// loads go through the constant address space (scalar loads).  In that kernel
+// A twiddle table that the persistent batch kernel reads (every body, the
+// 2^15 one included): its wave-uniform loads go through the constant address
+// space (scalar loads).  In that kernel
 // the compiler cannot prove on its own that the table is not written before
 // a load (the previous work item's stores to c come first in the loop), so
 // through the global address space every lane would load the same pair on
