@@ -184,6 +184,54 @@ def test_every_pass_shape_gives_the_same_words():
             capi.call("ya_tune_set", 2, lg, le)
 
 
+def test_staggered_start_gives_the_same_words():
+    """The first wave's staggered start (tune kind 5) only delays workgroups:
+    at 0, 3 and 8 steps the fused grid kernel (1,280 residues of 2^14, five
+    waves) and the persistent kernel (1,024 items of 2^14, four per
+    workgroup) give the same words, and sampled polys the oracle's."""
+    lib = device()
+    n, L, count = 1 << 14, 8, 160
+    a = np.concatenate([capi.sample_uniform(n, L, 7100 + k) for k in range(count)])
+    b = np.concatenate([capi.sample_uniform(n, L, 7300 + k) for k in range(count)])
+    da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
+    dc = capi.DeviceBuffer(n * L * count)
+    rng = np.random.default_rng(13)
+    host = [(n, L, capi.sample_uniform(n, L, 3000 + 2 * k), capi.sample_uniform(n, L, 3001 + 2 * k))
+            for k in range(128)]
+    saved = {lg: lib.ya_tune_get(5, lg) for lg in (14, 15)}
+    capi.call("ya_batch_force_form", 1)
+    try:
+        bt = Batch(host)
+        got_grid, got_pers = {}, {}
+        for steps in (0, 3, 8):
+            for lg in (14, 15):
+                capi.call("ya_tune_set", 5, lg, steps)
+            assert lib.ya_tune_get(5, 14) == steps
+            say("stagger steps", steps)
+            capi.call("ya_poly_multiply_many", dc.ptr, da.ptr, db.ptr, n, L, count, None)
+            capi.call("ya_batch_plan_run", bt.plan, None)
+            capi.call("ya_device_synchronize")
+            got_grid[steps] = dc.download()
+            got_pers[steps] = np.concatenate(bt.results())
+        for steps in (3, 8):
+            np.testing.assert_array_equal(got_grid[steps], got_grid[0])
+            np.testing.assert_array_equal(got_pers[steps], got_pers[0])
+        w = n * L
+        for k in (0, int(rng.integers(1, count - 1)), count - 1):
+            np.testing.assert_array_equal(got_grid[0][k * w:(k + 1) * w],
+                                          orc.polymul(a[k * w:(k + 1) * w], b[k * w:(k + 1) * w], n, L))
+        for k in (0, 127):
+            _, _, ha, hb = host[k]
+            np.testing.assert_array_equal(got_pers[0][k * w:(k + 1) * w], orc.polymul(ha, hb, n, L))
+        assert bt.faults() == 0
+        bt.close()
+    finally:
+        capi.call("ya_batch_force_form", 0)
+        for lg, v in saved.items():
+            capi.call("ya_tune_set", 5, lg, v)
+    assert lib.ya_tune_set(5, 14, 9) != 0 and lib.ya_tune_set(5, 14, -1) != 0
+
+
 def test_autotune_runs_and_fills_table():
     """Every timed shape of 2^12 and 2^13 reports a time: batched radix 8 / 16,
     fused radix 8 / 16 / 32 and the fused two-sequence form at radix 8 / 16;

@@ -257,8 +257,10 @@ __device__ __forceinline__ uint32_t claim_item(const BatchSegs &segs, BatchQueue
 
 __global__ void __launch_bounds__(kBatchThreads)
 k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
-                const BatchRes *__restrict__ res, BatchQueue *q, uint64_t *__restrict__ scratch) {
+                const BatchRes *__restrict__ res, BatchQueue *q, uint64_t *__restrict__ scratch,
+                uint32_t stagger) {
   __shared__ uint64_t lds[LdsWords<14>::value];
+  stagger_start(stagger);
   // claim-ahead: item i is read from s_item[i & 1] while thread 0 claims
   // item i + 1 into the other slot, so the queue atomic's round trip overlaps
   // the work and one barrier per item remains
@@ -447,8 +449,14 @@ int build_batch(DeviceState *st, const ya_polymul_job *jobs, size_t njobs,
 
 hipError_t launch_batch(const BatchItem *items, const BatchSegs &segs, const BatchRes *res,
                         BatchQueue *q, uint64_t *scratch, unsigned grid, hipStream_t s) {
+  // every workgroup is resident (one per CU): the whole grid is the first
+  // wave, staggered when each workgroup has several items to run
+  const int steps = tuned_stagger(kBatchMaxLog);
+  const uint32_t stagger = (steps > 0 && segs.begin[kSegs] >= 4u * grid && grid <= 0xffffffu)
+                               ? (grid << 8) | (uint32_t)steps
+                               : 0u;
   hipLaunchKernelGGL(k_batch_polymul, dim3(grid), dim3(kBatchThreads), 0, s, items, segs, res, q,
-                     scratch);
+                     scratch, stagger);
   return hipGetLastError();
 }
 
@@ -746,6 +754,7 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
       if (cd.persistent) return ya_batch_plan_run(cd.pl, s);
       NttShape sh{cd.le};
       sh.dual = cd.dual;
+      sh.stagger = tuned_stagger(lg);
       hipError_t ee = launch_polymul(c, a, b, desc_ptr(st, lg, 0), lg, njobs * L, L, sh, s);
       return ee == hipSuccess ? YA_OK : hip_fail(ee, "tune fused launch");
     };
@@ -792,9 +801,10 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
 }
 
 extern "C" int ya_tune_get(int kind, int logn) {
-  if (kind < 0 || kind > 4 || logn < 0 || logn > kMaxLogN) return -1;
+  if (kind < 0 || kind > 5 || logn < 0 || logn > kMaxLogN) return -1;
   if (kind == 3) return tuned_shape(logn, 1).dual;
   if (kind == 4) return tuned_mac_block_log(logn);
+  if (kind == 5) return tuned_stagger(logn);
   return tuned_shape(logn, kind).le;
 }
 
@@ -805,6 +815,10 @@ extern "C" int ya_tune_set(int kind, int logn, int le) {
   }
   if (kind == 4 && logn >= 0 && logn <= kMaxLogN && (le == 14 || (le == 13 && logn <= 18))) {
     set_tuned_mac_block_log(logn, le);
+    return YA_OK;
+  }
+  if (kind == 5 && logn >= 0 && logn <= kMaxLogN && le >= 0 && le <= kMaxStaggerSteps) {
+    set_tuned_stagger(logn, le);
     return YA_OK;
   }
   if (kind < 0 || kind > 2 || logn < 0 || logn > kMaxLogN || le < 3 || le > 5 || (kind == 2 && le == 5))

@@ -47,6 +47,12 @@ void set_tuned_dual(int logn, int dual);
 // 512-thread workgroups per CU); the result never depends on it.
 int tuned_mac_block_log(int logn);
 void set_tuned_mac_block_log(int logn, int lb);
+// Staggered start of the first wave of workgroups (stagger_start in
+// ya_blocks.hpp): steps of ~0.85 us per group for the fused kernel of degree
+// 2^logn, and for the persistent kernel at logn = 15; 0 = off.
+constexpr int kMaxStaggerSteps = 8;
+int tuned_stagger(int logn);
+void set_tuned_stagger(int logn, int steps);
 
 // Pinned host memory registered for kernel access (ya_host_alloc_pinned
 // allocates through pinned_alloc).  pinned_device_range returns the device

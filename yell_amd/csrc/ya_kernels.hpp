@@ -21,7 +21,8 @@ constexpr int kMaxLogN = 19;
 // Elements per thread (log2) chosen for a block of 2^lb words; 0 = default.
 struct NttShape {
   int le = 4;
-  int dual = 0;  // fused poly-multiplication: both forward transforms in one pass sequence
+  int dual = 0;     // fused poly-multiplication: both forward transforms in one pass sequence
+  int stagger = 0;  // fused poly-multiplication: first-wave stagger steps (stagger_start)
 };
 
 // forward / forward_lazy / backward over `nres` residues of 2^logn words,

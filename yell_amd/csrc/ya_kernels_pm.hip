@@ -1,4 +1,6 @@
 // ya_kernels_pm.hip -- fused RNS poly-multiplication kernel.
+#include <atomic>
+
 #include "ya_batch.hpp"
 #include "ya_blocks.hpp"
 
@@ -50,8 +52,9 @@ __device__ __forceinline__ void polymul_residue(uint64_t *__restrict__ c, const 
 template <int LB, int LE, bool DUAL>
 __global__ void __launch_bounds__(1 << (LB - LE))
 k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
-          const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod) {
+          const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod, uint32_t stagger) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
+  stagger_start(stagger);
   const uint32_t g = gridDim.x;
   const uint32_t v = xcd_order(blockIdx.x, g);
   const uint32_t count = g / nmod;  // polynomials in the launch
@@ -69,8 +72,9 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
 // class); the table is ordered by prime, and the XCD ranges follow it.
 template <int LB, int LE, bool DUAL>
 __global__ void __launch_bounds__(1 << (LB - LE))
-k_polymul_tab(const BatchRes *__restrict__ res, uint32_t nres) {
+k_polymul_tab(const BatchRes *__restrict__ res, uint32_t nres, uint32_t stagger) {
   __shared__ uint64_t lds[LdsWords<LB>::value];
+  stagger_start(stagger);
   const uint32_t v = xcd_order(blockIdx.x, nres);
   const BatchRes r = res[v];
   const ModDesc d = load_desc(r.d);
@@ -111,32 +115,69 @@ namespace {
 template <int LB, int LE>
 constexpr bool kDualShape = LB >= 10 && LE <= 4;
 
+// Compute units of the current device (cached per device).
+int current_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int c = cache[dev].load(std::memory_order_relaxed);
+  if (c <= 0) {
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cache[dev].store(c, std::memory_order_relaxed);
+  }
+  return c;
+}
+
+// Workgroups of `kernel` one CU holds at once.
+template <class K>
+int resident_per_cu(K kernel, unsigned threads) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, (int)threads, 0) != hipSuccess || n <= 0) n = 1;
+  return n;
+}
+
+// stagger_start's word for a launch of nres workgroups, `per_cu` resident per
+// CU: the whole first wave waits in steps; only a launch of at least four
+// waves gains more than the wait costs.
+uint32_t stagger_word(int steps, int per_cu, size_t nres) {
+  if (steps <= 0) return 0;
+  const size_t first = (size_t)current_cus() * (size_t)per_cu;
+  if (nres < 4 * first || first > 0xffffff) return 0;
+  return (uint32_t)(first << 8) | (uint32_t)(steps > 255 ? 255 : steps);
+}
+
 template <int LB, int LE>
 hipError_t polymul_lb(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
-                      size_t nres, size_t nmod, bool dual, hipStream_t st) {
+                      size_t nres, size_t nmod, NttShape shape, hipStream_t st) {
+  constexpr unsigned kThreads = 1u << (LB - LE);
   if constexpr (kDualShape<LB, LE>) {
-    if (dual) {
-      hipLaunchKernelGGL((k_polymul<LB, LE, true>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c,
-                         a, b, descs, (uint32_t)nmod);
+    if (shape.dual) {
+      static const int per_cu = resident_per_cu(k_polymul<LB, LE, true>, kThreads);
+      hipLaunchKernelGGL((k_polymul<LB, LE, true>), dim3((unsigned)nres), dim3(kThreads), 0, st, c, a, b, descs,
+                         (uint32_t)nmod, stagger_word(shape.stagger, per_cu, nres));
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((k_polymul<LB, LE, false>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, c, a,
-                     b, descs, (uint32_t)nmod);
+  static const int per_cu = resident_per_cu(k_polymul<LB, LE, false>, kThreads);
+  hipLaunchKernelGGL((k_polymul<LB, LE, false>), dim3((unsigned)nres), dim3(kThreads), 0, st, c, a, b, descs,
+                     (uint32_t)nmod, stagger_word(shape.stagger, per_cu, nres));
   return hipGetLastError();
 }
 
 template <int LB, int LE>
-hipError_t polymul_tab_lb(const BatchRes *res, size_t nres, bool dual, hipStream_t st) {
+hipError_t polymul_tab_lb(const BatchRes *res, size_t nres, NttShape shape, hipStream_t st) {
+  constexpr unsigned kThreads = 1u << (LB - LE);
   if constexpr (kDualShape<LB, LE>) {
-    if (dual) {
-      hipLaunchKernelGGL((k_polymul_tab<LB, LE, true>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st,
-                         res, (uint32_t)nres);
+    if (shape.dual) {
+      static const int per_cu = resident_per_cu(k_polymul_tab<LB, LE, true>, kThreads);
+      hipLaunchKernelGGL((k_polymul_tab<LB, LE, true>), dim3((unsigned)nres), dim3(kThreads), 0, st, res,
+                         (uint32_t)nres, stagger_word(shape.stagger, per_cu, nres));
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((k_polymul_tab<LB, LE, false>), dim3((unsigned)nres), dim3(1u << (LB - LE)), 0, st, res,
-                     (uint32_t)nres);
+  static const int per_cu = resident_per_cu(k_polymul_tab<LB, LE, false>, kThreads);
+  hipLaunchKernelGGL((k_polymul_tab<LB, LE, false>), dim3((unsigned)nres), dim3(kThreads), 0, st, res,
+                     (uint32_t)nres, stagger_word(shape.stagger, per_cu, nres));
   return hipGetLastError();
 }
 
@@ -147,7 +188,7 @@ hipError_t launch_polymul_table(const BatchRes *res, size_t nres, int logn, NttS
   if (logn < kMinLogN || logn > kMaxBlockLog || nres == 0 || nres > 0x7fffffff)
     return hipErrorInvalidValue;
   const int le = clamp_le(logn, shape.le);
-#define YA_PMT_CALL(LB, LE) polymul_tab_lb<LB, LE>(res, nres, shape.dual != 0, st)
+#define YA_PMT_CALL(LB, LE) polymul_tab_lb<LB, LE>(res, nres, shape, st)
   YA_LB_SWITCH(logn, le, YA_PMT_CALL)
 #undef YA_PMT_CALL
 }
@@ -164,7 +205,7 @@ hipError_t launch_polymul(uint64_t *c, const uint64_t *a, const uint64_t *b, con
     return hipGetLastError();
   }
   const int le = clamp_le(logn, shape.le);
-#define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, nmod, shape.dual != 0, st)
+#define YA_PM_CALL(LB, LE) polymul_lb<LB, LE>(c, a, b, descs, nres, nmod, shape, st)
   YA_LB_SWITCH(logn, le, YA_PM_CALL)
 #undef YA_PM_CALL
 }

@@ -11,6 +11,7 @@ namespace ya {
 static std::atomic<int> g_le[3][kMaxLogN + 1];
 static std::atomic<int> g_dual[kMaxLogN + 1];  // fused kernel: dual-forward form
 static std::atomic<int> g_mac_lb[kMaxLogN + 1];  // large-ring MAC: pair-product block log
+static std::atomic<int> g_stagger[kMaxLogN + 1];  // first-wave stagger steps
 static std::once_flag g_init;
 
 // Defaults, set once; a thread that arrives while another sets them waits.
@@ -21,6 +22,9 @@ static void init_defaults() {
     // dual-forward fused form: measured faster at every class it is built for
     for (int l = 0; l <= kMaxLogN; ++l) g_dual[l].store(l >= 10 ? 1 : 0);
     for (int l = 0; l <= kMaxLogN; ++l) g_mac_lb[l].store(14);
+    // measured faster where one workgroup fills a CU: the fused kernel at
+    // 2^14 and the persistent kernel (its 2^15 entry)
+    for (int l = 0; l <= kMaxLogN; ++l) g_stagger[l].store(l == 14 || l == 15 ? 3 : 0);
   });
 }
 
@@ -35,11 +39,24 @@ void set_tuned_mac_block_log(int logn, int lb) {
   if (logn >= 0 && logn <= kMaxLogN) g_mac_lb[logn].store(lb);
 }
 
+int tuned_stagger(int logn) {
+  init_defaults();
+  return logn >= 0 && logn <= kMaxLogN ? g_stagger[logn].load() : 0;
+}
+
+void set_tuned_stagger(int logn, int steps) {
+  init_defaults();
+  if (logn >= 0 && logn <= kMaxLogN) g_stagger[logn].store(steps);
+}
+
 NttShape tuned_shape(int logn, int kind) {
   init_defaults();
   NttShape s;
   if (logn >= 0 && logn <= kMaxLogN && kind >= 0 && kind < 3) s.le = g_le[kind][logn].load();
-  if (kind == 1 && logn >= 0 && logn <= kMaxLogN) s.dual = g_dual[logn].load();
+  if (kind == 1 && logn >= 0 && logn <= kMaxLogN) {
+    s.dual = g_dual[logn].load();
+    s.stagger = g_stagger[logn].load();
+  }
   return s;
 }
 
