@@ -186,18 +186,18 @@ def test_every_pass_shape_gives_the_same_words():
 
 def test_staggered_start_gives_the_same_words():
     """The first wave's staggered start (tune kind 5) only delays workgroups:
-    at 0, 3 and 8 steps the fused grid kernel (1,280 residues of 2^14, five
-    waves) and the persistent kernel (1,024 items of 2^14, four per
+    at 0, 3 and 8 steps the fused grid kernel (2,560 residues of 2^14, ten
+    waves) and the persistent kernel (2,560 items of 2^14, ten per
     workgroup) give the same words, and sampled polys the oracle's."""
     lib = device()
-    n, L, count = 1 << 14, 8, 160
+    n, L, count = 1 << 14, 8, 320
     a = np.concatenate([capi.sample_uniform(n, L, 7100 + k) for k in range(count)])
     b = np.concatenate([capi.sample_uniform(n, L, 7300 + k) for k in range(count)])
     da, db = capi.DeviceBuffer.from_host(a), capi.DeviceBuffer.from_host(b)
     dc = capi.DeviceBuffer(n * L * count)
     rng = np.random.default_rng(13)
     host = [(n, L, capi.sample_uniform(n, L, 3000 + 2 * k), capi.sample_uniform(n, L, 3001 + 2 * k))
-            for k in range(128)]
+            for k in range(count)]
     saved = {lg: lib.ya_tune_get(5, lg) for lg in (14, 15)}
     capi.call("ya_batch_force_form", 1)
     try:
@@ -220,7 +220,7 @@ def test_staggered_start_gives_the_same_words():
         for k in (0, int(rng.integers(1, count - 1)), count - 1):
             np.testing.assert_array_equal(got_grid[0][k * w:(k + 1) * w],
                                           orc.polymul(a[k * w:(k + 1) * w], b[k * w:(k + 1) * w], n, L))
-        for k in (0, 127):
+        for k in (0, count - 1):
             _, _, ha, hb = host[k]
             np.testing.assert_array_equal(got_pers[0][k * w:(k + 1) * w], orc.polymul(ha, hb, n, L))
         assert bt.faults() == 0

@@ -450,9 +450,9 @@ int build_batch(DeviceState *st, const ya_polymul_job *jobs, size_t njobs,
 hipError_t launch_batch(const BatchItem *items, const BatchSegs &segs, const BatchRes *res,
                         BatchQueue *q, uint64_t *scratch, unsigned grid, hipStream_t s) {
   // every workgroup is resident (one per CU): the whole grid is the first
-  // wave, staggered when each workgroup has several items to run
+  // wave, staggered when each workgroup has at least 8 items to run
   const int steps = tuned_stagger(kBatchMaxLog);
-  const uint32_t stagger = (steps > 0 && segs.begin[kSegs] >= 4u * grid && grid <= 0xffffffu)
+  const uint32_t stagger = (steps > 0 && segs.begin[kSegs] >= 8u * grid && grid <= 0xffffffu)
                                ? (grid << 8) | (uint32_t)steps
                                : 0u;
   hipLaunchKernelGGL(k_batch_polymul, dim3(grid), dim3(kBatchThreads), 0, s, items, segs, res, q,
