@@ -328,12 +328,12 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
  * dual-forward form (both operands' forward transforms in one pass sequence
  * with shared twiddles; radix 8 / 16 at 2^10 and up), value 0 or 1.  kind 4:
  * the block size (log2: 14, or 13 up to degree 2^18) of the pair-product pass
- * of ya_poly_add_products_backward.  kind 5: staggered start, 0..8 steps of
+ * of ya_poly_add_products_backward.  kind 5: staggered start, 0..16 steps of
  * ~0.85 us (0 = off), of the fused kernel at degree 2^logn and, at logn = 15,
  * of the batched (persistent) kernel: in a launch of at least eight waves of
  * workgroups, the first wave's workgroups wait (group mod 16) steps before
  * they start, so the CUs' operand loads do not all fall together; default 3
- * at 2^14 and 15, 0 elsewhere.  No kind changes a result.
+ * at 2^14, 8 at 15, 0 elsewhere.  No kind changes a result.
  * ya_tune_get returns the entry's value, or -1. */
 int ya_tune_get(int kind, int logn);
 int ya_tune_set(int kind, int logn, int le);

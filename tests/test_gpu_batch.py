@@ -229,7 +229,7 @@ def test_staggered_start_gives_the_same_words():
         capi.call("ya_batch_force_form", 0)
         for lg, v in saved.items():
             capi.call("ya_tune_set", 5, lg, v)
-    assert lib.ya_tune_set(5, 14, 9) != 0 and lib.ya_tune_set(5, 14, -1) != 0
+    assert lib.ya_tune_set(5, 14, 17) != 0 and lib.ya_tune_set(5, 14, -1) != 0
 
 
 def test_autotune_runs_and_fills_table():

@@ -50,7 +50,7 @@ void set_tuned_mac_block_log(int logn, int lb);
 // Staggered start of the first wave of workgroups (stagger_start in
 // ya_blocks.hpp): steps of ~0.85 us per group for the fused kernel of degree
 // 2^logn, and for the persistent kernel at logn = 15; 0 = off.
-constexpr int kMaxStaggerSteps = 8;
+constexpr int kMaxStaggerSteps = 16;
 int tuned_stagger(int logn);
 void set_tuned_stagger(int logn, int steps);
 
