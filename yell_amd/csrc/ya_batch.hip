@@ -655,8 +655,9 @@ int ya_batch_polymul(const ya_polymul_job *jobs, size_t njobs, void *stream) {
 // ===========================================================================
 // Autotuner.  For each degree class it times every instantiated pass shape
 // (LE = 3: radix-8 register passes and twice as many residues per workgroup
-// tile; LE = 4: radix-16 passes) on zero-filled synthetic operands sized to
-// keep every CU busy for several items, and keeps the faster one in the
+// tile; LE = 4: radix-16 passes) on zero-filled synthetic operands of
+// 16 x 2^14 words per CU (3 x 512 MiB on 256 CUs, freed before it returns),
+// and keeps the faster one in the
 // launch-shape table used by later plans and fused launches.  Timing is
 // data-independent (the butterflies are branch-free); each candidate's time
 // is its best of kTuneRounds single launches, taken round-robin over the
@@ -703,7 +704,10 @@ extern "C" int ya_autotune(int logn_lo, int logn_hi, double *report) {
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const size_t L = 8;
   if ((rc = ensure_tables(st, 14, 0, L))) return rc;
-  const size_t words = (size_t)cus * 4 * (1u << 14);  // 4 full items per CU at 2^14
+  // 16 full items per CU at 2^14 (the size of a 512-poly batch of 8
+  // moduli): the persistent kernel's start-up and the staggered start are
+  // then weighed as in a throughput batch
+  const size_t words = (size_t)cus * 16 * (1u << 14);
   uint64_t *buf = nullptr;
   hipStream_t s = nullptr;
   hipError_t e = hipMalloc(&buf, 3 * words * 8);
