@@ -137,9 +137,14 @@ int split_choose_k(int logn, size_t nres, int cus) {
   return k;
 }
 
+// 2^12 blocks at 2^15 and 2^16 (twice the workgroups of 2^13 blocks: 2^15 x 24
+// primes 0.0535 -> 0.0428 ms, 2^16 x 24 0.0621 -> 0.0606, 2^16 x 4 0.0485 ->
+// 0.0382; 2^11 blocks were slower again), 2^13 blocks up to 2^18 (at 2^17 x 12
+// 2^12 and 2^13 blocks measured the same), 2^14 blocks at 2^19 (the column
+// kernels take at most 5 index bits).
 int large_ring_split_k(int logn) {
   if (logn <= kMaxBlockLog || logn > kMaxLogN) return 0;
-  return logn <= 18 ? logn - 13 : logn - 14;
+  return logn <= 16 ? logn - 12 : logn <= 18 ? logn - 13 : logn - 14;
 }
 
 hipError_t launch_polymul_split(const SplitArgs &s, int logn, size_t nres, int k, hipStream_t st) {
@@ -153,7 +158,7 @@ hipError_t launch_polymul_split(const SplitArgs &s, int logn, size_t nres, int k
     YA_SPLIT(11, 2) YA_SPLIT(11, 3)
     YA_SPLIT(12, 2)
     // rings above 2^14 (large_ring_split_k)
-    YA_SPLIT(13, 2) YA_SPLIT(13, 3) YA_SPLIT(13, 4) YA_SPLIT(13, 5)
+    YA_SPLIT(12, 3) YA_SPLIT(12, 4) YA_SPLIT(13, 4) YA_SPLIT(13, 5)
     YA_SPLIT(14, 5)
     default: return hipErrorInvalidValue;
   }
