@@ -224,7 +224,7 @@ def c4_bench(stream, ev, steps):
     the NTT domain, one inverse NTT (17 * N * L * 8 algorithmic bytes).  The
     job is ya_poly_sum_products_backward (a fresh accumulator: the pair-product
     pass and the summing inverse, which writes the result without reading it;
-    both block sizes of the pair pass timed, the faster kept); beside it
+    the pair pass timed at 2^14, 2^13 and 2^12 blocks, the fastest kept); beside it
     ya_poly_add_products_backward after a zero fill of acc (the same words,
     checked) and the reference's sequence as separate launches (forward of the
     16 operands, the MAC, the inverse)."""
@@ -254,7 +254,7 @@ def c4_bench(stream, ev, steps):
 
     lg = n.bit_length() - 1
     by_lb = {}
-    for lb in (14, 13):
+    for lb in (14, 13, 12):
         capi.call("ya_tune_set", 4, lg, lb)
         fresh()
         capi.call("ya_stream_synchronize", stream)

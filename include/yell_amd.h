@@ -327,8 +327,9 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
  * radix (3, 4, or 5 for kinds 0 and 1).  kind 3: the fused kernel's
  * dual-forward form (both operands' forward transforms in one pass sequence
  * with shared twiddles; radix 8 / 16 at 2^10 and up), value 0 or 1.  kind 4:
- * the block size (log2: 14, or 13 up to degree 2^18) of the pair-product pass
- * of ya_poly_add_products_backward.  kind 5: staggered start, 0..16 steps of
+ * the block size (log2: 14, 13 up to degree 2^18, or 12 up to 2^17; default
+ * 12 from 2^15 to 2^17, 13 at 2^18, 14 otherwise) of the pair-product pass
+ * of ya_poly_add_products_backward / ya_poly_sum_products_backward.  kind 5: staggered start, 0..16 steps of
  * ~0.85 us (0 = off), of the fused kernel at degree 2^logn and, at logn = 15,
  * of the batched (persistent) kernel: in a launch of at least eight waves of
  * workgroups, the first wave's workgroups wait (group mod 16) steps before

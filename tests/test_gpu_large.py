@@ -162,7 +162,8 @@ def _mac_want(acc, a, b, n, L):
 
 
 @pytest.mark.parametrize("logn,L,npairs,lb", [(12, 2, 3, 14), (13, 3, 3, 14), (14, 2, 17, 14), (16, 5, 4, 14),
-                                              (16, 5, 4, 13), (18, 2, 2, 13), (19, 1, 2, 14)])
+                                              (16, 5, 4, 13), (16, 5, 4, 12), (15, 3, 17, 12), (17, 2, 2, 12),
+                                              (18, 2, 2, 13), (19, 1, 2, 14)])
 def test_add_products_backward(logn, L, npairs, lb):
     """ya_poly_add_products_backward = the reference's sequence a_j.forward();
     b_j.forward(); acc.add_product_of(a_j, b_j) for every pair, then
@@ -189,7 +190,7 @@ def test_add_products_backward(logn, L, npairs, lb):
         capi.call("ya_poly_add_products_backward", dacc.ptr, pa, pb, npairs, n, L, None)
         np.testing.assert_array_equal(dacc.download(), _mac_want(acc, a, b, n, L))
     finally:
-        capi.call("ya_tune_set", 4, logn, old if old in (13, 14) else 14)
+        capi.call("ya_tune_set", 4, logn, old if old in (12, 13, 14) else 14)
 
 
 def test_add_products_backward_c4_and_rejects_overlap():
@@ -215,7 +216,7 @@ def test_add_products_backward_c4_and_rejects_overlap():
 
 
 @pytest.mark.parametrize("logn,L,npairs,lb", [(12, 2, 3, 14), (13, 3, 1, 14), (14, 2, 17, 14), (16, 5, 4, 13),
-                                              (16, 24, 8, 13), (16, 3, 33, 13), (19, 1, 2, 14)])
+                                              (16, 24, 8, 13), (16, 24, 8, 12), (16, 3, 33, 12), (19, 1, 2, 14)])
 def test_sum_products_backward(logn, L, npairs, lb):
     """ya_poly_sum_products_backward = the reference's sequence on a fresh
     (zero-filled) accumulator: below 2^13 the first pair's product written by
@@ -244,4 +245,4 @@ def test_sum_products_backward(logn, L, npairs, lb):
         pb[0] = dout.ptr
         assert lib.ya_poly_sum_products_backward(dout.ptr, pa, pb, npairs, n, L, None) == 1
     finally:
-        capi.call("ya_tune_set", 4, logn, old if old in (13, 14) else 14)
+        capi.call("ya_tune_set", 4, logn, old if old in (12, 13, 14) else 14)

@@ -817,7 +817,8 @@ extern "C" int ya_tune_set(int kind, int logn, int le) {
     set_tuned_dual(logn, le);
     return YA_OK;
   }
-  if (kind == 4 && logn >= 0 && logn <= kMaxLogN && (le == 14 || (le == 13 && logn <= 18))) {
+  if (kind == 4 && logn >= 0 && logn <= kMaxLogN &&
+      (le == 14 || (le == 13 && logn <= 18) || (le == 12 && logn <= 17))) {
     set_tuned_mac_block_log(logn, le);
     return YA_OK;
   }

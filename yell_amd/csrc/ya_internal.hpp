@@ -43,8 +43,9 @@ void set_tuned_shape(int logn, int kind, NttShape s);  // sets le only
 // Fused poly-multiplication of degree 2^logn: dual-forward form on/off.
 void set_tuned_dual(int logn, int dual);
 // Large-ring MAC (ya_poly_add_products_backward) above 2^13: block log of
-// its pair-product pass, 14 (one 1024-thread workgroup per CU) or 13 (two
-// 512-thread workgroups per CU); the result never depends on it.
+// its pair-product pass, 14 (one 1024-thread workgroup per CU), 13 (two
+// 512-thread workgroups) or 12 (four 256-thread workgroups); the result
+// never depends on it.
 int tuned_mac_block_log(int logn);
 void set_tuned_mac_block_log(int logn, int lb);
 // Staggered start of the first wave of workgroups (stagger_start in

@@ -651,14 +651,16 @@ hipError_t launch_pair_products(const PairPtrs &pp, int npairs, const ModDesc *d
     return hipErrorInvalidValue;
   if (((nmod * npairs) << (logn - kMinPairLog)) > 0x7fffffffu) return hipErrorInvalidValue;
   if (logn <= lb) lb = logn;
-  if ((lb != 13 && lb != 14) || logn - lb > 5) return hipErrorInvalidValue;
+  if ((lb != 12 && lb != 13 && lb != 14) || logn - lb > 5) return hipErrorInvalidValue;
   if (logn > lb) {
-    const hipError_t e = lb == 13 ? fwd_top_pairs_b<13>(pp, npairs, descs, logn - lb, nmod, st)
-                                  : fwd_top_pairs_b<14>(pp, npairs, descs, logn - lb, nmod, st);
+    const hipError_t e = lb == 12 ? fwd_top_pairs_b<12>(pp, npairs, descs, logn - lb, nmod, st)
+                         : lb == 13 ? fwd_top_pairs_b<13>(pp, npairs, descs, logn - lb, nmod, st)
+                                    : fwd_top_pairs_b<14>(pp, npairs, descs, logn - lb, nmod, st);
     if (e != hipSuccess) return e;
   }
-  return lb == 13 ? pair_product_lb<13>(pp, npairs, descs, logn, nmod, st)
-                  : pair_product_lb<14>(pp, npairs, descs, logn, nmod, st);
+  return lb == 12 ? pair_product_lb<12>(pp, npairs, descs, logn, nmod, st)
+         : lb == 13 ? pair_product_lb<13>(pp, npairs, descs, logn, nmod, st)
+                    : pair_product_lb<14>(pp, npairs, descs, logn, nmod, st);
 }
 
 template <int LB, int TOP, int LE = 4>

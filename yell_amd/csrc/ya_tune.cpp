@@ -21,7 +21,12 @@ static void init_defaults() {
       for (int l = 0; l <= kMaxLogN; ++l) g_le[k][l].store(4);
     // dual-forward fused form: measured faster at every class it is built for
     for (int l = 0; l <= kMaxLogN; ++l) g_dual[l].store(l >= 10 ? 1 : 0);
-    for (int l = 0; l <= kMaxLogN; ++l) g_mac_lb[l].store(14);
+    // large-ring MAC pair pass: 2^12 blocks where the top pass takes the
+    // rest (four 256-thread workgroups per CU; 2^16 x 24 primes, 8 pairs:
+    // 0.206 ms against 0.230 with 2^13 and 0.266 with 2^14 blocks), 2^13
+    // at 2^18, 2^14 at 2^19 (at most 5 top layers); up to 2^14 the ring is
+    // one block
+    for (int l = 0; l <= kMaxLogN; ++l) g_mac_lb[l].store(l <= 14 ? 14 : l <= 17 ? 12 : l == 18 ? 13 : 14);
     // measured faster where one workgroup fills a CU: the fused kernel at
     // 2^14 (best at 1..3 steps) and the persistent kernel (its 2^15 entry;
     // best at 8 steps on the uneven 1024-job batch)
