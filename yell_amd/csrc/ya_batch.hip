@@ -28,17 +28,7 @@
 namespace ya {
 
 // Barriers of the sub-groups.  A sub-group of one wave or less needs no
-// barrier between its passes: a wave's LDS accesses are performed in issue
-// order, so the wave waits for its own LDS operations and the compiler is
-// kept from moving LDS accesses across the point.
-struct WaveSync {
-  __device__ __forceinline__ void operator()() const {
-    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
-};
+// barrier between its passes (WaveSync, ya_blocks.hpp).
 
 // A sub-group of 2..8 waves: a sense-reversing barrier in LDS, so that the
 // sub-groups of a workgroup run their passes independently (the workgroup's
@@ -79,6 +69,8 @@ struct SubSync {
     sense ^= 1u;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
+  // an exchange inside each wave (kWaveLocal): the wave's own order only
+  __device__ __forceinline__ void local() const { WaveSync()(); }
 };
 
 // The fused poly-multiplication of one sub-group's residue (sub-group `sub`

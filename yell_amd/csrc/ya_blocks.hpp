@@ -66,10 +66,43 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g) {
 // Barriers between the passes of a block transform.  WgSync is the
 // workgroup's barrier; a transform whose threads are a part of a larger
 // workgroup (the persistent batch kernel's sub-groups) passes a barrier of
-// its own threads instead (ya_batch.hip).
+// its own threads instead (ya_batch.hip).  Each also has local(), enough for
+// an exchange in which every wave reads back only words it wrote itself
+// (kWaveLocal below): a wave's LDS accesses are performed in issue order, so
+// the wave waits for its own LDS operations and the compiler is kept from
+// moving LDS accesses across the point.
+struct WaveSync {
+  __device__ __forceinline__ void operator()() const {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __device__ __forceinline__ void local() const { (*this)(); }
+};
+
 struct WgSync {
   __device__ __forceinline__ void operator()() const { __syncthreads(); }
+  __device__ __forceinline__ void local() const { WaveSync()(); }
 };
+
+// An exchange from a pass over window [WA, WA+LE) to one over [WB, WB+LE)
+// of a block of 2^LB words held by 2^(LB-LE) threads, 64 to a wave: with
+// the window at bit 6 or below, wave w holds exactly the 2^(6+LE) words
+// whose bits [6+LE, LB) are w (its lanes and registers cover bits
+// [0, 6+LE)), so if both windows start at or below bit 6 every word a wave
+// reads back in the exchange is one it stored.
+template <int LB, int LE, int WA, int WB>
+constexpr bool kWaveLocal = (LB - LE >= 6) && WA <= 6 && WB <= 6;
+
+template <bool LOCAL, class S>
+__device__ __forceinline__ void xsync(S &sync) {
+  if constexpr (LOCAL) {
+    sync.local();
+  } else {
+    sync();
+  }
+}
 
 // Window start of the first forward pass of a 2^LB block.
 template <int LB, int LE>
@@ -99,9 +132,11 @@ __device__ __forceinline__ void fwd_block(W (&x)[1 << LE], const typename Ident<
     const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
     fwd_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, p);
     if constexpr (P + 1 < NP) {
-      sync();
+      constexpr int HI2 = LB - (P + 1) * LE;
+      constexpr bool LOC = kWaveLocal<LB, LE, WLO, (HI2 - LE > 0 ? HI2 - LE : 0)>;
+      xsync<LOC>(sync);
       lds_store<WLO, LE>(lds, x, t);
-      sync();
+      xsync<LOC>(sync);
     }
   });
 }
@@ -125,13 +160,14 @@ __device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE],
     if constexpr (P + 1 < NP) {
       constexpr int HI2 = LB - (P + 1) * LE;
       constexpr int WLO2 = HI2 - LE > 0 ? HI2 - LE : 0;
-      sync();
+      constexpr bool LOC = kWaveLocal<LB, LE, WLO, WLO2>;
+      xsync<LOC>(sync);
       lds_store<WLO, LE>(lds, x, t);
-      sync();
+      xsync<LOC>(sync);
       lds_load<WLO2, LE>(lds, x, t);
-      sync();
+      xsync<LOC>(sync);
       lds_store<WLO, LE>(lds, y, t);
-      sync();
+      xsync<LOC>(sync);
       lds_load<WLO2, LE>(lds, y, t);
     }
   });
@@ -146,6 +182,22 @@ __device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE],
 //               lib/ntt.cpp:192), outputs in [0, 2p).
 enum { TOP_GS = 0, TOP_MERGED = 1, TOP_SKIP = 2 };
 
+// Pass Q of the inverse of a 2^LB block, layers [lo, hi) in registers over
+// window [wlo, wlo + LE): the last pass runs the top LE layers
+// [LB - LE, LB); the ones before run [0, LB - LE) upwards in chunks of LE,
+// the last chunk short, its window ending at its top layer (so that the
+// exchanges up to the last pass stay inside each wave: kWaveLocal).  The
+// layers run in the same order as one pass per layer.
+template <int LB, int LE, int Q>
+struct InvPass {
+  static constexpr int np = (LB + LE - 1) / LE;
+  static constexpr int top = LB - LE > 0 ? LB - LE : 0;
+  static constexpr bool last = Q + 1 == np;
+  static constexpr int lo = last ? top : Q * LE;
+  static constexpr int hi = last ? LB : (Q * LE + LE < top ? Q * LE + LE : top);
+  static constexpr int wlo = last ? top : (hi - LE > 0 ? hi - LE : 0);
+};
+
 template <int LB, int LE, int TOP, class W, class TW, class D, class S = WgSync>
 __device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::type *__restrict__ dst,
                                           W *lds, const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
@@ -153,10 +205,10 @@ __device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::ty
   constexpr int NP = (LB + LE - 1) / LE;
   static_for<0, NP>([&](auto QC) {
     constexpr int Q = decltype(QC)::value;
-    constexpr int LO = Q * LE;
-    constexpr int HI = LO + LE < LB ? LO + LE : LB;
-    constexpr int WLO = LO + LE <= LB ? LO : LB - LE;
     constexpr bool LASTQ = (Q + 1 == NP);
+    constexpr int LO = InvPass<LB, LE, Q>::lo;
+    constexpr int HI = InvPass<LB, LE, Q>::hi;
+    constexpr int WLO = InvPass<LB, LE, Q>::wlo;
     if constexpr (Q > 0) lds_load<WLO, LE>(lds, x, t);
     const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
     if constexpr (LASTQ && TOP == TOP_MERGED) {
@@ -168,9 +220,10 @@ __device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::ty
       inv_layers<WLO, LE, HI, LO>(x, tw, nlog, gbase, d.p);
     }
     if constexpr (!LASTQ) {
-      sync();
+      constexpr bool LOC = kWaveLocal<LB, LE, WLO, InvPass<LB, LE, Q + 1>::wlo>;
+      xsync<LOC>(sync);
       lds_store<WLO, LE>(lds, x, t);
-      sync();
+      xsync<LOC>(sync);
     } else if (dst != nullptr) {  // null: a sub-group with no residue of its own
       gmem_store<WLO, LE>(dst, x, t);
     }
