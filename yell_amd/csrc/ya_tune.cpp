@@ -29,8 +29,8 @@ static void init_defaults() {
     for (int l = 0; l <= kMaxLogN; ++l) g_mac_lb[l].store(l <= 14 ? 14 : l <= 17 ? 12 : l == 18 ? 13 : 14);
     // measured faster where one workgroup fills a CU: the fused kernel at
     // 2^14 (best at 1..3 steps) and the persistent kernel (its 2^15 entry;
-    // best at 8 steps on the uneven 1024-job batch)
-    for (int l = 0; l <= kMaxLogN; ++l) g_stagger[l].store(l == 14 ? 3 : l == 15 ? 8 : 0);
+    // best at 6 steps on the uneven 1024-job batch)
+    for (int l = 0; l <= kMaxLogN; ++l) g_stagger[l].store(l == 14 ? 3 : l == 15 ? 6 : 0);
   });
 }
 
