@@ -206,7 +206,8 @@ __device__ __forceinline__ void batch_15(const BatchItem &it, const BatchRes *__
   const uint32_t t2 = phase_v(t);
   gmem_load<W0, LE>(r.c + (1u << 14), x, t2);
   gmem_load<W0, LE>(scr, y, t2);
-  fwd_block_dual<14, LE>(x, y, lds, (const TwC *)d.fwd + phase_zero_s(), 15, 1u << 14, d.p, t2);
+  // the lower half's inverse may still be reading the LDS in other waves
+  fwd_block_dual<14, LE, true>(x, y, lds, (const TwC *)d.fwd + phase_zero_s(), 15, 1u << 14, d.p, t2);
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);

@@ -145,7 +145,10 @@ __device__ __forceinline__ void fwd_block(W (&x)[1 << LE], const typename Ident<
 // with gmem_load<FirstWindow<LB,LE>::value, LE>): per pass the layers of both
 // share their twiddles; the exchanges go through LDS one block after the
 // other.  On return x and y hold window [0, LE), as fwd_block leaves them.
-template <int LB, int LE, class W, class TW, class S = WgSync>
+// The LDS is taken to be free on entry (nothing of the workgroup still reads
+// it: the kernel's or the work item's start) unless ENTRY_SYNC, in which
+// case the first exchange begins with a barrier.
+template <int LB, int LE, bool ENTRY_SYNC = false, class W, class TW, class S = WgSync>
 __device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE], W *lds,
                                                const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
                                                typename Ident<W>::type p, uint32_t t, S &&sync = S()) {
@@ -161,7 +164,7 @@ __device__ __forceinline__ void fwd_block_dual(W (&x)[1 << LE], W (&y)[1 << LE],
       constexpr int HI2 = LB - (P + 1) * LE;
       constexpr int WLO2 = HI2 - LE > 0 ? HI2 - LE : 0;
       constexpr bool LOC = kWaveLocal<LB, LE, WLO, WLO2>;
-      xsync<LOC>(sync);
+      if constexpr (P > 0 || ENTRY_SYNC) xsync<LOC>(sync);
       lds_store<WLO, LE>(lds, x, t);
       xsync<LOC>(sync);
       lds_load<WLO2, LE>(lds, x, t);
