@@ -224,7 +224,8 @@ def c4_bench(stream, ev, steps):
     the NTT domain, one inverse NTT (17 * N * L * 8 algorithmic bytes).  The
     job is ya_poly_sum_products_backward (a fresh accumulator: the pair-product
     pass and the summing inverse, which writes the result without reading it;
-    the pair pass timed at 2^14, 2^13 and 2^12 blocks, the fastest kept); beside it
+    the pair pass's block size chosen by ya_autotune_mac, which times 2^14,
+    2^13 and 2^12 blocks); beside it
     ya_poly_add_products_backward after a zero fill of acc (the same words,
     checked) and the reference's sequence as separate launches (forward of the
     16 operands, the MAC, the inverse)."""
@@ -253,15 +254,14 @@ def c4_bench(stream, ev, steps):
         capi.call("ya_poly_backward", dacc.ptr, n, L, stream)
 
     lg = n.bit_length() - 1
-    by_lb = {}
-    for lb in (14, 13, 12):
-        capi.call("ya_tune_set", 4, lg, lb)
-        fresh()
-        capi.call("ya_stream_synchronize", stream)
-        by_lb[lb] = timed(ev, stream, fresh, steps)
-    best = min(by_lb, key=by_lb.get)
-    capi.call("ya_tune_set", 4, lg, best)
-    ms = by_lb[best]
+    # the library's tuner picks the pair pass's block size for this shape
+    rep = (ctypes.c_double * 3)()
+    capi.call("ya_autotune_mac", n, L, pairs, rep)
+    tuned = {str(14 - k): round(rep[k], 4) for k in range(3) if rep[k] > 0}
+    best = capi.load().ya_tune_get(4, lg)
+    fresh()
+    capi.call("ya_stream_synchronize", stream)
+    ms = timed(ev, stream, fresh, steps)
     fused()
     capi.call("ya_stream_synchronize", stream)
     ms_acc = timed(ev, stream, fused, steps)
@@ -300,7 +300,7 @@ def c4_bench(stream, ev, steps):
             "jobs_per_s": 1e3 / ms, "algorithmic_bytes": alg,
             "hbm_GBps_algorithmic": alg / (ms / 1e3) / 1e9,
             "entry_point": "ya_poly_sum_products_backward",
-            "ms_by_pair_block_log": {str(k): v for k, v in by_lb.items()}, "pair_block_log": best,
+            "ms_by_pair_block_log_tuner": tuned, "pair_block_log": best,
             "accumulate_form": {"entry_point": "ya_poly_add_products_backward (after a zero fill of acc)",
                                 "ms_per_job": ms_acc, "same_words_as_fresh": same},
             "separate_launches": {"ms_per_job": ms_sep, "mac_ms": ms_mac,

@@ -329,8 +329,9 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
  * with shared twiddles; radix 8 / 16 at 2^10 and up), value 0 or 1.  kind 4:
  * the block size (log2: 14, 13 up to degree 2^18, or 12 up to 2^17; default
  * 12 from 2^15 to 2^17, 13 at 2^18, 14 otherwise) of the pair-product pass
- * of ya_poly_add_products_backward / ya_poly_sum_products_backward.  kind 5: staggered start, 0..16 steps of
- * ~0.85 us (0 = off), of the fused kernel at degree 2^logn and, at logn = 15,
+ * of ya_poly_add_products_backward / ya_poly_sum_products_backward.  kind 5:
+ * staggered start, 0..16 steps of ~0.85 us (0 = off), of the fused kernel at
+ * degree 2^logn and, at logn = 15,
  * of the batched (persistent) kernel: in a launch of at least eight waves of
  * workgroups, the first wave's workgroups wait (group mod 16) steps before
  * they start, so the CUs' operand loads do not all fall together; default 3
@@ -338,6 +339,15 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
  * ya_tune_get returns the entry's value, or -1. */
 int ya_tune_get(int kind, int logn);
 int ya_tune_set(int kind, int logn, int le);
+/* Autotuner of the large-ring multiply-accumulate (no counterpart in the
+ * reference): times ya_poly_sum_products_backward at this degree (above
+ * 2^14), nmoduli (up to 65535) and npairs (1..16) on zero operands in its
+ * own device buffers, with each accepted pair-pass block size, and keeps the
+ * fastest as tune kind 4 for the degree.  report (optional) receives 3
+ * doubles: ms per job with 2^14, 2^13, 2^12 blocks (-1 where not accepted).
+ * It rewrites kind 4 while it runs: not concurrently with other calls at
+ * that degree. */
+int ya_autotune_mac(size_t degree, size_t nmoduli, size_t npairs, double *report);
 /* Split launches (no counterpart in the reference).  A poly-multiplication
  * launch of few residues of degree 2^11 .. 2^14 (ya_poly_multiply(_many), the
  * served form's small batches) may spread each residue over 2^k workgroups

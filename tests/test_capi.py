@@ -99,3 +99,14 @@ def test_tune_table_defaults_and_bounds():
     assert lib.ya_tune_set(5, 12, 16) == 0 and lib.ya_tune_get(5, 12) == 16
     assert lib.ya_tune_set(5, 12, 0) == 0 and lib.ya_tune_get(5, 12) == 0
     assert lib.ya_tune_get(6, 14) == -1
+
+
+def test_autotune_mac_rejects_shapes_before_touching_the_device():
+    """ya_autotune_mac refuses a ring of one block, no pairs, more pairs than a
+    pass takes, and more moduli than the pair pass takes, before any GPU call."""
+    lib = capi.load()
+    assert lib.ya_autotune_mac(1 << 14, 3, 2, None) != 0
+    assert lib.ya_autotune_mac(1 << 16, 3, 0, None) != 0
+    assert lib.ya_autotune_mac(1 << 16, 3, 17, None) != 0
+    assert lib.ya_autotune_mac(1 << 16, 65536, 2, None) != 0
+    assert lib.ya_autotune_mac(3 << 14, 3, 2, None) != 0

@@ -246,3 +246,35 @@ def test_sum_products_backward(logn, L, npairs, lb):
         assert lib.ya_poly_sum_products_backward(dout.ptr, pa, pb, npairs, n, L, None) == 1
     finally:
         capi.call("ya_tune_set", 4, logn, old if old in (12, 13, 14) else 14)
+
+
+def test_autotune_mac_times_every_accepted_block_size():
+    """ya_autotune_mac times the job with each accepted pair-pass block size
+    (2^14, 2^13, 2^12 at 2^16; 2^12 refused at 2^18) and keeps one of them;
+    the job then still gives the oracle's words."""
+    lib = device()
+    saved = {lg: lib.ya_tune_get(4, lg) for lg in (16, 18)}
+    try:
+        rep = (ctypes.c_double * 3)()
+        say("autotune_mac 2^16 x 3, 2 pairs")
+        capi.call("ya_autotune_mac", 1 << 16, 3, 2, rep)
+        assert all(rep[k] > 0 for k in range(3))
+        assert lib.ya_tune_get(4, 16) in (12, 13, 14)
+        say("autotune_mac 2^18 x 1, 1 pair")
+        capi.call("ya_autotune_mac", 1 << 18, 1, 1, rep)
+        assert rep[0] > 0 and rep[1] > 0 and rep[2] == -1.0
+        assert lib.ya_tune_get(4, 18) in (13, 14)
+        n, L, npairs = 1 << 16, 3, 2
+        rng = np.random.default_rng(16)
+        a = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+        b = [rand_bound(rng, n, L, 1, P) for _ in range(npairs)]
+        da = [capi.DeviceBuffer.from_host(v) for v in a]
+        db = [capi.DeviceBuffer.from_host(v) for v in b]
+        dout = capi.DeviceBuffer(n * L)
+        pa = (ctypes.c_void_p * npairs)(*[x.ptr for x in da])
+        pb = (ctypes.c_void_p * npairs)(*[x.ptr for x in db])
+        capi.call("ya_poly_sum_products_backward", dout.ptr, pa, pb, npairs, n, L, None)
+        np.testing.assert_array_equal(dout.download(), _mac_want(np.zeros(n * L, dtype=np.uint64), a, b, n, L))
+    finally:
+        for lg, v in saved.items():
+            capi.call("ya_tune_set", 4, lg, v)

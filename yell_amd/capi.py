@@ -139,6 +139,7 @@ _SIGS = {
     "ya_batch_plan_info": (i32, [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz),
                                  ctypes.POINTER(i32)]),
     "ya_batch_force_form": (i32, [i32]),
+    "ya_autotune_mac": (i32, [sz, sz, sz, ctypes.POINTER(ctypes.c_double)]),
     "ya_batch_plan_faults": (i32, [vp, ctypes.POINTER(ctypes.c_uint)]),
     "ya_batch_plan_destroy": (i32, [vp]),
     "ya_batch_polymul": (i32, [vp, sz, vp]),

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -809,6 +810,86 @@ int ya_poly_add_products_backward(uint64_t *acc, uint64_t *const *a, uint64_t *c
 int ya_poly_sum_products_backward(uint64_t *out, uint64_t *const *a, uint64_t *const *b, size_t npairs,
                                   size_t degree, size_t nmoduli, void *stream) {
   return products_backward(out, a, b, npairs, degree, nmoduli, false, stream);
+}
+
+// Times the fresh-accumulator job (products_backward) on zero operands of
+// the given shape with every accepted pair-pass block size -- the words do
+// not change the timing -- after 0.15 s of untimed jobs (the clocks ramp up
+// after an idle period), best of 5 taken round-robin, and keeps the fastest
+// in the tune table (kind 4).
+int ya_autotune_mac(size_t degree, size_t nmoduli, size_t npairs, double *report) {
+  static const int kLogs[3] = {14, 13, 12};
+  if (report)
+    for (int k = 0; k < 3; ++k) report[k] = -1.0;
+  int logn;
+  int rc = check_shape(degree, 0, nmoduli, &logn);
+  if (rc) return rc;
+  if (logn <= kMaxBlockLog) return fail(YA_ERR_INVALID, "degree 2^%d is one block: no pair-pass block size", logn);
+  if (npairs == 0 || npairs > (size_t)kMacPairs || nmoduli > 65535)
+    return fail(YA_ERR_INVALID, "tune MAC: %zu pairs (1..%d), %zu moduli (up to 65535)", npairs, kMacPairs,
+                nmoduli);
+  DeviceState *st;
+  if ((rc = current_device(&st))) return rc;
+  if ((rc = ensure_tables(st, logn, 0, nmoduli))) return rc;
+  const size_t pw = degree * nmoduli;
+  uint64_t *buf = nullptr;
+  hipError_t e = hipMalloc(&buf, (2 * npairs + 1) * pw * 8);
+  if (e != hipSuccess) return fail(YA_ERR_ALLOC, "tune MAC buffers: %s", hipGetErrorString(e));
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  e = hipMemset(buf, 0, (2 * npairs + 1) * pw * 8);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  if (e != hipSuccess) rc = hip_fail(e, "tune MAC set-up");
+  std::vector<uint64_t *> a(npairs), b(npairs);
+  for (size_t j = 0; j < npairs; ++j) {
+    a[j] = buf + 2 * j * pw;
+    b[j] = buf + (2 * j + 1) * pw;
+  }
+  uint64_t *acc = buf + 2 * npairs * pw;
+  const int saved = tuned_mac_block_log(logn);
+  auto run = [&](int lb) {
+    set_tuned_mac_block_log(logn, lb);
+    return products_backward(acc, a.data(), b.data(), npairs, degree, nmoduli, false, s);
+  };
+  int cand[3], nc = 0;
+  for (int lb : kLogs)
+    if (lb == 14 || (lb == 13 && logn <= 18) || (lb == 12 && logn <= 17)) cand[nc++] = lb;
+  float best[3] = {1e30f, 1e30f, 1e30f};
+  if (rc == YA_OK) {
+    const auto t0 = std::chrono::steady_clock::now();
+    do {
+      rc = run(cand[0]);
+      if (rc == YA_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(YA_ERR_LAUNCH, "tune MAC settle");
+    } while (rc == YA_OK && std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < 0.15);
+  }
+  for (int i = 0; rc == YA_OK && i < nc; ++i) rc = run(cand[i]);
+  for (int r = 0; rc == YA_OK && r < 5; ++r)
+    for (int i = 0; rc == YA_OK && i < nc; ++i) {
+      (void)hipEventRecord(ev0, s);
+      rc = run(cand[i]);
+      (void)hipEventRecord(ev1, s);
+      if (hipEventSynchronize(ev1) != hipSuccess) rc = fail(YA_ERR_LAUNCH, "tune MAC sync");
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ev0, ev1);
+      if (rc == YA_OK && ms < best[i]) best[i] = ms;
+    }
+  int arg = saved;
+  if (rc == YA_OK) {
+    int bi = 0;
+    for (int i = 1; i < nc; ++i)
+      if (best[i] < best[bi]) bi = i;
+    arg = cand[bi];
+    if (report)
+      for (int i = 0; i < nc; ++i) report[14 - cand[i]] = best[i];
+  }
+  set_tuned_mac_block_log(logn, arg);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (s) (void)hipStreamDestroy(s);
+  (void)hipFree(buf);
+  return rc;
 }
 
 static int seam_host(uint64_t *x, size_t degree, const uint64_t *w, const uint64_t *ws, uint64_t p,
