@@ -19,7 +19,7 @@ __device__ __forceinline__ void polymul_residue(uint64_t *__restrict__ c, const 
     constexpr int W0 = FirstWindow<LB, LE>::value;
     gmem_load<W0, LE>(a, x, t);
     gmem_load<W0, LE>(b, y, t);
-    fwd_block_dual<LB, LE>(x, y, lds, d.fwd, LB, 0u, d.p, t);
+    fwd_block_dual<LB, LE>(x, y, lds, (const TwC *)d.fwd, LB, 0u, d.p, t);
   } else if constexpr (LE <= 4) {
     // both operands are requested at once (b waits in y during forward(a)),
     // so the second transform does not stall on HBM latency again
@@ -43,7 +43,7 @@ __device__ __forceinline__ void polymul_residue(uint64_t *__restrict__ c, const 
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);
-  inv_block<LB, LE, TOP_MERGED>(x, c, lds, d.inv, LB, 0u, d, t3);
+  inv_block<LB, LE, TOP_MERGED>(x, c, lds, (const TwC *)d.inv, LB, 0u, d, t3);
 }
 
 // Residue-major polynomials: residue res of the launch uses descs[res % nmod];

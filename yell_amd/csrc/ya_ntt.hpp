@@ -79,16 +79,17 @@ __device__ __forceinline__ Tw32 load_tw(const Tw32 *p) {
   return Tw32{v.x, v.y};
 }
 
-// A twiddle table that the persistent batch kernel reads (every body, the
-// 2^15 one included): its wave-uniform loads go through the constant address
-// space (scalar loads).  In that kernel
-// the compiler cannot prove on its own that the table is not written before
-// a load (the previous work item's stores to c come first in the loop), so
-// through the global address space every lane would load the same pair on
-// the vector memory path.  The fused and block kernels keep the plain Tw
-// tables: there the compiler already picks scalar loads where they pay, and
-// more of them measured slower (they share the lgkm counter with the LDS
-// exchanges).
+// A twiddle table read through the constant address space: its wave-uniform
+// pairs come through scalar loads.  The persistent batch kernel reads every
+// body's twiddles so (there the compiler cannot prove on its own that the
+// table is not written before a load -- the previous work item's stores to c
+// come first in the loop -- and through the global address space every lane
+// would load the same pair on the vector memory path), and so does the fused
+// poly-multiplication kernel's dual form (2.2% faster at N = 2^14, L = 8 once
+// most of its exchanges waited for their wave only; 2.4% slower while every
+// exchange waited for the workgroup, the scalar loads sharing the lgkm counter
+// with the LDS exchanges).  The block, split and pair kernels keep the plain
+// Tw tables (the same change measured 1-4% slower there).
 struct TwC : Tw {};
 template <int WLO>
 __device__ __forceinline__ Tw load_tw_w(const Tw *p) {
