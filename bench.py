@@ -39,11 +39,11 @@ TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3283596.0, "k_polymul_tab<1
 # instruction per CU per cycle (4 SIMDs of 16 lanes, 4 cycles per wave64
 # instruction), 256 CUs at the 2.4 GHz peak engine clock.  VALU wave-
 # instructions per poly-multiplication from SQ_INSTS_VALU at --batch 512
-# (profiles/k_polymul_dual_n16384_l8.md): 8,106 per wave x 16 waves per
+# (profiles/k_polymul_dual_n16384_l8.md): 8,097 per wave x 16 waves per
 # residue x 8 residues.
 CUS, ENGINE_CLOCK_GHZ = 256, 2.4
 VALU_PEAK_G_PER_S = CUS * ENGINE_CLOCK_GHZ
-VALU_INSTS_PER_POLYMUL = {"k_polymul<14,4,dual>": 8106 * 16 * 8}
+VALU_INSTS_PER_POLYMUL = {"k_polymul<14,4,dual>": 8097 * 16 * 8}
 
 
 def log(*a):
