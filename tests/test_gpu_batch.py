@@ -271,11 +271,12 @@ def test_both_launch_forms_of_a_uniform_batch(form):
         capi.call("ya_batch_force_form", 0)
 
 
-@pytest.mark.parametrize("logn,L,count", [(14, 8, 128), (12, 3, 200), (10, 1, 600)])
+@pytest.mark.parametrize("logn,L,count", [(14, 8, 128), (14, 8, 320), (12, 3, 200), (10, 1, 600)])
 def test_contiguous_plan_strided_and_table(logn, L, count):
     """Back-to-back jobs of one shape: the direct-addressed path and the
     residue-table path give the same words, and both the oracle's (first and
-    last job), over two graph replays."""
+    last job), over two graph replays (2^14 x 8 x 320: ten waves of
+    workgroups, so the captured launches start staggered)."""
     device()
     n = 1 << logn
     w = n * L
