@@ -62,7 +62,7 @@ __device__ __forceinline__ void polymul32(uint32_t *__restrict__ c, const uint32
   gmem_load<W0, LE>(br, y, t);
   if constexpr (DUAL) {
     // both forward transforms in one pass sequence (shared twiddles)
-    fwd_block_dual<LB, LE>(x, y, lds, d.fwd, LB, 0u, d.p, t);
+    fwd_block_dual<LB, LE>(x, y, lds, (const TwC32 *)d.fwd, LB, 0u, d.p, t);
   } else {
     fwd_block<LB, LE, true>(x, nullptr, lds, d.fwd, LB, 0u, d.p, t);
     gmem_store<0, LE>(cr, x, t);  // park forward(a) at this thread's own words
@@ -75,7 +75,7 @@ __device__ __forceinline__ void polymul32(uint32_t *__restrict__ c, const uint32
 #pragma unroll
   for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
   const uint32_t t3 = phase_v(t);
-  inv_block<LB, LE, TOP_MERGED>(x, cr, lds, d.inv, LB, 0u, d, t3);
+  inv_block<LB, LE, TOP_MERGED>(x, cr, lds, (const TwC32 *)d.inv, LB, 0u, d, t3);
 }
 
 template <int LB, int LE, bool DUAL>

@@ -89,7 +89,9 @@ __device__ __forceinline__ Tw32 load_tw(const Tw32 *p) {
 // most of its exchanges waited for their wave only; 2.4% slower while every
 // exchange waited for the workgroup, the scalar loads sharing the lgkm counter
 // with the LDS exchanges).  The block, split and pair kernels keep the plain
-// Tw tables (the same change measured 1-4% slower there).
+// Tw tables (the same change measured 1-4% slower there).  TwC32 is the same
+// for the 32-bit moduli build's fused kernel (0.405 -> 0.395-0.401 ms per
+// 256 polys of 2^14 x 16 primes).
 struct TwC : Tw {};
 template <int WLO>
 __device__ __forceinline__ Tw load_tw_w(const Tw *p) {
@@ -98,6 +100,21 @@ __device__ __forceinline__ Tw load_tw_w(const Tw *p) {
 template <int WLO>
 __device__ __forceinline__ Tw32 load_tw_w(const Tw32 *p) {
   return load_tw(p);
+}
+struct TwC32 : Tw32 {};
+template <int WLO>
+__device__ __forceinline__ TwC32 load_tw_w(const TwC32 *p) {
+  TwC32 r;
+  if constexpr (WLO >= 6) {
+    const u32x2 v = *(const __attribute__((address_space(4))) u32x2 *)p;
+    r.w = v.x;
+    r.ws = v.y;
+  } else {
+    const u32x2 v = *(const __attribute__((address_space(1))) u32x2 *)p;
+    r.w = v.x;
+    r.ws = v.y;
+  }
+  return r;
 }
 template <int WLO>
 __device__ __forceinline__ TwC load_tw_w(const TwC *p) {
