@@ -285,12 +285,6 @@ int ya_batch_plan_run_graph(ya_batch_plan *plan, void *stream);
 enum { YA_BATCH_PERSISTENT = 1, YA_BATCH_GRID = 2, YA_BATCH_SPLIT = 3 };
 int ya_batch_plan_info(const ya_batch_plan *plan, size_t *items, size_t *residues,
                        size_t *workgroups, int *form);
-/* Persistent form: sub-groups of 2..8 waves that share a workgroup wait at a
- * barrier of their own in LDS, with a bounded wait; a wait that ran out
- * (which would leave wrong words) is counted in the plan's queue.  Returns
- * the count since the last call (synchronous; 0 in a working device) and
- * clears it. */
-int ya_batch_plan_faults(ya_batch_plan *plan, unsigned *count);
 /* Forces the launch form of plans created afterwards (0 = automatic; GRID
  * and SPLIT only where the batch allows them). */
 int ya_batch_force_form(int form);

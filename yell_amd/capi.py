@@ -140,7 +140,6 @@ _SIGS = {
                                  ctypes.POINTER(i32)]),
     "ya_batch_force_form": (i32, [i32]),
     "ya_autotune_mac": (i32, [sz, sz, sz, ctypes.POINTER(ctypes.c_double)]),
-    "ya_batch_plan_faults": (i32, [vp, ctypes.POINTER(ctypes.c_uint)]),
     "ya_batch_plan_destroy": (i32, [vp]),
     "ya_batch_polymul": (i32, [vp, sz, vp]),
     "ya_autotune": (i32, [i32, i32, ctypes.POINTER(ctypes.c_double)]),

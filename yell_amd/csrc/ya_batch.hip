@@ -27,51 +27,8 @@
 
 namespace ya {
 
-// Barriers of the sub-groups.  A sub-group of one wave or less needs no
-// barrier between its passes (WaveSync, ya_blocks.hpp).
-
-// A sub-group of 2..8 waves: a sense-reversing barrier in LDS, so that the
-// sub-groups of a workgroup run their passes independently (the workgroup's
-// barrier would hold them all in lockstep).  Lane 0 of each wave counts its
-// wave in; the last wave of the sub-group resets the count and flips the
-// flag, the others wait for the flip.  The wait is bounded (kSubSpinLimit
-// polls of about a hundred cycles each): a barrier that never completed
-// counts a fault in the queue (ya_batch_plan_faults) instead of hanging the
-// kernel.
-constexpr uint32_t kSubSpinLimit = 1u << 20;
-struct SubSync {
-  uint32_t *cnt, *flag;  // this sub-group's arrival count and completion flag (LDS)
-  uint32_t *faults;      // the queue's fault count (global)
-  uint32_t last;         // waves in the sub-group - 1
-  uint32_t sense;        // flag value while this wave's current barrier is open
-  bool lane0;
-  __device__ __forceinline__ void operator()() {
-    __builtin_amdgcn_s_waitcnt(0xC07F);  // this wave's LDS accesses are done
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    uint32_t old = 0;
-    if (lane0) old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    old = __builtin_amdgcn_readfirstlane(old);
-    if (old == last) {
-      if (lane0) {
-        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_store(flag, sense ^ 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-    } else {
-      uint32_t spin = 0;
-      while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == sense) {
-        if (++spin == kSubSpinLimit) {
-          if (lane0) atomicAdd(faults, 1u);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    sense ^= 1u;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  }
-  // an exchange inside each wave (kWaveLocal): the wave's own order only
-  __device__ __forceinline__ void local() const { WaveSync()(); }
-};
+// Barriers of the sub-groups (batch_small): a sub-group of one wave or less
+// needs no barrier between its passes (WaveSync, ya_blocks.hpp).
 
 // The fused poly-multiplication of one sub-group's residue (sub-group `sub`
 // of 2^(LB-LE) threads), its exchanges separated by `sync`.
@@ -121,32 +78,27 @@ __device__ __forceinline__ void batch_small_body(const BatchItem &it, const Batc
 }
 
 // Sub-grouped fused poly-multiplication of up to 2^(10+LE-LB) residues.
-// s_cnt / s_flag: the workgroup's sub-group barrier words (8 each, zeroed at
-// kernel start; every barrier leaves its count at zero).
+// A sub-group of whole waves orders its exchanges that cross its waves with
+// the workgroup's barrier: every sub-group of an item runs the same body (one
+// without a residue computes on the item's first and stores nothing), so all
+// of them reach the same barriers; the exchanges inside each wave need the
+// wave only (kWaveLocal).  (A barrier of each sub-group's own, in LDS, paid
+// while every exchange had a barrier; with only the crossing exchanges left
+// the workgroup's measured as fast or faster: 2^13 1.086 -> 1.067-1.072 ms on
+// the tuner's batch.)
 template <int LB, int LE>
 __device__ __forceinline__ void batch_small(const BatchItem &it, const BatchRes *__restrict__ res,
-                                            uint64_t *lds, uint32_t *s_cnt, uint32_t *s_flag, uint32_t *faults,
-                                            uint32_t t) {
+                                            uint64_t *lds, uint32_t t) {
   constexpr uint32_t TS = 1u << (LB - LE);  // threads per residue
   // a sub-group of whole waves has a wave-uniform index: take it from lane
   // 0's thread index (a scalar), so the LDS slice address is scalar too
   const uint32_t sub = TS >= (uint32_t)kBatchThreads ? 0u
                        : TS >= 64u        ? __builtin_amdgcn_readfirstlane(t) / TS
                                           : t / TS;
-  if constexpr (TS >= (uint32_t)kBatchThreads) {
+  if constexpr (TS > 64u) {
     batch_small_body<LB, LE>(it, res, lds, sub, t, WgSync());
-  } else if constexpr (TS <= 64u) {
-    batch_small_body<LB, LE>(it, res, lds, sub, t, WaveSync());
   } else {
-    static_assert(TS / 64u <= 8u, "sub-group barrier words");
-    SubSync sy;
-    sy.cnt = s_cnt + sub;
-    sy.flag = s_flag + sub;
-    sy.faults = faults;
-    sy.last = TS / 64u - 1u;
-    sy.sense = __builtin_amdgcn_readfirstlane(__hip_atomic_load(s_flag + sub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
-    sy.lane0 = (t & 63u) == 0u;
-    batch_small_body<LB, LE>(it, res, lds, sub, t, sy);
+    batch_small_body<LB, LE>(it, res, lds, sub, t, WaveSync());
   }
 }
 
@@ -258,11 +210,6 @@ k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
   // item i + 1 into the other slot, so the queue atomic's round trip overlaps
   // the work and one barrier per item remains
   __shared__ uint32_t s_item[2], s_seg, s_tried;
-  __shared__ uint32_t s_cnt[8], s_flag[8];  // sub-group barriers (SubSync)
-  if (threadIdx.x < 8) {
-    s_cnt[threadIdx.x] = 0;
-    s_flag[threadIdx.x] = 0;
-  }
   if (threadIdx.x == 0) {
     s_seg = blockIdx.x % kSegs;
     s_tried = 0;
@@ -282,21 +229,21 @@ k_batch_polymul(const BatchItem *__restrict__ items, BatchSegs segs,
     if (t == 0) s_item[(i + 1) & 1u] = claim_item(segs, q, s_seg, s_tried);
     const BatchItem it = items[k];
     switch (it.logn * 8 + it.le) {
-      case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 8 * 8 + 3: batch_small<8, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 8 * 8 + 4: batch_small<8, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 9 * 8 + 3: batch_small<9, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 9 * 8 + 4: batch_small<9, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 10 * 8 + 3: batch_small<10, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 10 * 8 + 4: batch_small<10, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 11 * 8 + 3: batch_small<11, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 11 * 8 + 4: batch_small<11, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 12 * 8 + 3: batch_small<12, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 12 * 8 + 4: batch_small<12, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
-      case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, s_cnt, s_flag, &q->faults, t); break;
+      case 7 * 8 + 3: batch_small<7, 3>(it, res, lds, t); break;
+      case 7 * 8 + 4: batch_small<7, 4>(it, res, lds, t); break;
+      case 8 * 8 + 3: batch_small<8, 3>(it, res, lds, t); break;
+      case 8 * 8 + 4: batch_small<8, 4>(it, res, lds, t); break;
+      case 9 * 8 + 3: batch_small<9, 3>(it, res, lds, t); break;
+      case 9 * 8 + 4: batch_small<9, 4>(it, res, lds, t); break;
+      case 10 * 8 + 3: batch_small<10, 3>(it, res, lds, t); break;
+      case 10 * 8 + 4: batch_small<10, 4>(it, res, lds, t); break;
+      case 11 * 8 + 3: batch_small<11, 3>(it, res, lds, t); break;
+      case 11 * 8 + 4: batch_small<11, 4>(it, res, lds, t); break;
+      case 12 * 8 + 3: batch_small<12, 3>(it, res, lds, t); break;
+      case 12 * 8 + 4: batch_small<12, 4>(it, res, lds, t); break;
+      case 13 * 8 + 3: batch_small<13, 3>(it, res, lds, t); break;
+      case 13 * 8 + 4: batch_small<13, 4>(it, res, lds, t); break;
+      case 14 * 8 + 4: batch_small<14, 4>(it, res, lds, t); break;
       case 15 * 8 + 4: batch_15(it, res, lds, scratch + ((size_t)blockIdx.x << 14), t); break;
       default: break;  // rejected by the host
     }
@@ -599,18 +546,6 @@ int ya_batch_plan_info(const ya_batch_plan *pl, size_t *items, size_t *residues,
                   : pl->form == YA_BATCH_SPLIT ? pl->nres << pl->split_k
                                                : pl->grid;
   if (form) *form = pl->form;
-  return YA_OK;
-}
-
-int ya_batch_plan_faults(ya_batch_plan *pl, unsigned *count) {
-  if (!pl || !count) return fail(YA_ERR_INVALID, "null argument");
-  *count = 0;
-  if (!pl->d_q) return YA_OK;
-  uint32_t n = 0;
-  hipError_t e = hipMemcpy(&n, &pl->d_q->faults, sizeof n, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && n) e = hipMemset(&pl->d_q->faults, 0, sizeof n);
-  if (e != hipSuccess) return hip_fail(e, "batch fault count");
-  *count = n;
   return YA_OK;
 }
 

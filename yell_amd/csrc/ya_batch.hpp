@@ -37,7 +37,6 @@ constexpr int kSegs = 8;
 struct BatchQueue {
   uint32_t next[kSegs];  // next item to claim in each segment (relative)
   uint32_t done;         // workgroups that have drained every segment
-  uint32_t faults;       // sub-group barriers that gave up waiting (never reset by the kernel)
 };
 
 struct BatchSegs {
