@@ -31,19 +31,22 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one poly-multiplication of the batched step moves, from the
 # FETCH_SIZE / WRITE_SIZE counter passes (calibrated against a 256 MiB copy)
 # in profiles/k_polymul_dual_n16384_l8.md, per launch of 512 poly-mults:
-# k_polymul (the direct-addressed kernel of a plan over back-to-back
-# polynomials) 1.1443e9 B read + 5.369e8 B written; k_polymul_tab (the
-# residue-table kernel) 1.1535e9 + 5.369e8.
-TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul<14,4,dual>": 3283596.0, "k_polymul_tab<14,4,dual>": 3301576.0}
+# k_polymul_loop (the direct-addressed looping kernel a plan over 512
+# back-to-back polynomials runs) 1.1365e9 B read + 5.369e8 B written;
+# k_polymul (one workgroup per residue) 1.1443e9 + 5.369e8; k_polymul_tab
+# (the residue-table kernel) 1.1535e9 + 5.369e8.
+TRAFFIC_BYTES_PER_POLYMUL = {"k_polymul_loop<14,4>": 3268262.0, "k_polymul<14,4,dual>": 3283596.0,
+                             "k_polymul_tab<14,4,dual>": 3301576.0}
 # The binding resource of the fused kernel is VALU issue: one wave64 VALU
 # instruction per CU per cycle (4 SIMDs of 16 lanes, 4 cycles per wave64
 # instruction), 256 CUs at the 2.4 GHz peak engine clock.  VALU wave-
 # instructions per poly-multiplication from SQ_INSTS_VALU at --batch 512
-# (profiles/k_polymul_dual_n16384_l8.md): 8,097 per wave x 16 waves per
-# residue x 8 residues.
+# (profiles/k_polymul_dual_n16384_l8.md): the looping kernel 525,484,032 per
+# launch = 128,292 per residue x 8 residues; one workgroup per residue
+# 8,097 per wave x 16 waves per residue x 8 residues.
 CUS, ENGINE_CLOCK_GHZ = 256, 2.4
 VALU_PEAK_G_PER_S = CUS * ENGINE_CLOCK_GHZ
-VALU_INSTS_PER_POLYMUL = {"k_polymul<14,4,dual>": 8097 * 16 * 8}
+VALU_INSTS_PER_POLYMUL = {"k_polymul_loop<14,4>": 128292 * 8, "k_polymul<14,4,dual>": 8097 * 16 * 8}
 
 
 def log(*a):
@@ -692,8 +695,10 @@ def main():
     capi.call("ya_batch_plan_info", plan, None, None, None, ctypes.byref(form))
     strided = ctypes.c_int()
     capi.call("ya_batch_plan_is_strided", plan, ctypes.byref(strided))
-    step_kernel = ("k_polymul<14,4,dual>" if strided.value else "k_polymul_tab<14,4,dual>") \
-        if form.value == 2 else "k_batch_polymul"
+    # a strided plan runs the looping kernel when the batch gives every CU's
+    # one 2^14 workgroup at least two residues
+    step_kernel = (("k_polymul_loop<14,4>" if B * L >= 2 * CUS else "k_polymul<14,4,dual>") if strided.value
+                   else "k_polymul_tab<14,4,dual>") if form.value == 2 else "k_batch_polymul"
 
     def batched():
         capi.call("ya_batch_plan_run_graph", plan, stream)

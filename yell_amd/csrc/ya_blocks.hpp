@@ -201,10 +201,33 @@ struct InvPass {
   static constexpr int wlo = last ? top : (hi - LE > 0 ? hi - LE : 0);
 };
 
-template <int LB, int LE, int TOP, class W, class TW, class D, class S = WgSync>
+// The first pass after pass 0 whose window starts at bit 6 or above: its
+// threads' twiddles are wave-uniform (scalar loads), so from there to the
+// end the inverse issues no vector load of its own.  inv_block runs its
+// caller's hook there, before the pass's exchange load: vector loads the hook
+// issues (the next residue's operand) are not waited for by the inverse.
+template <int LB, int LE>
+struct InvHookPass {
+  static constexpr int find() {
+    const int np = (LB + LE - 1) / LE, top = LB - LE > 0 ? LB - LE : 0;
+    for (int q = 1; q < np; ++q) {  // InvPass<LB, LE, q>::wlo
+      const int hi = q * LE + LE < top ? q * LE + LE : top;
+      const int wlo = q + 1 == np ? top : (hi - LE > 0 ? hi - LE : 0);
+      if (wlo >= 6) return q;
+    }
+    return np - 1;
+  }
+  static constexpr int value = find();
+};
+
+struct NoHook {
+  __device__ __forceinline__ void operator()() const {}
+};
+
+template <int LB, int LE, int TOP, class W, class TW, class D, class S = WgSync, class H = NoHook>
 __device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::type *__restrict__ dst,
                                           W *lds, const TW *__restrict__ tw, uint32_t nlog, uint32_t boff,
-                                          const D &d, uint32_t t, S &&sync = S()) {
+                                          const D &d, uint32_t t, S &&sync = S(), H &&hook = H()) {
   constexpr int NP = (LB + LE - 1) / LE;
   static_for<0, NP>([&](auto QC) {
     constexpr int Q = decltype(QC)::value;
@@ -212,6 +235,7 @@ __device__ __forceinline__ void inv_block(W (&x)[1 << LE], typename Ident<W>::ty
     constexpr int LO = InvPass<LB, LE, Q>::lo;
     constexpr int HI = InvPass<LB, LE, Q>::hi;
     constexpr int WLO = InvPass<LB, LE, Q>::wlo;
+    if constexpr (Q == InvHookPass<LB, LE>::value) hook();
     if constexpr (Q > 0) lds_load<WLO, LE>(lds, x, t);
     const uint32_t gbase = boff | ((t >> WLO) << (WLO + LE));
     if constexpr (LASTQ && TOP == TOP_MERGED) {

@@ -68,6 +68,62 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
                                 threadIdx.x);
 }
 
+// Looping form of the fused 2^LB kernel for launches of many residues, one
+// workgroup per CU: workgroup w takes residues w, w + G, w + 2G, ... of the
+// prime-major order (G = the grid, a multiple of 8, so all of them lie in
+// w's XCD range), and the next residue's b operand is loaded into y, which
+// the inverse leaves free, from the inverse's first pass with wave-uniform
+// twiddles on (InvHookPass): those loads overlap the inverse's last passes
+// and stores instead of following them.  The same words as k_polymul.
+template <int LB, int LE>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_polymul_loop(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
+               const uint64_t *__restrict__ b, const ModDesc *__restrict__ descs, uint32_t nmod,
+               uint32_t nres, uint32_t stagger) {
+  static_assert(InvHookPass<LB, LE>::value > 0, "the next operand's load must follow the product");
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  stagger_start(stagger);
+  constexpr int W0 = FirstWindow<LB, LE>::value;
+  const uint32_t g = gridDim.x, count = nres / nmod;
+  // residue v of the prime-major order: prime cm = v / count, polynomial
+  // v % count, stored at residue (v % count) * nmod + cm (wave-uniform)
+  uint32_t v = blockIdx.x;
+  uint32_t u = xcd_order(v, nres);
+  uint32_t cm = __builtin_amdgcn_readfirstlane(u / count);
+  uint32_t res = __builtin_amdgcn_readfirstlane((u % count) * nmod + cm);
+  uint64_t x[1 << LE], y[1 << LE];
+  uint32_t t = threadIdx.x;
+  gmem_load<W0, LE>(uniform_ptr(a + ((size_t)res << LB)), x, t);
+  gmem_load<W0, LE>(uniform_ptr(b + ((size_t)res << LB)), y, t);
+  for (;;) {
+    const ModDesc d = descs[cm];
+    // the previous residue's inverse may still read the LDS in other waves
+    fwd_block_dual<LB, LE, true>(x, y, lds, (const TwC *)d.fwd, LB, 0u, d.p, t);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+    const uint32_t vn = v + g;
+    const bool more = vn < nres;
+    uint32_t cmn = 0, resn = 0;
+    if (more) {
+      const uint32_t un = xcd_order(vn, nres);
+      cmn = __builtin_amdgcn_readfirstlane(un / count);
+      resn = __builtin_amdgcn_readfirstlane((un % count) * nmod + cmn);
+    }
+    const uint32_t t3 = phase_v(threadIdx.x);
+    inv_block<LB, LE, TOP_MERGED>(x, uniform_ptr(c + ((size_t)res << LB)), lds, (const TwC *)d.inv, LB, 0u, d,
+                                  t3, WgSync(), [&] {
+                                    if (more) gmem_load<W0, LE>(uniform_ptr(b + ((size_t)resn << LB)), y,
+                                                                phase_v(threadIdx.x));
+                                  });
+    if (!more) break;
+    t = phase_v(threadIdx.x);
+    gmem_load<W0, LE>(uniform_ptr(a + ((size_t)resn << LB)), x, t);
+    v = vn;
+    cm = cmn;
+    res = resn;
+  }
+}
+
 // One residue per workgroup from a residue table (a batch plan of one degree
 // class); the table is ordered by prime, and the XCD ranges follow it.
 template <int LB, int LE, bool DUAL>
@@ -148,10 +204,26 @@ uint32_t stagger_word(int steps, int per_cu, size_t nres) {
   return (uint32_t)(first << 8) | (uint32_t)(steps > 255 ? 255 : steps);
 }
 
+// The looping kernel is instantiated for the one shape of the batched
+// 2^14 step; it runs launches of at least two residues per workgroup.
+template <int LB, int LE>
+constexpr bool kLoopShape = LB == 14 && LE == 4;
+
 template <int LB, int LE>
 hipError_t polymul_lb(uint64_t *c, const uint64_t *a, const uint64_t *b, const ModDesc *descs,
                       size_t nres, size_t nmod, NttShape shape, hipStream_t st) {
   constexpr unsigned kThreads = 1u << (LB - LE);
+  if constexpr (kLoopShape<LB, LE>) {
+    if (shape.dual) {
+      static const int per_cu = resident_per_cu(k_polymul_loop<LB, LE>, kThreads);
+      const size_t grid = ((size_t)current_cus() * (size_t)per_cu) & ~(size_t)7;
+      if (grid >= 8 && nres >= 2 * grid && nres <= 0x7fffffff) {
+        hipLaunchKernelGGL((k_polymul_loop<LB, LE>), dim3((unsigned)grid), dim3(kThreads), 0, st, c, a, b, descs,
+                           (uint32_t)nmod, (uint32_t)nres, stagger_word(shape.stagger, per_cu, nres));
+        return hipGetLastError();
+      }
+    }
+  }
   if constexpr (kDualShape<LB, LE>) {
     if (shape.dual) {
       static const int per_cu = resident_per_cu(k_polymul<LB, LE, true>, kThreads);
