@@ -766,7 +766,8 @@ def main():
                   "launches_per_step": B},
         "batched_over_plain": value / plain_value,
         "grid_form": {"value": B / (ms_grid / 1e3), "ms_per_step": ms_grid,
-                      "kernel": "k_polymul<14,4>, one workgroup per residue"},
+                      "kernel": ("k_polymul_loop<14,4>, one workgroup per CU" if B * L >= 2 * CUS
+                                 else "k_polymul<14,4>, one workgroup per residue")},
         "roofline": roofline(step_kernel, ms_step, B, achieved, bytes_per),
         "modmul_rate": {"modmuls_per_polymul": mm,
                         "achieved_T_per_s": mm * B / (ms_step / 1e3) / 1e12},
