@@ -64,9 +64,9 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g) {
 
 // ---------------------------------------------------------------------------
 // Barriers between the passes of a block transform.  WgSync is the
-// workgroup's barrier; a transform whose threads are a part of a larger
-// workgroup (the persistent batch kernel's sub-groups) passes a barrier of
-// its own threads instead (ya_batch.hip).  Each also has local(), enough for
+// workgroup's barrier; WaveSync orders one wave's LDS accesses, all a
+// transform of one wave or less needs (the persistent batch kernel's
+// sub-groups of that size, ya_batch.hip).  Each also has local(), enough for
 // an exchange in which every wave reads back only words it wrote itself
 // (kWaveLocal below): a wave's LDS accesses are performed in issue order, so
 // the wave waits for its own LDS operations and the compiler is kept from
