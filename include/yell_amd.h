@@ -328,7 +328,7 @@ int ya_autotune(int logn_lo, int logn_hi, double *report);
  * degree 2^logn and, at logn = 15,
  * of the batched (persistent) kernel: in a launch of at least eight waves of
  * workgroups, the first wave's workgroups wait (group mod 16) steps before
- * they start, so the CUs' operand loads do not all fall together; default 3
+ * they start, so the CUs' operand loads do not all fall together; default 2
  * at 2^14, 6 at 15, 0 elsewhere.  No kind changes a result.
  * ya_tune_get returns the entry's value, or -1. */
 int ya_tune_get(int kind, int logn);

@@ -91,11 +91,11 @@ def test_tune_table_defaults_and_bounds():
     staggered start (kind 5); a refused entry leaves the table unchanged."""
     lib = capi.load()
     assert lib.ya_tune_get(4, 16) == 12 and lib.ya_tune_get(4, 18) == 13 and lib.ya_tune_get(4, 19) == 14
-    assert lib.ya_tune_get(5, 14) == 3 and lib.ya_tune_get(5, 15) == 6 and lib.ya_tune_get(5, 13) == 0
+    assert lib.ya_tune_get(5, 14) == 2 and lib.ya_tune_get(5, 15) == 6 and lib.ya_tune_get(5, 13) == 0
     assert lib.ya_tune_set(4, 18, 12) != 0 and lib.ya_tune_set(4, 19, 13) != 0 and lib.ya_tune_set(4, 16, 11) != 0
     assert lib.ya_tune_get(4, 18) == 13
     assert lib.ya_tune_set(5, 14, 17) != 0 and lib.ya_tune_set(5, 14, -1) != 0
-    assert lib.ya_tune_get(5, 14) == 3
+    assert lib.ya_tune_get(5, 14) == 2
     assert lib.ya_tune_set(5, 12, 16) == 0 and lib.ya_tune_get(5, 12) == 16
     assert lib.ya_tune_set(5, 12, 0) == 0 and lib.ya_tune_get(5, 12) == 0
     assert lib.ya_tune_get(6, 14) == -1

@@ -27,10 +27,11 @@ static void init_defaults() {
     // at 2^18, 2^14 at 2^19 (at most 5 top layers); up to 2^14 the ring is
     // one block
     for (int l = 0; l <= kMaxLogN; ++l) g_mac_lb[l].store(l <= 14 ? 14 : l <= 17 ? 12 : l == 18 ? 13 : 14);
-    // measured faster where one workgroup fills a CU: the fused kernel at
-    // 2^14 (best at 1..3 steps) and the persistent kernel (its 2^15 entry;
-    // best at 6 steps on the uneven 1024-job batch)
-    for (int l = 0; l <= kMaxLogN; ++l) g_stagger[l].store(l == 14 ? 3 : l == 15 ? 6 : 0);
+    // measured faster where one workgroup fills a CU: the fused kernels at
+    // 2^14 (one workgroup per residue: best at 1..3 steps; the looping
+    // kernel: 0..2 at 512 polys, 2..3 at 1,024) and the persistent kernel
+    // (its 2^15 entry; best at 6 steps on the uneven 1024-job batch)
+    for (int l = 0; l <= kMaxLogN; ++l) g_stagger[l].store(l == 14 ? 2 : l == 15 ? 6 : 0);
   });
 }
 
