@@ -69,9 +69,9 @@ k_polymul(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
 }
 
 // Looping form of the fused 2^LB kernel for launches of many residues, one
-// workgroup per CU: workgroup w takes residues w, w + G, w + 2G, ... of the
-// prime-major order (G = the grid, a multiple of 8, so all of them lie in
-// w's XCD range), and the next residue's b operand is loaded into y, which
+// workgroup per CU: workgroup w takes launch indices w, w + G, w + 2G, ...
+// (G = the grid, a multiple of 8, so all of them map into w's XCD range of
+// the prime-major order), and the next residue's b operand is loaded into y, which
 // the inverse leaves free, from the inverse's first pass with wave-uniform
 // twiddles on (InvHookPass): those loads overlap the inverse's last passes
 // and stores instead of following them.  The same words as k_polymul.
@@ -85,8 +85,9 @@ k_polymul_loop(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
   stagger_start(stagger);
   constexpr int W0 = FirstWindow<LB, LE>::value;
   const uint32_t g = gridDim.x, count = nres / nmod;
-  // residue v of the prime-major order: prime cm = v / count, polynomial
-  // v % count, stored at residue (v % count) * nmod + cm (wave-uniform)
+  // launch index v -> position u of the prime-major order: prime cm =
+  // u / count, polynomial u % count, stored at residue (u % count) * nmod + cm
+  // (wave-uniform)
   uint32_t v = blockIdx.x;
   uint32_t u = xcd_order(v, nres);
   uint32_t cm = __builtin_amdgcn_readfirstlane(u / count);
