@@ -697,8 +697,10 @@ def main():
     capi.call("ya_batch_plan_is_strided", plan, ctypes.byref(strided))
     # a strided plan runs the looping kernel when the batch gives every CU's
     # one 2^14 workgroup at least two residues
-    step_kernel = (("k_polymul_loop<14,4>" if B * L >= 2 * CUS else "k_polymul<14,4,dual>") if strided.value
-                   else "k_polymul_tab<14,4,dual>") if form.value == 2 else "k_batch_polymul"
+    looping = B * L >= 2 * CUS
+    step_kernel = (("k_polymul_loop<14,4>" if looping else "k_polymul<14,4,dual>") if strided.value
+                   else ("k_polymul_tab_loop<14,4>" if looping else "k_polymul_tab<14,4,dual>")) \
+        if form.value == 2 else "k_batch_polymul"
 
     def batched():
         capi.call("ya_batch_plan_run_graph", plan, stream)

@@ -125,6 +125,44 @@ k_polymul_loop(uint64_t *__restrict__ c, const uint64_t *__restrict__ a,
   }
 }
 
+// The looping form over a residue table (a batch plan of one degree class
+// over arbitrary job pointers): launch index v reads entry xcd_order(v) of
+// the prime-ordered table; otherwise as k_polymul_loop.
+template <int LB, int LE>
+__global__ void __launch_bounds__(1 << (LB - LE))
+k_polymul_tab_loop(const BatchRes *__restrict__ tab, uint32_t nres, uint32_t stagger) {
+  static_assert(InvHookPass<LB, LE>::value > 0, "the next operand's load must follow the product");
+  __shared__ uint64_t lds[LdsWords<LB>::value];
+  stagger_start(stagger);
+  constexpr int W0 = FirstWindow<LB, LE>::value;
+  const uint32_t g = gridDim.x;
+  uint32_t v = blockIdx.x;
+  BatchRes r = tab[xcd_order(v, nres)];
+  uint64_t x[1 << LE], y[1 << LE];
+  uint32_t t = threadIdx.x;
+  gmem_load<W0, LE>(uniform_ptr(r.a), x, t);
+  gmem_load<W0, LE>(uniform_ptr(r.b), y, t);
+  for (;;) {
+    const ModDesc d = load_desc(r.d);
+    fwd_block_dual<LB, LE, true>(x, y, lds, (const TwC *)d.fwd, LB, 0u, d.p, t);
+#pragma unroll
+    for (int e = 0; e < (1 << LE); ++e) x[e] = mulmod_barrett(y[e], x[e], d.p, d.pn);
+    const uint32_t vn = v + g;
+    const bool more = vn < nres;
+    BatchRes rn = r;
+    if (more) rn = tab[xcd_order(vn, nres)];
+    const uint32_t t3 = phase_v(threadIdx.x);
+    inv_block<LB, LE, TOP_MERGED>(x, uniform_ptr(r.c), lds, (const TwC *)d.inv, LB, 0u, d, t3, WgSync(), [&] {
+      if (more) gmem_load<W0, LE>(uniform_ptr(rn.b), y, phase_v(threadIdx.x));
+    });
+    if (!more) break;
+    t = phase_v(threadIdx.x);
+    gmem_load<W0, LE>(uniform_ptr(rn.a), x, t);
+    v = vn;
+    r = rn;
+  }
+}
+
 // One residue per workgroup from a residue table (a batch plan of one degree
 // class); the table is ordered by prime, and the XCD ranges follow it.
 template <int LB, int LE, bool DUAL>
@@ -242,6 +280,17 @@ hipError_t polymul_lb(uint64_t *c, const uint64_t *a, const uint64_t *b, const M
 template <int LB, int LE>
 hipError_t polymul_tab_lb(const BatchRes *res, size_t nres, NttShape shape, hipStream_t st) {
   constexpr unsigned kThreads = 1u << (LB - LE);
+  if constexpr (kLoopShape<LB, LE>) {
+    if (shape.dual) {
+      static const int per_cu = resident_per_cu(k_polymul_tab_loop<LB, LE>, kThreads);
+      const size_t grid = ((size_t)current_cus() * (size_t)per_cu) & ~(size_t)7;
+      if (grid >= 8 && nres >= 2 * grid) {
+        hipLaunchKernelGGL((k_polymul_tab_loop<LB, LE>), dim3((unsigned)grid), dim3(kThreads), 0, st, res,
+                           (uint32_t)nres, stagger_word(shape.stagger, per_cu, nres));
+        return hipGetLastError();
+      }
+    }
+  }
   if constexpr (kDualShape<LB, LE>) {
     if (shape.dual) {
       static const int per_cu = resident_per_cu(k_polymul_tab<LB, LE, true>, kThreads);
